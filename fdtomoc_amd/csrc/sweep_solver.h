@@ -1,0 +1,605 @@
+// sweep_solver.h -- one workgroup solves one travel-time table: the whole
+// expanding-box sweep of FDtomoC src/sphfd.c (first pass plus head-wave reverse
+// passes, sphfd.c:1008-5678), written bulk-synchronously against a small
+// executor interface X:
+//
+//   int  X::tid() const, X::nthr() const   thread index / count in the group
+//   void X::sync()                         group barrier with memory ordering
+//   int  X::atomic_add(int *p, int v)      returns the old value
+//   int  X::atomic_exch(int *p, int v)     returns the old value
+//   unsigned X::atomic_or(unsigned*,unsigned), X::atomic_and(...)
+//
+// On the GPU X wraps threadIdx/__syncthreads/atomics (sweep_kernel.hip).
+//
+// Exactness.  Inside a face the reference updates nodes one by one in
+// ascending (inner-layer time, fill index) order; a node reads the new values
+// of in-face neighbours that precede it and the pre-face values of those that
+// follow it.  Two schedules reproduce that bit for bit:
+//
+//   LEVEL  (faces holding uncomputed nodes, i.e. the expanding first pass)
+//     A stable radix sort gives every node its rank.  A node becomes ready when
+//     all its preceding in-face neighbours are final (pending counters).  Ready
+//     nodes never neighbour each other, so a round updates them in parallel.
+//     fdsph2d also reads t[0]/t[4] left by whichever node ran before it in the
+//     global order (sphfd.c:6116-6121); a node that runs a 2D trial without
+//     having set them itself walks back through the rank order to the last node
+//     that did, and waits a round if that node is not final yet.
+//
+//   CHANGE (faces whose nodes are all computed, i.e. reverse passes)
+//     Every node is evaluated once from a snapshot of the pre-face values;
+//     afterwards a node is re-evaluated only when a preceding neighbour changed.
+//     This converges to the sequential result (it is the same DAG evaluated in
+//     topological order) and needs far fewer serial rounds, because reverse
+//     passes change few nodes.  If any node would need the carried t[0]/t[4],
+//     the face is restored from the snapshot and redone with LEVEL.
+#pragma once
+
+#include "sweep_core.h"
+
+namespace fdsweep {
+
+// --------------------------------------------------------------- job layout
+
+// One table, prepared by the host (geometry and source initialisation follow
+// sphfd.c:415-509 and 696-975, evaluated with host libm).
+struct TableJob {
+    int nx, ny, nz;
+    double h, dq, df, y0, z0r;
+    const double *sinq;        // ny + 2 values, sin(qc(j)) at [j + 1]
+    const float *slow;         // slowness at node (0,0,0) of this table
+    long long s_sy, s_sz;      // slowness strides (x stride is 1)
+    float *time;               // nx*ny*nz output grid, reference layout
+    int init_mode;             // 0: fill 1e10, cube times s000*dist; 1: grid preset; 2: fill, explicit values
+    long long src_sidx;        // slowness index of the source node (s000, sphfd.c:730)
+    int ninit;
+    const int *init_idx;
+    const double *init_dist;   // mode 0: straight-ray distances (sphfd.c:727-749)
+    const float *init_val;     // mode 2: box-file values (sphfd.c:763)
+    int box[3][2];             // x1 x2 / y1 y2 / z1 z2 after source setup
+    int dflag[6];              // dx1 dx2 dy1 dy2 dz1 dz2
+    int grow[6];               // kminus kplus iplus iminus jminus jplus
+    int reverse, headpref, maxrad, radius0, srcwall;
+    int *stats;                // kStat* counters (device)
+};
+
+enum {
+    kStatPasses = 0, kStatRestarts, kStatFacesLevel, kStatFacesChange, kStatRoundsLevel,
+    kStatRoundsChange, kStatFallbacks, kStatEvals, kStatDefers, kStatError, kStatLastWall,
+    kStatHeadw0,  // 7 entries (slots 0..6) of the first pass
+    kStatCount = kStatHeadw0 + 7
+};
+
+// Per-slot scratch, sized for the largest face of any table in the launch.
+struct Scratch {
+    float *O;            // pre-face values of the face nodes (fill order)
+    unsigned *key;       // ordered key of the inner-layer time (fill order)
+    unsigned *key2;      // sort ping-pong
+    int *val, *val2;     // sort ping-pong payload (fill index)
+    int *rank;           // fill index -> rank
+    int *ord;            // rank -> fill index
+    int *pend;           // preceding neighbours not yet final
+    int *stat;           // per rank: 0 not final, 1 set t0/t4, 2 passed them on
+    float *tout0, *tout4;  // per rank (LEVEL) or fill index (CHANGE)
+    float *rt0, *rt4;    // per fill index, results of the current round
+    int *rstat;          // per fill index, result status of the current round
+    int *list0, *list1, *list2;  // work lists
+    int *stamp;          // CHANGE: last round a node was queued
+    unsigned char *flag; // CHANGE: bit0 head wave, bit1 used carried t0/t4, bit2 sets t0/t4
+};
+
+constexpr int kMaxThreads = 1024;
+
+struct Shared {
+    int n[3];            // list sizes
+    int processed;
+    int headw[7];
+    unsigned kor, kand;
+    int flag;            // face flags: bit0 uncomputed node present, bit1 NaN key, bit2 fall back
+    int best;            // CHANGE: fill index of the last node in face order
+    float cin0, cin4;    // carried t[0]/t[4] entering the face
+    int job;
+    int hist[16 * 256];  // radix histograms, digit-major, up to 256 threads
+    int part[kMaxThreads];
+};
+
+// ---------------------------------------------------------------- helpers
+
+FD_INLINE unsigned order_key(float v) {
+    unsigned u;
+    __builtin_memcpy(&u, &v, 4);
+    if (u == 0x80000000u) u = 0;   // -0 sorts with +0 (the reference compares floats)
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+FD_INLINE bool key_less(unsigned ka, int a, unsigned kb, int b) { return ka < kb || (ka == kb && a < b); }
+
+struct FaceGeom {
+    int face, F, wa, wb, a0, b0, fixed;
+};
+
+template <int F>
+FD_INLINE void face_node(const FaceGeom &fg, int p, int &i, int &j, int &k, int &a, int &b) {
+    a = p % fg.wa;
+    b = p / fg.wa;
+    const int A = fg.a0 + a, B = fg.b0 + b;
+    if (F == TOP || F == BOTTOM) { i = A; j = B; k = fg.fixed; }
+    else if (F == EAST || F == WEST) { i = fg.fixed; j = A; k = B; }
+    else { i = A; j = fg.fixed; k = B; }
+}
+
+// Offsets of an in-face neighbour along the face's (a, b) axes.
+template <int F>
+FD_INLINE void inface(int dx, int dy, int dz, int &dn, int &da, int &db) {
+    if (F == TOP || F == BOTTOM) { dn = dz; da = dx; db = dy; }
+    else if (F == EAST || F == WEST) { dn = dx; da = dy; db = dz; }
+    else { dn = dy; da = dx; db = dz; }
+}
+
+// Accessor: CHANGE mode reads the pre-face snapshot for in-face neighbours that
+// follow the node; LEVEL mode reads the grid (followers are untouched there).
+template <int F, bool CHG>
+struct Acc {
+    const float *time;
+    const float *slow;
+    long long node, snode;
+    long long nx, nxy, s_sy, s_sz;
+    int a, b, p;
+    unsigned kp;
+    const FaceGeom *fg;
+    const float *O;
+    const unsigned *key;
+    FD_INLINE float t(int dx, int dy, int dz) const {
+        if (CHG) {
+            int dn, da, db;
+            inface<F>(dx, dy, dz, dn, da, db);
+            if (dn == 0) {
+                const int aa = a + da, bb = b + db;
+                if (aa >= 0 && bb >= 0 && aa < fg->wa && bb < fg->wb) {
+                    const int q = bb * fg->wa + aa;
+                    if (!key_less(key[q], q, kp, p)) return O[q];
+                }
+            }
+        }
+        return time[node + dx + dy * nx + dz * nxy];
+    }
+    FD_INLINE float s(int dx, int dy, int dz) const { return slow[snode + dx + dy * s_sy + dz * s_sz]; }
+};
+
+// ------------------------------------------------------------ radix sort
+
+// Stable LSD sort of F (key, val) pairs by key, 4-bit digits, skipping digits
+// on which all keys agree.  Returns true if the result is in (key2, val2).
+template <class X>
+FD_HD bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int *v1, int F) {
+    const int T = x.nthr(), tid = x.tid();
+    const int chunk = (F + T - 1) / T;
+    const int lo = tid * chunk < F ? tid * chunk : F;
+    const int hi = lo + chunk < F ? lo + chunk : F;
+    bool swapped = false;
+    for (int d = 0; d < 8; d++) {
+        const unsigned vary = (sh.kor ^ sh.kand) >> (4 * d);
+        if ((vary & 15u) == 0) continue;
+        int cnt[16];
+        for (int q = 0; q < 16; q++) cnt[q] = 0;
+        for (int e = lo; e < hi; e++) cnt[(k0[e] >> (4 * d)) & 15u]++;
+        for (int q = 0; q < 16; q++) sh.hist[q * T + tid] = cnt[q];
+        x.sync();
+        // exclusive scan over 16*T counters in digit-major order: each thread
+        // owns 16 consecutive counters
+        int s = 0;
+        for (int q = 0; q < 16; q++) s += sh.hist[tid * 16 + q];
+        sh.part[tid] = s;
+        x.sync();
+        if (tid == 0) {
+            int run = 0;
+            for (int q = 0; q < T; q++) {
+                const int v = sh.part[q];
+                sh.part[q] = run;
+                run += v;
+            }
+        }
+        x.sync();
+        int run = sh.part[tid];
+        for (int q = 0; q < 16; q++) {
+            const int v = sh.hist[tid * 16 + q];
+            sh.hist[tid * 16 + q] = run;
+            run += v;
+        }
+        x.sync();
+        int off[16];
+        for (int q = 0; q < 16; q++) off[q] = sh.hist[q * T + tid];
+        for (int e = lo; e < hi; e++) {
+            const unsigned kk = k0[e];
+            const int dg = (kk >> (4 * d)) & 15u;
+            const int pos = off[dg]++;
+            k1[pos] = kk;
+            v1[pos] = v0[e];
+        }
+        x.sync();
+        unsigned *tk = k0; k0 = k1; k1 = tk;
+        int *tv = v0; v0 = v1; v1 = tv;
+        swapped = !swapped;
+    }
+    return swapped;
+}
+
+// ------------------------------------------------------------- one face
+
+template <int F, class X>
+FD_HD void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
+                      const int *ord, int *stats) {
+    const int T = x.nthr(), tid = x.tid();
+    const int Fn = fg.F;
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    // ranks and pending counters
+    for (int r = tid; r < Fn; r += T) W.rank[ord[r]] = r;
+    x.sync();
+    if (tid == 0) { sh.n[0] = 0; sh.n[1] = 0; sh.n[2] = 0; sh.processed = 0; }
+    x.sync();
+    for (int p = tid; p < Fn; p += T) {
+        const int a = p % fg.wa, b = p / fg.wa, rp = W.rank[p];
+        int c = 0;
+        for (int db = -1; db <= 1; db++)
+            for (int da = -1; da <= 1; da++) {
+                if (!da && !db) continue;
+                const int aa = a + da, bb = b + db;
+                if (aa < 0 || bb < 0 || aa >= fg.wa || bb >= fg.wb) continue;
+                c += W.rank[bb * fg.wa + aa] < rp;
+            }
+        W.pend[p] = c;
+        W.stat[rp] = 0;
+        if (c == 0) W.list0[x.atomic_add(&sh.n[0], 1)] = p;
+    }
+    x.sync();
+    int *cur = W.list0, *nxt = W.list1, *done = W.list2;
+    int icur = 0, inxt = 1;
+    while (true) {
+        const int ncur = sh.n[icur];
+        if (ncur == 0) break;
+        // phase B: evaluate ready nodes (statuses published by earlier rounds only)
+        for (int it = tid; it < ncur; it += T) {
+            const int p = cur[it];
+            int i, j, k, a, b;
+            face_node<F>(fg, p, i, j, k, a, b);
+            Acc<F, false> A;
+            A.time = J.time; A.slow = J.slow;
+            A.node = (long long)k * nxy + (long long)j * nx + i;
+            A.snode = (long long)k * J.s_sz + (long long)j * J.s_sy + i;
+            A.nx = nx; A.nxy = nxy; A.s_sy = J.s_sy; A.s_sz = J.s_sz;
+            A.a = a; A.b = b; A.p = p; A.kp = 0; A.fg = &fg; A.O = W.O; A.key = W.key;
+            const float told = W.O[p];
+            float tin0 = sh.cin0, tin4 = sh.cin4;
+            if (needs_carry<F>(A, G, i, j, k, told)) {
+                int r = W.rank[p] - 1;
+                while (r >= 0 && W.stat[r] == 2) r--;
+                if (r >= 0) {
+                    if (W.stat[r] == 0) {   // the node that sets them is not final yet
+                        nxt[x.atomic_add(&sh.n[inxt], 1)] = p;
+                        x.atomic_add(&stats[kStatDefers], 1);
+                        continue;
+                    }
+                    tin0 = W.tout0[r];
+                    tin4 = W.tout4[r];
+                }
+            }
+            NodeOut o;
+            eval_node<F>(A, G, i, j, k, told, J.slow[A.snode], tin0, tin4, o);
+            if (o.stored) {
+                J.time[A.node] = o.tnew;
+                if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
+            }
+            W.rstat[p] = o.sets ? 1 : 2;
+            W.rt0[p] = o.tout0;
+            W.rt4[p] = o.tout4;
+            done[x.atomic_add(&sh.n[2], 1)] = p;
+        }
+        x.sync();
+        // phase C: publish statuses and release successors
+        const int ndone = sh.n[2];
+        for (int it = tid; it < ndone; it += T) {
+            const int p = done[it];
+            const int rp = W.rank[p];
+            W.tout0[rp] = W.rt0[p];
+            W.tout4[rp] = W.rt4[p];
+            W.stat[rp] = W.rstat[p];
+            const int a = p % fg.wa, b = p / fg.wa;
+            for (int db = -1; db <= 1; db++)
+                for (int da = -1; da <= 1; da++) {
+                    if (!da && !db) continue;
+                    const int aa = a + da, bb = b + db;
+                    if (aa < 0 || bb < 0 || aa >= fg.wa || bb >= fg.wb) continue;
+                    const int q = bb * fg.wa + aa;
+                    if (W.rank[q] > rp && x.atomic_add(&W.pend[q], -1) == 1)
+                        nxt[x.atomic_add(&sh.n[inxt], 1)] = q;
+                }
+        }
+        if (tid == 0) {
+            stats[kStatRoundsLevel] += 1;
+            stats[kStatEvals] += ndone;
+            sh.processed += ndone;
+        }
+        x.sync();
+        if (tid == 0) { sh.n[icur] = 0; sh.n[2] = 0; }
+        int *tl = cur; cur = nxt; nxt = tl;
+        const int ti = icur; icur = inxt; inxt = ti;
+        x.sync();
+    }
+    // t[0]/t[4] leaving the face: those of the last node in face order that set them
+    if (tid == 0) {
+        int r = Fn - 1;
+        while (r >= 0 && W.stat[r] == 2) r--;
+        if (r >= 0) { sh.cin0 = W.tout0[r]; sh.cin4 = W.tout4[r]; }
+        if (sh.processed != Fn) stats[kStatError] |= 4;
+    }
+    x.sync();
+}
+
+// CHANGE schedule.  Returns false (face restored) if LEVEL must redo the face.
+// Work list: W.list0 (nodes to evaluate), W.list2 (nodes that changed).
+template <int F, class X>
+FD_HD bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
+                       int *stats) {
+    const int T = x.nthr(), tid = x.tid();
+    const int Fn = fg.F;
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    if (tid == 0) { sh.n[0] = 0; sh.n[2] = 0; sh.flag &= ~4; sh.best = 0; }
+    for (int p = tid; p < Fn; p += T) W.stamp[p] = 0;
+    x.sync();
+    for (int round = 1;; round++) {
+        const int nwork = (round == 1) ? Fn : sh.n[0];
+        if (nwork == 0) break;
+        // evaluate: preceding neighbours from the grid, followers from the snapshot
+        for (int it = tid; it < nwork; it += T) {
+            const int p = (round == 1) ? it : W.list0[it];
+            int i, j, k, a, b;
+            face_node<F>(fg, p, i, j, k, a, b);
+            Acc<F, true> A;
+            A.time = J.time; A.slow = J.slow;
+            A.node = (long long)k * nxy + (long long)j * nx + i;
+            A.snode = (long long)k * J.s_sz + (long long)j * J.s_sy + i;
+            A.nx = nx; A.nxy = nxy; A.s_sy = J.s_sy; A.s_sz = J.s_sz;
+            A.a = a; A.b = b; A.p = p; A.kp = W.key[p]; A.fg = &fg; A.O = W.O; A.key = W.key;
+            const float told = W.O[p];
+            NodeOut o;
+            eval_node<F>(A, G, i, j, k, told, J.slow[A.snode], 0.0f, 0.0f, o);
+            const float nv = o.stored ? o.tnew : told;
+            W.flag[p] = (unsigned char)((o.head ? 1 : 0) | (o.used_in ? 2 : 0) | (o.sets ? 4 : 0));
+            W.tout0[p] = o.tout0;
+            W.tout4[p] = o.tout4;
+            const float cv = J.time[A.node];
+            unsigned un, uc;
+            __builtin_memcpy(&un, &nv, 4);
+            __builtin_memcpy(&uc, &cv, 4);
+            if (un != uc) {
+                J.time[A.node] = nv;
+                W.list2[x.atomic_add(&sh.n[2], 1)] = p;
+            }
+        }
+        if (tid == 0) {
+            stats[kStatRoundsChange] += 1;
+            stats[kStatEvals] += nwork;
+        }
+        x.sync();
+        if (tid == 0) sh.n[0] = 0;
+        x.sync();
+        // queue the followers of every node that changed, once each
+        const int nch = sh.n[2];
+        for (int it = tid; it < nch; it += T) {
+            const int p = W.list2[it];
+            const int a = p % fg.wa, b = p / fg.wa;
+            const unsigned kp = W.key[p];
+            for (int db = -1; db <= 1; db++)
+                for (int da = -1; da <= 1; da++) {
+                    if (!da && !db) continue;
+                    const int aa = a + da, bb = b + db;
+                    if (aa < 0 || bb < 0 || aa >= fg.wa || bb >= fg.wb) continue;
+                    const int q = bb * fg.wa + aa;
+                    if (!key_less(kp, p, W.key[q], q)) continue;
+                    if (x.atomic_exch(&W.stamp[q], round) != round) W.list0[x.atomic_add(&sh.n[0], 1)] = q;
+                }
+        }
+        x.sync();
+        if (tid == 0) sh.n[2] = 0;
+        x.sync();
+    }
+    // validity: no node may depend on carried t[0]/t[4]; the last node must set them
+    int bad = 0;
+    int best = -1;
+    unsigned bestk = 0;
+    for (int p = tid; p < Fn; p += T) {
+        bad |= (W.flag[p] & 2);
+        if (best < 0 || key_less(bestk, best, W.key[p], p)) { best = p; bestk = W.key[p]; }
+    }
+    if (bad) x.atomic_or((unsigned *)&sh.flag, 4u);
+    sh.part[tid] = best;
+    x.sync();
+    if (tid == 0) {
+        int bp = -1;
+        unsigned bk = 0;
+        for (int q = 0; q < T; q++) {
+            const int c = sh.part[q];
+            if (c >= 0 && (bp < 0 || key_less(bk, bp, W.key[c], c))) { bp = c; bk = W.key[c]; }
+        }
+        sh.best = bp;
+        if (bp >= 0 && !(W.flag[bp] & 4)) sh.flag |= 4;
+    }
+    x.sync();
+    if (sh.flag & 4) {
+        // restore the face and let LEVEL redo it
+        for (int p = tid; p < Fn; p += T) {
+            int i, j, k, a, b;
+            face_node<F>(fg, p, i, j, k, a, b);
+            J.time[(long long)k * nxy + (long long)j * nx + i] = W.O[p];
+        }
+        if (tid == 0) stats[kStatFallbacks] += 1;
+        x.sync();
+        return false;
+    }
+    // head waves of the final evaluation of every node
+    int hw = 0;
+    for (int p = tid; p < Fn; p += T) hw += W.flag[p] & 1;
+    if (hw) x.atomic_add(&sh.headw[face_slot(F)], hw);
+    if (tid == 0 && sh.best >= 0) { sh.cin0 = W.tout0[sh.best]; sh.cin4 = W.tout4[sh.best]; }
+    x.sync();
+    return true;
+}
+
+template <int F, class X>
+FD_HD void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, int *stats) {
+    const int T = x.nthr(), tid = x.tid();
+    const int(*B)[2] = G.box;
+    FaceGeom fg;
+    fg.face = F;
+    if (F == TOP || F == BOTTOM) {
+        fg.fixed = (F == TOP) ? B[2][0] : B[2][1];
+        fg.a0 = B[0][0] + 1; fg.wa = B[0][1] - 1 - fg.a0 + 1;
+        fg.b0 = B[1][0] + 1; fg.wb = B[1][1] - 1 - fg.b0 + 1;
+    } else if (F == EAST || F == WEST) {
+        fg.fixed = (F == EAST) ? B[0][1] : B[0][0];
+        fg.a0 = B[1][0] + 1; fg.wa = B[1][1] - 1 - fg.a0 + 1;
+        fg.b0 = B[2][0] + 1; fg.wb = B[2][1] - 1 - fg.b0 + 1;
+    } else {
+        fg.fixed = (F == NORTH) ? B[1][0] : B[1][1];
+        fg.a0 = B[0][0] + 1; fg.wa = B[0][1] - 1 - fg.a0 + 1;
+        fg.b0 = B[2][0] + 1; fg.wb = B[2][1] - 1 - fg.b0 + 1;
+    }
+    if (fg.wa <= 0 || fg.wb <= 0) return;
+    fg.F = fg.wa * fg.wb;
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    using S = FaceSpec<F>;
+    const int di = S::nax == AXX ? S::ndir : 0, dj = S::nax == AXY ? S::ndir : 0, dk = S::nax == AXZ ? S::ndir : 0;
+    if (tid == 0) { sh.flag = 0; sh.kor = 0u; sh.kand = 0xffffffffu; }
+    x.sync();
+    // gather the pre-face values and the inner-layer keys
+    unsigned kor = 0u, kand = 0xffffffffu;
+    int fl = 0;
+    for (int p = tid; p < fg.F; p += T) {
+        int i, j, k, a, b;
+        face_node<F>(fg, p, i, j, k, a, b);
+        const long long idx = (long long)k * nxy + (long long)j * nx + i;
+        const float o = J.time[idx];
+        const float kv = J.time[idx + di + dj * nx + dk * nxy];
+        W.O[p] = o;
+        const unsigned kk = order_key(kv);
+        W.key[p] = kk;
+        W.val[p] = p;
+        kor |= kk;
+        kand &= kk;
+        if (!((double)o < kBig)) fl |= 1;
+        if (kv != kv) fl |= 2;
+    }
+    x.atomic_or(&sh.kor, kor);
+    x.atomic_and(&sh.kand, kand);
+    if (fl) x.atomic_or((unsigned *)&sh.flag, (unsigned)fl);
+    x.sync();
+    if (sh.flag & 2) {
+        if (tid == 0) stats[kStatError] |= 1;   // NaN keys: the reference's qsort order is undefined
+    }
+    const bool try_change = !(sh.flag & 1);
+    x.sync();
+    if (try_change) {
+        if (face_change<F>(x, sh, J, W, G, fg, stats)) {
+            if (tid == 0) stats[kStatFacesChange] += 1;
+            x.sync();
+            return;
+        }
+    }
+    // LEVEL needs ranks only; the sort may leave the keys permuted
+    const bool sw = radix_sort(x, sh, W.key, W.val, W.key2, W.val2, fg.F);
+    const int *ord = sw ? W.val2 : W.val;
+    face_level<F>(x, sh, J, W, G, fg, ord, stats);
+    if (tid == 0) stats[kStatFacesLevel] += 1;
+    x.sync();
+}
+
+// ------------------------------------------------------------- one table
+
+template <class X>
+FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
+    const int T = x.nthr(), tid = x.tid();
+    int *stats = J.stats;
+    const long long N = (long long)J.nx * J.ny * J.nz;
+    if (J.init_mode != 1) {
+        for (long long q = tid; q < N; q += T) J.time[q] = 1.0e10f;
+        x.sync();
+        const float s000 = J.slow[J.src_sidx];
+        for (int q = tid; q < J.ninit; q += T)
+            J.time[J.init_idx[q]] = J.init_mode == 0 ? (float)((double)s000 * J.init_dist[q]) : J.init_val[q];
+    }
+    if (tid == 0) {
+        for (int s = 0; s < kStatCount; s++) stats[s] = 0;
+        sh.cin0 = 0.0f;
+        sh.cin4 = 0.0f;
+    }
+    x.sync();
+    Geo G;
+    G.h = J.h; G.dq = J.dq; G.df = J.df; G.y0 = J.y0; G.z0r = J.z0r; G.sinq = J.sinq;
+    G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;
+    int x1 = J.box[0][0], x2 = J.box[0][1], y1 = J.box[1][0], y2 = J.box[1][1], z1 = J.box[2][0], z2 = J.box[2][1];
+    int dx1 = J.dflag[0], dx2 = J.dflag[1], dy1 = J.dflag[2], dy2 = J.dflag[3], dz1 = J.dflag[4], dz2 = J.dflag[5];
+    int rad0 = 1, radius = J.radius0, srcwall = J.srcwall, reverse = J.reverse;
+    const int nx = J.nx, ny = J.ny, nz = J.nz;
+    int passes = 0;
+#define FD_SETBOX()                                  \
+    do {                                             \
+        G.box[0][0] = x1; G.box[0][1] = x2;          \
+        G.box[1][0] = y1; G.box[1][1] = y2;          \
+        G.box[2][0] = z1; G.box[2][1] = z2;          \
+    } while (0)
+    while (reverse > -1) {
+        if (tid == 0)
+            for (int w = 0; w < 7; w++) sh.headw[w] = 0;
+        x.sync();
+        passes++;
+        while (rad0 && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
+            for (int g = 1; g <= J.grow[0]; g++)
+                if (dz1) { FD_SETBOX(); sweep_face<TOP>(x, sh, J, W, G, stats); if (z1 == 0) dz1 = 0; z1--; }
+            for (int g = 1; g <= J.grow[1]; g++)
+                if (dz2) { FD_SETBOX(); sweep_face<BOTTOM>(x, sh, J, W, G, stats); if (z2 == nz - 1) dz2 = 0; z2++; }
+            for (int g = 1; g <= J.grow[2]; g++)
+                if (dx2) { FD_SETBOX(); sweep_face<EAST>(x, sh, J, W, G, stats); if (x2 == nx - 1) dx2 = 0; x2++; }
+            for (int g = 1; g <= J.grow[3]; g++)
+                if (dx1) { FD_SETBOX(); sweep_face<WEST>(x, sh, J, W, G, stats); if (x1 == 0) dx1 = 0; x1--; }
+            for (int g = 1; g <= J.grow[4]; g++)
+                if (dy1) { FD_SETBOX(); sweep_face<NORTH>(x, sh, J, W, G, stats); if (y1 == 0) dy1 = 0; y1--; }
+            for (int g = 1; g <= J.grow[5]; g++)
+                if (dy2) { FD_SETBOX(); sweep_face<SOUTH>(x, sh, J, W, G, stats); if (y2 == ny - 1) dy2 = 0; y2++; }
+            radius++;
+            if (radius == J.maxrad) rad0 = 0;
+        }
+        x.sync();
+        int hw[7];
+        for (int w = 0; w < 7; w++) hw[w] = sh.headw[w];
+        if (passes == 1 && tid == 0)
+            for (int w = 0; w < 7; w++) stats[kStatHeadw0 + w] = hw[w];
+        // sphfd.c:5570-5676
+        if (hw[1] == 0 && hw[2] == 0 && hw[3] == 0 && hw[4] == 0 && hw[5] == 0 && hw[6] == 0) {
+            reverse = 0;
+        } else {
+            int head = 0;
+            for (int w = 1; w <= 6; w++)
+                if (hw[w] > 0 && hw[w] > head) { head = hw[w]; srcwall = w; }
+            if (J.headpref > 0 && hw[J.headpref] > 0) srcwall = J.headpref;
+            dx1 = dx2 = dy1 = dy2 = dz1 = dz2 = 1;
+            rad0 = 1;
+            radius = 1;
+            if (srcwall == 1) x2 = 1; else { x2 = nx; dx2 = 0; }
+            if (srcwall == 2) x1 = nx - 2; else { x1 = -1; dx1 = 0; }
+            if (srcwall == 3) y2 = 1; else { y2 = ny; dy2 = 0; }
+            if (srcwall == 4) y1 = ny - 2; else { y1 = -1; dy1 = 0; }
+            if (srcwall == 5) z2 = 1; else { z2 = nz; dz2 = 0; }
+            if (srcwall == 6) z1 = nz - 2; else { z1 = -1; dz1 = 0; }
+            if (tid == 0) {
+                stats[kStatLastWall] = srcwall;
+                if (reverse > 0) stats[kStatRestarts] += 1;
+            }
+        }
+        reverse--;
+        x.sync();
+    }
+#undef FD_SETBOX
+    if (tid == 0) stats[kStatPasses] = passes;
+    x.sync();
+}
+
+}  // namespace fdsweep

@@ -1,0 +1,39 @@
+"""ctypes binding of the host-thread executor of the sweep solver
+(tests/native/libemu_solver.so).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+import oracle_bind as ob
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU_SO = os.path.join(REPO, "tests", "native", "libemu_solver.so")
+STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",
+              "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(EMU_SO)
+        L.emu_solve.argtypes = [C.POINTER(ob._Par), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.emu_solve.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def solve(par: "ob.Par", slow: np.ndarray, nthreads: int = 4, extra=None):
+    slow = np.ascontiguousarray(slow, dtype=np.float32)
+    time = np.empty_like(slow)
+    stats = np.zeros(len(STAT_NAMES), dtype=np.int32)
+    ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float32)
+    rc = lib().emu_solve(C.byref(par.to_c()), slow.ctypes.data, time.ctypes.data,
+                         None if ex is None else ex.ctypes.data, nthreads, stats.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"emu_solve failed: {rc}")
+    return time, dict(zip(STAT_NAMES, stats.tolist()))

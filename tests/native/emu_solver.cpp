@@ -1,0 +1,95 @@
+// Host-thread executor for the sweep solver (fdtomoc_amd/csrc/sweep_solver.h).
+//
+// Test infrastructure: runs the exact bulk-synchronous code that the gfx950
+// kernel runs, with std::thread workers standing in for the lanes of one
+// workgroup, so the schedule (LEVEL / CHANGE, carried-register resolution,
+// radix sort) can be checked against the oracle on a machine without a GPU.
+// It is not part of the product and is never loaded by it.
+#include <barrier>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../fdtomoc_amd/csrc/sweep_solver.h"
+#include "../../fdtomoc_amd/csrc/table_prep.h"
+
+namespace {
+struct HostExec {
+    int t, n;
+    std::barrier<> *bar;
+    int tid() const { return t; }
+    int nthr() const { return n; }
+    void sync() { bar->arrive_and_wait(); }
+    int atomic_add(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+    int atomic_exch(int *p, int v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+    unsigned atomic_or(unsigned *p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+    unsigned atomic_and(unsigned *p, unsigned v) { return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
+};
+}  // namespace
+
+// Same layout as oracle_par (oracle/sphfd_oracle.h) so one ctypes struct serves both.
+struct EmuPar {
+    int nx, ny, nz;
+    double h, dq, df, x0, y0, z0;
+    int iminus, iplus, jminus, jplus, kminus, kplus;
+    int floatsrc, srctype, srcwall;
+    double fxs, fys, fzs;
+    int xs, ys, zs;
+    int ncube, reverse, headpref, parse, fill;
+    float maxoff;
+    int invh, invq, invf, invx0, invy0, invz0;
+};
+
+extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const float *extra, int nthreads,
+                         int *stats) {
+    fdprep::ParValues p;
+    p.nx = pc->nx; p.ny = pc->ny; p.nz = pc->nz;
+    p.h = pc->h; p.dq = pc->dq; p.df = pc->df; p.x0 = pc->x0; p.y0 = pc->y0; p.z0 = pc->z0;
+    p.iminus = pc->iminus; p.iplus = pc->iplus; p.jminus = pc->jminus; p.jplus = pc->jplus;
+    p.kminus = pc->kminus; p.kplus = pc->kplus;
+    p.floatsrc = pc->floatsrc; p.srctype = pc->srctype; p.srcwall = pc->srcwall;
+    p.fxs = pc->fxs; p.fys = pc->fys; p.fzs = pc->fzs; p.xs = pc->xs; p.ys = pc->ys; p.zs = pc->zs;
+    p.ncube = pc->ncube; p.reverse = pc->reverse; p.headpref = pc->headpref; p.parse = pc->parse; p.fill = pc->fill;
+    p.maxoff = pc->maxoff;
+    p.invh = pc->invh; p.invq = pc->invq; p.invf = pc->invf; p.invx0 = pc->invx0; p.invy0 = pc->invy0; p.invz0 = pc->invz0;
+    fdprep::Prepared pr;
+    int rc = fdprep::prepare_geometry(p, pr);
+    if (rc) return rc;
+    const bool preset = p.srctype != 1;
+    rc = fdprep::prepare_source(p, pr, preset ? extra : nullptr, (p.fill == 1) ? extra : nullptr, preset ? time : nullptr);
+    if (rc) return rc;
+    const long long maxF = fdprep::max_face(p.nx, p.ny, p.nz);
+    std::vector<float> O(maxF), tout0(maxF), tout4(maxF), rt0(maxF), rt4(maxF);
+    std::vector<unsigned> key(maxF), key2(maxF);
+    std::vector<int> val(maxF), val2(maxF), rank(maxF), ord(maxF), pend(maxF), stat(maxF), rstat(maxF), l0(maxF),
+        l1(maxF), l2(maxF), stamp(maxF);
+    std::vector<unsigned char> flag(maxF);
+    fdsweep::Scratch W;
+    W.O = O.data(); W.key = key.data(); W.key2 = key2.data(); W.val = val.data(); W.val2 = val2.data();
+    W.rank = rank.data(); W.ord = ord.data(); W.pend = pend.data(); W.stat = stat.data();
+    W.tout0 = tout0.data(); W.tout4 = tout4.data(); W.rt0 = rt0.data(); W.rt4 = rt4.data(); W.rstat = rstat.data();
+    W.list0 = l0.data(); W.list1 = l1.data(); W.list2 = l2.data(); W.stamp = stamp.data(); W.flag = flag.data();
+    fdsweep::TableJob J;
+    std::memset(&J, 0, sizeof J);
+    J.nx = p.nx; J.ny = p.ny; J.nz = p.nz;
+    J.h = pr.h; J.dq = pr.dq; J.df = pr.df; J.y0 = pr.y0; J.z0r = pr.z0r; J.sinq = pr.sinq.data();
+    J.slow = slow; J.s_sy = p.nx; J.s_sz = (long long)p.nx * p.ny; J.time = time;
+    J.init_mode = pr.init_mode; J.src_sidx = pr.src_index; J.ninit = (int)pr.init_idx.size();
+    J.init_idx = pr.init_idx.data(); J.init_dist = pr.init_dist.data(); J.init_val = pr.init_val.data();
+    std::memcpy(J.box, pr.box, sizeof J.box);
+    std::memcpy(J.dflag, pr.dflag, sizeof J.dflag);
+    std::memcpy(J.grow, pr.grow, sizeof J.grow);
+    J.reverse = pr.reverse; J.headpref = pr.headpref; J.maxrad = pr.maxrad; J.radius0 = pr.radius0;
+    J.srcwall = pr.srcwall; J.stats = stats;
+    fdsweep::Shared sh;
+    std::memset(&sh, 0, sizeof sh);
+    std::barrier<> bar(nthreads);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([&, t]() {
+            HostExec x{t, nthreads, &bar};
+            fdsweep::solve_table(x, sh, J, W);
+        });
+    for (auto &t : th) t.join();
+    return 0;
+}

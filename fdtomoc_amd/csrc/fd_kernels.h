@@ -1,0 +1,12 @@
+// fd_kernels.h -- launch wrappers of the gfx950 kernels (sweep_kernel.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sweep_solver.h"
+
+#define FD_SWEEP_THREADS 256
+
+extern "C" hipError_t fd_launch_sweep(const fdsweep::TableJob *d_jobs, int njobs, const fdsweep::Scratch *d_slots,
+                                      int nslots, int *d_queue, hipStream_t stream);
+extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream);

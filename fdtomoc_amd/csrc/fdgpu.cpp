@@ -1,0 +1,505 @@
+// fdgpu.cpp -- host side of the batched/served sphfd GPU API (include/fdtomoc_amd.h).
+//
+// Owns device memory (models, resident tables, per-workgroup scratch),
+// prepares each table on the host (table_prep.cpp: the reference's geometry and
+// source set-up) and launches the persistent sweep kernel.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtomoc_amd.h"
+#include "fd_kernels.h"
+#include "fdgpu_internal.h"
+#include "sphfd_io.h"
+#include "table_prep.h"
+
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Model {
+    int nx = 0, ny = 0, nz = 0;
+    float *d_slow = nullptr;
+    size_t bytes = 0;
+};
+
+struct Table {
+    fdgpu_par par{};
+    int model = -1;
+    int ox = 0, oy = 0, oz = 0;
+    long long n = 0;
+    float *d_time = nullptr;
+    char *d_aux = nullptr;     // sinq | init_idx | init_dist | init_val | stats
+    size_t bytes = 0;
+    int *d_stats = nullptr;
+    std::vector<int> stats;
+    fdprep::Prepared prep;
+};
+
+}  // namespace
+
+struct fdgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long budget = 0, used = 0;
+    int num_cu = 0;
+    int wg_per_cu = 1;
+    std::map<int, Model> models;
+    std::map<int, Table> tables;
+    int next_id = 1;
+    // per-workgroup scratch, grown on demand
+    char *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    fdsweep::Scratch *d_slots = nullptr;
+    int slots_cap = 0;
+    long long slot_face_cap = 0;
+    int *d_queue = nullptr;
+    double last_ms = 0.0;
+    std::mutex mu;
+};
+
+namespace {
+
+int hip_fail(hipError_t e) { return e == hipErrorOutOfMemory ? FDGPU_ENOMEM : FDGPU_EHIP; }
+
+#define FD_HIP(call)                                  \
+    do {                                              \
+        hipError_t _e = (call);                       \
+        if (_e != hipSuccess) return hip_fail(_e);    \
+    } while (0)
+
+int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes) {
+    if (c->used + bytes > c->budget) return FDGPU_ENOMEM;
+    FD_HIP(hipMalloc(p, bytes ? bytes : 1));
+    c->used += bytes;
+    return FDGPU_OK;
+}
+
+void dev_free(fdgpu_ctx *c, void *p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    c->used -= bytes;
+}
+
+size_t slot_bytes(long long F) {
+    const size_t f = align256(sizeof(float) * (size_t)F);
+    return 18 * f + align256((size_t)F);
+}
+
+int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
+    if (nslots <= c->slots_cap && maxF <= c->slot_face_cap) return FDGPU_OK;
+    const int ns = std::max(nslots, c->slots_cap);
+    const long long F = std::max(maxF, c->slot_face_cap);
+    dev_free(c, c->d_scratch, c->scratch_bytes);
+    dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
+    c->d_scratch = nullptr;
+    c->d_slots = nullptr;
+    c->slots_cap = 0;
+    c->slot_face_cap = 0;
+    const size_t per = slot_bytes(F);
+    int rc = dev_alloc(c, (void **)&c->d_scratch, per * (size_t)ns);
+    if (rc) return rc;
+    c->scratch_bytes = per * (size_t)ns;
+    rc = dev_alloc(c, (void **)&c->d_slots, sizeof(fdsweep::Scratch) * (size_t)ns);
+    if (rc) return rc;
+    std::vector<fdsweep::Scratch> h((size_t)ns);
+    const size_t f = align256(sizeof(float) * (size_t)F);
+    for (int s = 0; s < ns; s++) {
+        char *b = c->d_scratch + per * (size_t)s;
+        fdsweep::Scratch &W = h[(size_t)s];
+        size_t o = 0;
+        auto take = [&](size_t n) { char *p = b + o; o += n; return p; };
+        W.O = (float *)take(f);
+        W.key = (unsigned *)take(f);
+        W.key2 = (unsigned *)take(f);
+        W.val = (int *)take(f);
+        W.val2 = (int *)take(f);
+        W.rank = (int *)take(f);
+        W.ord = (int *)take(f);
+        W.pend = (int *)take(f);
+        W.stat = (int *)take(f);
+        W.tout0 = (float *)take(f);
+        W.tout4 = (float *)take(f);
+        W.rt0 = (float *)take(f);
+        W.rt4 = (float *)take(f);
+        W.rstat = (int *)take(f);
+        W.list0 = (int *)take(f);
+        W.list1 = (int *)take(f);
+        W.list2 = (int *)take(f);
+        W.stamp = (int *)take(f);
+        W.flag = (unsigned char *)take(align256((size_t)F));
+    }
+    FD_HIP(hipMemcpy(c->d_slots, h.data(), sizeof(fdsweep::Scratch) * (size_t)ns, hipMemcpyHostToDevice));
+    c->slots_cap = ns;
+    c->slot_face_cap = F;
+    return FDGPU_OK;
+}
+
+fdprep::ParValues to_values(const fdgpu_par &p) {
+    fdprep::ParValues v;
+    v.nx = p.nx; v.ny = p.ny; v.nz = p.nz;
+    v.h = p.h; v.dq = p.dq; v.df = p.df; v.x0 = p.x0; v.y0 = p.y0; v.z0 = p.z0;
+    v.iminus = p.iminus; v.iplus = p.iplus; v.jminus = p.jminus; v.jplus = p.jplus; v.kminus = p.kminus; v.kplus = p.kplus;
+    v.floatsrc = p.floatsrc; v.srctype = p.srctype; v.srcwall = p.srcwall;
+    v.fxs = p.fxs; v.fys = p.fys; v.fzs = p.fzs; v.xs = p.xs; v.ys = p.ys; v.zs = p.zs;
+    v.ncube = p.ncube; v.reverse = p.reverse; v.headpref = p.headpref; v.parse = p.parse; v.fill = p.fill;
+    v.maxoff = p.maxoff;
+    v.invh = p.invh; v.invq = p.invq; v.invf = p.invf; v.invx0 = p.invx0; v.invy0 = p.invy0; v.invz0 = p.invz0;
+    v.swab = p.swab;
+    return v;
+}
+
+void release_table(fdgpu_ctx *c, Table &t) {
+    dev_free(c, t.d_time, sizeof(float) * (size_t)t.n);
+    dev_free(c, t.d_aux, t.bytes);
+    t.d_time = nullptr;
+    t.d_aux = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fdgpu_par_defaults(fdgpu_par *p) {
+    memset(p, 0, sizeof *p);
+    p->y0 = 90.0;
+    p->iminus = p->iplus = p->jminus = p->jplus = p->kminus = p->kplus = 1;
+    p->floatsrc = 1;
+    p->srctype = 1;
+    p->ncube = 2;
+    p->reverse = 1;
+    p->headpref = 6;
+    p->parse = 1;
+    p->maxoff = -1.f;
+}
+
+const char *fdgpu_strerror(int code) {
+    switch (code) {
+    case FDGPU_OK: return "ok";
+    case FDGPU_EHIP: return "HIP runtime error";
+    case FDGPU_ENOMEM: return "out of device memory or HBM budget";
+    case FDGPU_ESOURCE: return "source does not appear to be in the model";
+    case FDGPU_ESRCTYPE: return "incorrect value of srctype";
+    case FDGPU_EINPUT: return "bad argument or missing input data";
+    case FDGPU_EIO: return "file open/read/write failure";
+    case FDGPU_EPAR: return "required parameter missing";
+    case FDGPU_ESTOP: return "stop due to STOP in input";
+    case FDGPU_EKERNEL: return "sweep kernel reported an internal error";
+    case FDGPU_EBUSY: return "serving queue full";
+    }
+    return "unknown error";
+}
+
+int fdgpu_open(int device, unsigned long long budget, fdgpu_ctx **out) {
+    if (!out) return FDGPU_EINPUT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return FDGPU_EHIP;
+    FD_HIP(hipSetDevice(device));
+    fdgpu_ctx *c = new fdgpu_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return FDGPU_EHIP; }
+    c->num_cu = prop.multiProcessorCount;
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) { delete c; return FDGPU_EHIP; }
+    c->budget = budget ? budget : (unsigned long long)(0.9 * (double)freeb);
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fd_sweep_kernel_ptr(), FD_SWEEP_THREADS, 0) == hipSuccess &&
+        per_cu > 0)
+        c->wg_per_cu = per_cu;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc(&c->d_queue, sizeof(int)) != hipSuccess) {
+        fdgpu_close(c);
+        return FDGPU_EHIP;
+    }
+    *out = c;
+    return FDGPU_OK;
+}
+
+int fdgpu_close(fdgpu_ctx *c) {
+    if (!c) return FDGPU_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->tables) release_table(c, kv.second);
+    for (auto &kv : c->models) dev_free(c, kv.second.d_slow, kv.second.bytes);
+    dev_free(c, c->d_scratch, c->scratch_bytes);
+    dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
+    if (c->d_queue) (void)hipFree(c->d_queue);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FDGPU_OK;
+}
+
+int fdgpu_model_create(fdgpu_ctx *c, int nx, int ny, int nz, const float *vel, int *model) {
+    if (!c || !vel || !model || nx <= 0 || ny <= 0 || nz <= 0) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    FD_HIP(hipSetDevice(c->device));
+    Model m;
+    m.nx = nx; m.ny = ny; m.nz = nz;
+    const long long n = (long long)nx * ny * nz;
+    m.bytes = sizeof(float) * (size_t)n;
+    int rc = dev_alloc(c, (void **)&m.d_slow, m.bytes);
+    if (rc) return rc;
+    const int id = c->next_id++;
+    c->models[id] = m;
+    *model = id;
+    rc = fdgpu_model_update_locked(c, id, vel);
+    return rc;
+}
+
+int fdgpu_model_update(fdgpu_ctx *c, int model, const float *vel) {
+    if (!c || !vel) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    FD_HIP(hipSetDevice(c->device));
+    return fdgpu_model_update_locked(c, model, vel);
+}
+
+int fdgpu_model_release(fdgpu_ctx *c, int model) {
+    if (!c) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->models.find(model);
+    if (it == c->models.end()) return FDGPU_EINPUT;
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(c, it->second.d_slow, it->second.bytes);
+    c->models.erase(it);
+    return FDGPU_OK;
+}
+
+int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int *ids) {
+    if (!c || (!desc && n > 0) || n < 0 || !ids) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    FD_HIP(hipSetDevice(c->device));
+    // prepare every table on the host first: no device work on bad input
+    std::vector<Table> tabs((size_t)n);
+    std::vector<std::vector<float>> presets((size_t)n);
+    long long maxF = 1;
+    for (int t = 0; t < n; t++) {
+        const fdgpu_table_desc &d = desc[t];
+        auto mit = c->models.find(d.model);
+        if (mit == c->models.end()) return FDGPU_EINPUT;
+        const Model &m = mit->second;
+        const fdgpu_par &p = d.par;
+        if (p.nx < 1 || p.ny < 1 || p.nz < 1 || d.ox < 0 || d.oy < 0 || d.oz < 0 || d.ox + p.nx > m.nx ||
+            d.oy + p.ny > m.ny || d.oz + p.nz > m.nz || p.ncube < 0)
+            return FDGPU_EINPUT;
+        Table &T = tabs[(size_t)t];
+        T.par = p;
+        T.model = d.model;
+        T.ox = d.ox; T.oy = d.oy; T.oz = d.oz;
+        T.n = (long long)p.nx * p.ny * p.nz;
+        const fdprep::ParValues v = to_values(p);
+        int rc = fdprep::prepare_geometry(v, T.prep);
+        if (rc) return rc;
+        if (p.srctype != 1) {
+            if (!d.preset) return FDGPU_EINPUT;
+            presets[(size_t)t].resize((size_t)T.n);
+        }
+        if (p.fill == 1 && !d.boxvals) return FDGPU_EINPUT;
+        rc = fdprep::prepare_source(v, T.prep, d.preset, d.boxvals, p.srctype != 1 ? presets[(size_t)t].data() : nullptr);
+        if (rc) return rc;
+        maxF = std::max(maxF, fdprep::max_face(p.nx, p.ny, p.nz));
+    }
+    // device buffers per table
+    std::vector<fdsweep::TableJob> jobs((size_t)n);
+    int rc = FDGPU_OK;
+    int made = 0;
+    for (int t = 0; t < n && rc == FDGPU_OK; t++) {
+        Table &T = tabs[(size_t)t];
+        const Model &m = c->models[T.model];
+        const size_t b_sin = align256(sizeof(double) * T.prep.sinq.size());
+        const size_t b_idx = align256(sizeof(int) * T.prep.init_idx.size());
+        const size_t b_dist = align256(sizeof(double) * T.prep.init_dist.size());
+        const size_t b_val = align256(sizeof(float) * T.prep.init_val.size());
+        const size_t b_st = align256(sizeof(int) * fdsweep::kStatCount);
+        T.bytes = b_sin + b_idx + b_dist + b_val + b_st;
+        rc = dev_alloc(c, (void **)&T.d_time, sizeof(float) * (size_t)T.n);
+        if (rc) break;
+        rc = dev_alloc(c, (void **)&T.d_aux, T.bytes);
+        if (rc) { dev_free(c, T.d_time, sizeof(float) * (size_t)T.n); T.d_time = nullptr; break; }
+        made++;
+        std::vector<char> aux(T.bytes, 0);
+        size_t o = 0;
+        memcpy(aux.data() + o, T.prep.sinq.data(), sizeof(double) * T.prep.sinq.size());
+        const double *d_sin = (const double *)(T.d_aux + o);
+        o += b_sin;
+        memcpy(aux.data() + o, T.prep.init_idx.data(), sizeof(int) * T.prep.init_idx.size());
+        const int *d_idx = (const int *)(T.d_aux + o);
+        o += b_idx;
+        memcpy(aux.data() + o, T.prep.init_dist.data(), sizeof(double) * T.prep.init_dist.size());
+        const double *d_dist = (const double *)(T.d_aux + o);
+        o += b_dist;
+        memcpy(aux.data() + o, T.prep.init_val.data(), sizeof(float) * T.prep.init_val.size());
+        const float *d_val = (const float *)(T.d_aux + o);
+        o += b_val;
+        T.d_stats = (int *)(T.d_aux + o);
+        if (hipMemcpyAsync(T.d_aux, aux.data(), T.bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            rc = FDGPU_EHIP;
+            break;
+        }
+        if (T.par.srctype != 1 &&
+            hipMemcpyAsync(T.d_time, presets[(size_t)t].data(), sizeof(float) * (size_t)T.n, hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess) {
+            rc = FDGPU_EHIP;
+            break;
+        }
+        (void)hipStreamSynchronize(c->stream);   // host staging buffers are freed at scope end
+        fdsweep::TableJob &J = jobs[(size_t)t];
+        memset(&J, 0, sizeof J);
+        J.nx = T.par.nx; J.ny = T.par.ny; J.nz = T.par.nz;
+        J.h = T.prep.h; J.dq = T.prep.dq; J.df = T.prep.df; J.y0 = T.prep.y0; J.z0r = T.prep.z0r;
+        J.sinq = d_sin;
+        J.s_sy = m.nx;
+        J.s_sz = (long long)m.nx * m.ny;
+        J.slow = m.d_slow + (long long)T.oz * J.s_sz + (long long)T.oy * J.s_sy + T.ox;
+        J.time = T.d_time;
+        J.init_mode = T.prep.init_mode;
+        J.src_sidx = (long long)T.prep.zs * J.s_sz + (long long)T.prep.ys * J.s_sy + T.prep.xs;
+        J.ninit = (int)T.prep.init_idx.size();
+        J.init_idx = d_idx;
+        J.init_dist = d_dist;
+        J.init_val = d_val;
+        memcpy(J.box, T.prep.box, sizeof J.box);
+        memcpy(J.dflag, T.prep.dflag, sizeof J.dflag);
+        memcpy(J.grow, T.prep.grow, sizeof J.grow);
+        J.reverse = T.prep.reverse; J.headpref = T.prep.headpref; J.maxrad = T.prep.maxrad;
+        J.radius0 = T.prep.radius0; J.srcwall = T.prep.srcwall;
+        J.stats = T.d_stats;
+    }
+    auto unwind = [&]() {
+        for (int t = 0; t < made; t++) release_table(c, tabs[(size_t)t]);
+    };
+    if (rc) { unwind(); return rc; }
+    // queue order: largest table first (longest-processing-time-first)
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    if (mode == FDGPU_BATCHED)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tabs[(size_t)a].n > tabs[(size_t)b].n; });
+    std::vector<fdsweep::TableJob> qjobs((size_t)n);
+    for (int t = 0; t < n; t++) qjobs[(size_t)t] = jobs[(size_t)order[(size_t)t]];
+    int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu) : 1;
+    if (nslots < 1) nslots = 1;
+    rc = ensure_scratch(c, nslots, maxF);
+    if (rc) { unwind(); return rc; }
+    fdsweep::TableJob *d_jobs = nullptr;
+    rc = dev_alloc(c, (void **)&d_jobs, sizeof(fdsweep::TableJob) * (size_t)std::max(n, 1));
+    if (rc) { unwind(); return rc; }
+    bool ok = hipMemcpy(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
+    if (mode == FDGPU_BATCHED) {
+        ok = ok && hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream) == hipSuccess;
+        ok = ok && fd_launch_sweep(d_jobs, n, c->d_slots, nslots, c->d_queue, c->stream) == hipSuccess;
+    } else {
+        for (int t = 0; t < n && ok; t++) {
+            ok = hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream) == hipSuccess;
+            ok = ok && fd_launch_sweep(d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
+        }
+    }
+    ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    float ms = 0.f;
+    if (ok && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
+    dev_free(c, d_jobs, sizeof(fdsweep::TableJob) * (size_t)std::max(n, 1));
+    if (!ok) { unwind(); return FDGPU_EHIP; }
+    int kerr = 0;
+    for (int t = 0; t < n; t++) {
+        Table &T = tabs[(size_t)t];
+        T.stats.assign(fdsweep::kStatCount, 0);
+        if (hipMemcpy(T.stats.data(), T.d_stats, sizeof(int) * fdsweep::kStatCount, hipMemcpyDeviceToHost) != hipSuccess) {
+            unwind();
+            return FDGPU_EHIP;
+        }
+        if (T.stats[fdsweep::kStatError] & ~1) kerr = 1;   // bit0 (NaN keys) is reported, not fatal
+        const int id = c->next_id++;
+        ids[t] = id;
+        c->tables[id] = std::move(T);
+    }
+    return kerr ? FDGPU_EKERNEL : FDGPU_OK;
+}
+
+double fdgpu_last_kernel_ms(fdgpu_ctx *c) { return c ? c->last_ms : 0.0; }
+
+int fdgpu_table_read(fdgpu_ctx *c, int table, float *host, unsigned long long count) {
+    if (!c || !host) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->tables.find(table);
+    if (it == c->tables.end() || count < (unsigned long long)it->second.n) return FDGPU_EINPUT;
+    FD_HIP(hipSetDevice(c->device));
+    FD_HIP(hipMemcpy(host, it->second.d_time, sizeof(float) * (size_t)it->second.n, hipMemcpyDeviceToHost));
+    return FDGPU_OK;
+}
+
+int fdgpu_table_stats(fdgpu_ctx *c, int table, int *stats, int nstats) {
+    if (!c || !stats) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->tables.find(table);
+    if (it == c->tables.end()) return FDGPU_EINPUT;
+    for (int s = 0; s < nstats && s < (int)it->second.stats.size(); s++) stats[s] = it->second.stats[(size_t)s];
+    return FDGPU_OK;
+}
+
+int fdgpu_table_release(fdgpu_ctx *c, int table) {
+    if (!c) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->tables.find(table);
+    if (it == c->tables.end()) return FDGPU_EINPUT;
+    (void)hipSetDevice(c->device);
+    release_table(c, it->second);
+    c->tables.erase(it);
+    return FDGPU_OK;
+}
+
+int fdgpu_table_device_ptr(fdgpu_ctx *c, int table, void **dptr) {
+    if (!c || !dptr) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->tables.find(table);
+    if (it == c->tables.end()) return FDGPU_EINPUT;
+    *dptr = it->second.d_time;
+    return FDGPU_OK;
+}
+
+int fdgpu_table_write(fdgpu_ctx *c, int table, const char *path, const void *vel_header) {
+    if (!c || !path) return FDGPU_EINPUT;
+    std::vector<float> host;
+    fdgpu_par par;
+    fdprep::Prepared prep;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto it = c->tables.find(table);
+        if (it == c->tables.end()) return FDGPU_EINPUT;
+        host.resize((size_t)it->second.n);
+        FD_HIP(hipSetDevice(c->device));
+        FD_HIP(hipMemcpy(host.data(), it->second.d_time, sizeof(float) * host.size(), hipMemcpyDeviceToHost));
+        par = it->second.par;
+        prep = it->second.prep;
+    }
+    return fdio::write_table(path, par, prep, vel_header, host.data());
+}
+
+}  // extern "C"
+
+// model upload (called with the context lock held)
+int fdgpu_model_update_locked(fdgpu_ctx *c, int model, const float *vel) {
+    auto it = c->models.find(model);
+    if (it == c->models.end()) return FDGPU_EINPUT;
+    Model &m = it->second;
+    float *d_vel = nullptr;
+    int rc = dev_alloc(c, (void **)&d_vel, m.bytes);
+    if (rc) return rc;
+    const long long n = (long long)m.nx * m.ny * m.nz;
+    bool ok = hipMemcpyAsync(d_vel, vel, m.bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+    ok = ok && fd_launch_slowness(d_vel, m.d_slow, n, c->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    dev_free(c, d_vel, m.bytes);
+    return ok ? FDGPU_OK : FDGPU_EHIP;
+}

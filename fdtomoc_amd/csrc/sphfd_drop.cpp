@@ -1,0 +1,194 @@
+// sphfd_drop.cpp -- the reference's entry points on top of the GPU API:
+//   int sphfd(int ac, char **av, char *output_path)     src/sphfd.c:260-5805
+//   fdgpu_run_spec(spec)  = body of main()                src/sphfd.c:200-257
+// The per-table work of the reference (par intake, velocity read, solve,
+// table write) is kept; the OpenMP loop over tables becomes one batched launch
+// per distinct velocity input.
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/fdtomoc_amd.h"
+#include "sphfd_io.h"
+#include "table_prep.h"
+
+namespace {
+
+constexpr int kMaxStrLen = 132;    // sphfd.c:142
+constexpr int kMaxNumPar = 2000;   // sphfd.c:143
+
+struct Loaded {
+    fdprep::ParValues p;
+    fdprep::Prepared g;
+    std::string timefile;           // output_path + "/" + timefile (sphfd.c:525-529)
+    std::vector<float> preset, boxvals;
+};
+
+bool read_floats(const std::string &path, long long offset, long long n, std::vector<float> &out) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    out.resize((size_t)n);
+    bool ok = fseek(f, (long)offset, SEEK_SET) == 0 && (long long)fread(out.data(), 4, (size_t)n, f) == n;
+    fclose(f);
+    return ok;
+}
+
+// Par intake and the auxiliary inputs of one table (sphfd.c:383-606, 696-722,
+// 826-971).
+int load_table(int ac, char **av, const char *output_path, Loaded &L) {
+    fdio::GetPar gp;
+    int rc = gp.setpar(ac, av);
+    if (rc) return rc;
+    std::string missing;
+    rc = fdio::read_params(gp, L.p, missing);
+    if (rc == FDGPU_EPAR) fprintf(stderr, "sphfd: must specify value for '%s'\n", missing.c_str());
+    if (rc) return rc;
+    L.timefile = std::string(output_path ? output_path : ".") + "/" + L.p.timefile;
+    rc = fdprep::prepare_geometry(L.p, L.g);
+    if (rc) return rc;
+    const long long n = (long long)L.p.nx * L.p.ny * L.p.nz;
+    if (L.p.srctype == 2) {
+        if (!read_floats(L.p.wallfile, 0, 6 * fdprep::max_face(L.p.nx, L.p.ny, L.p.nz), L.preset)) return FDGPU_EIO;
+    } else if (L.p.srctype == 3) {
+        // sphfd.c:931 reads the first nx*ny*nz floats of the file, header included
+        if (!read_floats(L.p.oldtfile, 0, n, L.preset)) return FDGPU_EIO;
+    }
+    if (L.p.fill == 1) {
+        const int NC = L.p.ncube;
+        auto clip = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+        const int bx = clip(L.g.xs + NC, 0, L.p.nx - 1) - clip(L.g.xs - NC, 0, L.p.nx - 1) + 1;
+        const int by = clip(L.g.ys + NC, 0, L.p.ny - 1) - clip(L.g.ys - NC, 0, L.p.ny - 1) + 1;
+        const int bz = clip(L.g.zs + NC, 0, L.p.nz - 1) - clip(L.g.zs - NC, 0, L.p.nz - 1) + 1;
+        if (!read_floats(L.p.boxfile, 232, (long long)bx * by * bz, L.boxvals)) return FDGPU_EIO;
+    }
+    return 0;
+}
+
+std::mutex g_mu;
+fdgpu_ctx *g_ctx = nullptr;
+
+int shared_ctx(fdgpu_ctx **c) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx) {
+        const int rc = fdgpu_open(0, 0, &g_ctx);
+        if (rc) return rc;
+    }
+    *c = g_ctx;
+    return 0;
+}
+
+// Solves tables that share one velocity input in one batched launch and
+// writes them.
+int solve_group(fdgpu_ctx *c, std::vector<Loaded *> &grp, int mode) {
+    Loaded &L0 = *grp[0];
+    std::vector<float> vel;
+    fdio::VHead fhead;
+    bool has_head = false;
+    int rc = fdio::read_velocity(L0.p, L0.p.velfile, vel, fhead, has_head);
+    if (rc) {
+        fprintf(stderr, "sphfd: cannot read %s\n", L0.p.velfile.c_str());
+        return rc;
+    }
+    int model = 0;
+    rc = fdgpu_model_create(c, L0.p.nx, L0.p.ny, L0.p.nz, vel.data(), &model);
+    if (rc) return rc;
+    std::vector<fdgpu_table_desc> d(grp.size());
+    for (size_t t = 0; t < grp.size(); t++) {
+        memset(&d[t], 0, sizeof d[t]);
+        d[t].par = fdio::to_par(grp[t]->p);
+        d[t].model = model;
+        d[t].preset = grp[t]->preset.empty() ? nullptr : grp[t]->preset.data();
+        d[t].boxvals = grp[t]->boxvals.empty() ? nullptr : grp[t]->boxvals.data();
+    }
+    std::vector<int> ids(grp.size());
+    rc = fdgpu_solve(c, d.data(), (int)d.size(), mode, ids.data());
+    if (rc == 0) {
+        for (size_t t = 0; t < grp.size() && rc == 0; t++) {
+            const fdio::VHead h = fdio::table_header(grp[t]->p, grp[t]->g, fhead, has_head);
+            rc = fdgpu_table_write(c, ids[t], grp[t]->timefile.c_str(), &h);
+            if (rc) fprintf(stderr, "sphfd: cannot write %s\n", grp[t]->timefile.c_str());
+        }
+        for (int id : ids) fdgpu_table_release(c, id);
+    }
+    fdgpu_model_release(c, model);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int sphfd(int ac, char **av, char *output_path) {
+    Loaded L;
+    int rc = load_table(ac, av, output_path, L);
+    if (rc) {
+        fprintf(stderr, "sphfd: %s\n", fdgpu_strerror(rc));
+        return rc;
+    }
+    fdgpu_ctx *c = nullptr;
+    rc = shared_ctx(&c);
+    if (rc) return rc;
+    std::vector<Loaded *> grp{&L};
+    return solve_group(c, grp, FDGPU_PLAIN);
+}
+
+extern "C" int fdgpu_run_spec(const char *spec_path) {
+    // sphfd.c:207-244
+    std::string parlist, timedir = ".";
+    if (!fdio::get_vars(spec_path, "parlist", parlist)) {
+        fprintf(stderr, "Error on opening parlist in spec file(%s)\n", spec_path);
+        return FDGPU_EIO;
+    }
+    fdio::get_vars(spec_path, "timedir", timedir);
+    FILE *fl = fopen(parlist.c_str(), "r");
+    if (!fl) {
+        fprintf(stderr, "Error on opening parlist(%s)\n", parlist.c_str());
+        return FDGPU_EIO;
+    }
+    std::vector<std::string> lines;
+    char tmp[kMaxStrLen + 1];
+    while (fgets(tmp, kMaxStrLen + 1, fl)) {
+        if (tmp[0] == '\n') break;
+        std::string s(tmp);
+        if (!s.empty()) s.pop_back();   // the reference drops the last character of every line
+        lines.push_back(s);
+        if ((int)lines.size() > kMaxNumPar) {
+            fclose(fl);
+            fprintf(stderr, "number of parfiles exceed index\n");
+            return FDGPU_EINPUT;
+        }
+    }
+    fclose(fl);
+    std::vector<Loaded> tabs(lines.size());
+    std::map<std::tuple<std::string, int, int, int, int>, std::vector<Loaded *>> groups;
+    for (size_t t = 0; t < lines.size(); t++) {
+        char prog[] = "sphfd";
+        std::vector<char> arg(lines[t].begin(), lines[t].end());
+        arg.push_back('\0');
+        char *av[2] = {prog, arg.data()};
+        const int rc = load_table(2, av, timedir.c_str(), tabs[t]);
+        if (rc) {
+            fprintf(stderr, "sphfd: %s: %s\n", lines[t].c_str(), fdgpu_strerror(rc));
+            return rc;
+        }
+        const fdprep::ParValues &p = tabs[t].p;
+        groups[std::make_tuple(p.velfile, p.nx, p.ny, p.nz, p.swab)].push_back(&tabs[t]);
+    }
+    fdgpu_ctx *c = nullptr;
+    int rc = shared_ctx(&c);
+    if (rc) {
+        fprintf(stderr, "sphfd: %s\n", fdgpu_strerror(rc));
+        return rc;
+    }
+    for (auto &kv : groups) {
+        rc = solve_group(c, kv.second, FDGPU_BATCHED);
+        if (rc) {
+            fprintf(stderr, "sphfd: %s\n", fdgpu_strerror(rc));
+            return rc;
+        }
+    }
+    return 0;
+}

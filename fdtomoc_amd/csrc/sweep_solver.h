@@ -253,9 +253,15 @@ FD_HD void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo
     x.sync();
     int *cur = W.list0, *nxt = W.list1, *done = W.list2;
     int icur = 0, inxt = 1;
-    while (true) {
+    // every round finalises at least the lowest-ranked open node, so F rounds
+    // always suffice; the cap only guards against a broken invariant
+    for (int guard = 0;; guard++) {
         const int ncur = sh.n[icur];
         if (ncur == 0) break;
+        if (guard > 2 * Fn + 8) {
+            if (tid == 0) stats[kStatError] |= 8;
+            break;
+        }
         // phase B: evaluate ready nodes (statuses published by earlier rounds only)
         for (int it = tid; it < ncur; it += T) {
             const int p = cur[it];
@@ -348,6 +354,12 @@ FD_HD bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Ge
     for (int round = 1;; round++) {
         const int nwork = (round == 1) ? Fn : sh.n[0];
         if (nwork == 0) break;
+        // the evaluation DAG has depth <= F, so F + 1 rounds always suffice
+        if (round > 2 * Fn + 8) {
+            if (tid == 0) { stats[kStatError] |= 16; sh.flag |= 4; }
+            x.sync();
+            break;
+        }
         // evaluate: preceding neighbours from the grid, followers from the snapshot
         for (int it = tid; it < nwork; it += T) {
             const int p = (round == 1) ? it : W.list0[it];

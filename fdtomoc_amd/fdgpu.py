@@ -1,0 +1,193 @@
+"""Python binding of the sphfd GPU library (include/fdtomoc_amd.h).
+
+The compute path is the gfx950 HIP kernel in libfdtomoc_amd.so; this module
+only marshals arguments.  It raises if the library is not built or no GPU is
+present -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfdtomoc_amd.so")
+EXE_PATH = os.path.join(PKG, "bin", "sphfd")
+
+STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",
+              "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)]
+
+BATCHED, PLAIN = 0, 1
+
+
+class CPar(C.Structure):
+    """fdgpu_par: par-file values of one table (src/sphfd.c:383-530 keys)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("h", C.c_double), ("dq", C.c_double), ("df", C.c_double),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+                ("iminus", C.c_int), ("iplus", C.c_int), ("jminus", C.c_int),
+                ("jplus", C.c_int), ("kminus", C.c_int), ("kplus", C.c_int),
+                ("floatsrc", C.c_int), ("srctype", C.c_int), ("srcwall", C.c_int),
+                ("fxs", C.c_double), ("fys", C.c_double), ("fzs", C.c_double),
+                ("xs", C.c_int), ("ys", C.c_int), ("zs", C.c_int),
+                ("ncube", C.c_int), ("reverse", C.c_int), ("headpref", C.c_int),
+                ("parse", C.c_int), ("fill", C.c_int), ("maxoff", C.c_float),
+                ("invh", C.c_int), ("invq", C.c_int), ("invf", C.c_int),
+                ("invx0", C.c_int), ("invy0", C.c_int), ("invz0", C.c_int), ("swab", C.c_int)]
+
+
+class CTable(C.Structure):
+    _fields_ = [("par", CPar), ("model", C.c_int), ("ox", C.c_int), ("oy", C.c_int), ("oz", C.c_int),
+                ("preset", C.c_void_p), ("boxvals", C.c_void_p)]
+
+
+@dataclasses.dataclass
+class TableSpec:
+    """One station table: par values plus where it sits in a device model."""
+    par: dict
+    model: int
+    origin: tuple = (0, 0, 0)
+    preset: np.ndarray | None = None
+    boxvals: np.ndarray | None = None
+
+
+class FdgpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FdgpuError(f"{LIB_PATH} is not built; run `python -m fdtomoc_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.fdgpu_par_defaults.argtypes = [C.POINTER(CPar)]
+        L.fdgpu_par_defaults.restype = None
+        L.fdgpu_open.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]
+        L.fdgpu_close.argtypes = [C.c_void_p]
+        L.fdgpu_strerror.argtypes = [C.c_int]
+        L.fdgpu_strerror.restype = C.c_char_p
+        L.fdgpu_model_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
+        L.fdgpu_model_update.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.fdgpu_model_release.argtypes = [C.c_void_p, C.c_int]
+        L.fdgpu_solve.argtypes = [C.c_void_p, C.POINTER(CTable), C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.fdgpu_last_kernel_ms.argtypes = [C.c_void_p]
+        L.fdgpu_last_kernel_ms.restype = C.c_double
+        L.fdgpu_table_read.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_ulonglong]
+        L.fdgpu_table_write.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_void_p]
+        L.fdgpu_table_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.fdgpu_table_release.argtypes = [C.c_void_p, C.c_int]
+        L.fdgpu_table_device_ptr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.fdgpu_run_spec.argtypes = [C.c_char_p]
+        L.sphfd.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_char_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise FdgpuError(f"{what}: {lib().fdgpu_strerror(rc).decode()} ({rc})")
+
+
+def make_par(**kw) -> CPar:
+    p = CPar()
+    lib().fdgpu_par_defaults(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """A device context: resident models and tables on one GPU."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().fdgpu_open(device, hbm_budget, C.byref(self._h)), "fdgpu_open")
+
+    def close(self):
+        if self._h:
+            lib().fdgpu_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # models
+    def model(self, velocity: np.ndarray) -> int:
+        v = np.ascontiguousarray(velocity, dtype=np.float32)
+        nz, ny, nx = v.shape
+        mid = C.c_int()
+        _check(lib().fdgpu_model_create(self._h, nx, ny, nz, v.ctypes.data, C.byref(mid)), "fdgpu_model_create")
+        return mid.value
+
+    def update_model(self, model: int, velocity: np.ndarray):
+        v = np.ascontiguousarray(velocity, dtype=np.float32)
+        _check(lib().fdgpu_model_update(self._h, model, v.ctypes.data), "fdgpu_model_update")
+
+    def release_model(self, model: int):
+        _check(lib().fdgpu_model_release(self._h, model), "fdgpu_model_release")
+
+    # tables
+    def solve(self, tables: list[TableSpec], mode: int = BATCHED) -> list[int]:
+        n = len(tables)
+        arr = (CTable * max(n, 1))()
+        keep = []
+        for t, spec in enumerate(tables):
+            arr[t].par = make_par(**spec.par)
+            arr[t].model = spec.model
+            arr[t].ox, arr[t].oy, arr[t].oz = spec.origin
+            for name in ("preset", "boxvals"):
+                a = getattr(spec, name)
+                if a is not None:
+                    a = np.ascontiguousarray(a, dtype=np.float32)
+                    keep.append(a)
+                    setattr(arr[t], name, a.ctypes.data)
+        ids = (C.c_int * max(n, 1))()
+        _check(lib().fdgpu_solve(self._h, arr, n, mode, ids), "fdgpu_solve")
+        return list(ids[:n])
+
+    def last_kernel_ms(self) -> float:
+        return float(lib().fdgpu_last_kernel_ms(self._h))
+
+    def read(self, table: int, shape: tuple) -> np.ndarray:
+        out = np.empty(shape, dtype=np.float32)
+        _check(lib().fdgpu_table_read(self._h, table, out.ctypes.data, out.size), "fdgpu_table_read")
+        return out
+
+    def stats(self, table: int) -> dict:
+        s = np.zeros(len(STAT_NAMES), dtype=np.int32)
+        _check(lib().fdgpu_table_stats(self._h, table, s.ctypes.data, len(STAT_NAMES)), "fdgpu_table_stats")
+        return dict(zip(STAT_NAMES, s.tolist()))
+
+    def write(self, table: int, path: str, vel_header: bytes | None):
+        hb = None if vel_header is None else C.create_string_buffer(bytes(vel_header), 232)
+        _check(lib().fdgpu_table_write(self._h, table, path.encode(), hb), "fdgpu_table_write")
+
+    def release(self, table: int):
+        _check(lib().fdgpu_table_release(self._h, table), "fdgpu_table_release")
+
+    def device_ptr(self, table: int) -> int:
+        p = C.c_void_p()
+        _check(lib().fdgpu_table_device_ptr(self._h, table, C.byref(p)), "fdgpu_table_device_ptr")
+        return int(p.value)
+
+
+def run_spec(spec_path: str) -> None:
+    """The drop-in `sphfd` run of a spec file (src/sphfd.c:200-257), in process."""
+    _check(lib().fdgpu_run_spec(spec_path.encode()), "fdgpu_run_spec")

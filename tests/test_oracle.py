@@ -1,0 +1,218 @@
+"""The oracle (oracle/sphfd_oracle.c) pinned against the reference's own data,
+the reference compiled from its sources, and closed-form stencil answers."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from cases import make_case
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bits_diff(a, b):
+    return int((np.asarray(a, np.float32).view(np.uint32) != np.asarray(b, np.float32).view(np.uint32)).sum())
+
+
+def _golden_par(keys, vals):
+    kw = {}
+    ints = {"nx", "ny", "nz", "iminus", "iplus", "jminus", "jplus", "kminus", "kplus", "floatsrc", "srctype",
+            "srcwall", "xs", "ys", "zs", "ncube", "reverse", "headpref", "parse", "fill", "invh", "invq", "invf",
+            "invx0", "invy0", "invz0"}
+    for k, v in zip(keys, vals):
+        kw[str(k)] = int(v) if str(k) in ints else float(v)
+    return ob.Par(**kw)
+
+
+def test_m001_reference_fixture():
+    """verification/cmod/M001.ptimes (C output shipped with the reference) is
+    reproduced bit for bit from verification/cmod/tempvel."""
+    z = np.load(os.path.join(GOLDEN, "m001.npz"))
+    par = _golden_par(z["par_keys"], z["par_vals"])
+    t, st = ob.solve(par, ob.slowness(z["vel"]))
+    assert bits_diff(t, z["times"]) == 0
+    # header source = geocentric lon/colat/radius of the station (sphfd.c:5767-5769)
+    head = z["table_header"].tobytes()
+    fxs, fys, fzs = np.frombuffer(head[120:144], "<f8")
+    g = ob.geometry(par)
+    assert (g["fxss"], g["fyss"], g["fzss"]) == (fxs, fys, fzs)
+    assert head[12:16] == b"PTIM"
+
+
+def test_golden_reference_cases():
+    """Seeded cases solved by the compiled reference (tests/golden/make_golden.py)."""
+    z = np.load(os.path.join(GOLDEN, "ref_cases.npz"))
+    keys = z["par_keys"]
+    seeds = sorted({k.split("_")[0] for k in z.files if k.startswith("s")})
+    assert len(seeds) >= 40
+    restarts = 0
+    for s in seeds:
+        par = _golden_par(keys, z[f"{s}_par"])
+        t, st = ob.solve(par, ob.slowness(z[f"{s}_vel"]))
+        restarts += st["restarts"]
+        assert bits_diff(t, z[f"{s}_times"]) == 0, s
+    assert restarts > 0   # the set exercises reverse passes
+
+
+@pytest.mark.skipif(not os.path.exists(ob.REF_EXE), reason="reference not compiled here")
+@pytest.mark.parametrize("seed", range(3000, 3016))
+def test_live_reference(seed):
+    """Fresh seeds against the reference compiled from its own sources."""
+    par, v = make_case(seed)
+    _, t_ref, _ = ob.run_reference(par, v)
+    t, _ = ob.solve(par, ob.slowness(v))
+    assert bits_diff(t, t_ref) == 0
+
+
+# ----------------------------------------------------------------- stencils
+
+def _arr(vals, n=8):
+    a = (C.c_double * n)()
+    for i, v in enumerate(vals):
+        a[i] = v
+    return a
+
+
+def _iarr(vals):
+    a = (C.c_int * 8)()
+    for i, v in enumerate(vals):
+        a[i] = v
+    return a
+
+
+def test_2d_stencil_planar_exact():
+    """fdsph2d (sphfd.c:6077-6143) is exact for a planar field on a rectangle."""
+    L = ob.lib()
+    L.oracle_fdsph2d.argtypes = [C.c_void_p, C.c_void_p] + [C.c_double] * 4
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        d01, d02 = rng.uniform(0.3, 3.0, 2)
+        slo = rng.uniform(0.1, 0.4)
+        th = rng.uniform(0.05, math.pi / 2 - 0.05)
+        g1, g2 = -slo * math.cos(th), -slo * math.sin(th)   # wave arrives from the far corner
+        x = rng.uniform(5, 50)
+        t1, t2, t3 = x + g1 * d01, x + g2 * d02, x + g1 * d01 + g2 * d02
+        t = _arr([0.0, t1, t2, t3, 0.0])
+        s = _arr([slo] * 4)
+        got = L.oracle_fdsph2d(t, s, d01, d02, d02, d01)
+        assert abs(got - x) < 1e-9 * x
+        # acausal: a neighbour later than the answer rejects both roots
+        t4 = _arr([x + 1.0, t1, t2, t3, 0.0])
+        assert L.oracle_fdsph2d(t4, s, d01, d02, d02, d01) == 1e11
+
+
+def test_new_edge_and_face_stencils_planar_exact():
+    L = ob.lib()
+    L.oracle_fdsphne.argtypes = [C.c_void_p, C.c_void_p] + [C.c_double] * 5
+    L.oracle_fdsphnf.argtypes = [C.c_void_p, C.c_void_p] + [C.c_double] * 3
+    rng = np.random.default_rng(2)
+    done = 0
+    while done < 200:
+        slo = rng.uniform(0.1, 0.4)
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        gn, gb, gc = -abs(v[0]) * slo, -abs(v[1]) * slo, v[2] * slo   # n: 2->5, b: 4->5
+        d01, d12, d25 = rng.uniform(0.3, 3.0, 3)
+        # keep every known point earlier than the node (a causal configuration)
+        if abs(gc) * d01 > 0.5 * (abs(gn) * d25 + abs(gb) * d12) or abs(gb) * d01 > 0.5 * abs(gn) * d25 \
+                or abs(gc) * d12 > 0.5 * abs(gn) * d25:
+            continue
+        done += 1
+        d14, d45 = d25, d12
+        x = rng.uniform(5, 50)
+        # fdsphne geometry: 5 node, 2 = 5 - n*d25, 4 = 5 - b*d45, 1 = 2 - b*d12, 0/3 = 1 +- c*d01
+        t2 = x + gn * d25
+        t4 = x + gb * d45
+        t1 = t2 + gb * d12
+        t0, t3 = t1 + gc * d01, t1 - gc * d01
+        t = _arr([t0, t1, t2, t3, t4])
+        s = _arr([slo] * 6)
+        got = L.oracle_fdsphne(t, s, d01, d12, d14, d25, d45)
+        assert abs(got - x) < 1e-9 * x
+        # fdsphnf geometry: 2 below node at d25, 0/4 and 1/3 straddle 2 at d02, d12
+        d02 = d01
+        t2 = x + gn * d25
+        tn0, tn4 = t2 - gb * d02, t2 + gb * d02
+        tn1, tn3 = t2 + gc * d12, t2 - gc * d12
+        tt = _arr([tn0, tn1, t2, tn3, tn4])
+        got = L.oracle_fdsphnf(tt, s, d02, d12, d25)
+        assert abs(got - x) < 1e-9 * x
+
+
+def test_3d_stencil_cartesian_limit():
+    """fdsph3d (sphfd.c:5906-6013) on a tiny spherical cell far from the centre
+    is a Cartesian cube, where the 8-point stencil is exact for planar fields."""
+    L = ob.lib()
+    L.oracle_fdsph3d.argtypes = [C.c_void_p] * 7 + [C.c_double] * 3
+    rng = np.random.default_rng(3)
+    R, q0 = 1.0e7, math.pi / 2
+    h = 1.0
+    dq = h / R
+    df = h / (R * math.sin(q0))
+    for _ in range(100):
+        slo = rng.uniform(0.1, 0.4)
+        v = rng.normal(size=3)
+        g = -np.abs(v) / np.linalg.norm(v) * slo     # gradient pointing back to the far corner
+        x = rng.uniform(5, 50)
+        # point 7 is the node (0,0,0); points 0..6 the other cube corners (unit cell)
+        corners = [(1, 1, 1), (0, 1, 1), (0, 0, 1), (1, 0, 1), (1, 1, 0), (0, 1, 0), (1, 0, 0)]
+        t = [x + g @ np.array(c, float) for c in corners]
+        # c = (df step, dq step, radial step) -> radius decreases with the radial index
+        r = [R - c[2] * h for c in corners] + [R]
+        q = [q0 + c[1] * dq for c in corners] + [q0]
+        gsg = [-1 if c[2] == 1 else 1 for c in corners] + [1]
+        nsg = [1 if c[1] == 1 else -1 for c in corners] + [-1]
+        msg = [1 if c[0] == 1 else -1 for c in corners] + [-1]
+        got = L.oracle_fdsph3d(_arr(t), _arr([slo] * 8), _arr(r), _arr(q), _iarr(gsg), _iarr(nsg), _iarr(msg),
+                               C.c_double(h), C.c_double(dq), C.c_double(df))
+        assert abs(got - x) < 1e-5 * x
+
+
+def test_glath_closed_forms():
+    """glath (sphfd.c:5836-5893) at zero height is tan(geocentric) = (1-e^2) tan(geodetic);
+    the radius is a at the equator and b at the pole."""
+    L = ob.lib()
+    r = C.c_double()
+    e2 = 0.00669437978616733379
+    a = 6378.137
+    for lat in np.linspace(-89.0, 89.0, 37):
+        phi = math.radians(lat)
+        g = L.oracle_glath(phi, 0.0, C.byref(r))
+        assert abs(g - math.atan((1 - e2) * math.tan(phi))) < 1e-14
+    assert abs(L.oracle_glath(0.0, 0.0, C.byref(r))) < 1e-15 and abs(r.value - a) < 1e-9
+    L.oracle_glath(math.pi / 2, 0.0, C.byref(r))
+    assert abs(r.value - a * math.sqrt(1 - e2)) < 1e-6
+    # 10 km below the ellipsoid at the equator: radius a - 10
+    L.oracle_glath(0.0, 10.0, C.byref(r))
+    assert abs(r.value - (a - 10.0)) < 1e-9
+
+
+# --------------------------------------------------------- whole-solve physics
+
+def test_homogeneous_medium_close_to_chord_time():
+    """Constant velocity: FD times approach the straight-chord time (FD error
+    of a few percent of a cell transit at most), are causal and zero only at
+    the source."""
+    from fdtomoc_amd import synth
+    g = synth.centred_grid(33, 33, 17, 1.0)
+    v = np.full((g.nz, g.ny, g.nx), 6.0, np.float32)
+    lon, lat = g.node_lonlat(16, 16)
+    par = ob.Par(nx=g.nx, ny=g.ny, nz=g.nz, h=1.0, x0=g.x0, y0=g.y0, z0=0.0, fxs=round(lon, 5),
+                 fys=round(lat, 5), fzs=0.0, reverse=50)
+    t, st = ob.solve(par, ob.slowness(v))
+    geo = ob.geometry(par)
+    k, j, i = np.meshgrid(np.arange(g.nz), np.arange(g.ny), np.arange(g.nx), indexing="ij")
+    r = geo["z0r"] - geo["h"] * k
+    q = geo["y0"] + geo["dq"] * j
+    f = geo["x0"] + geo["df"] * i
+    P = np.stack([r * np.cos(q), r * np.sin(q) * np.cos(f), r * np.sin(q) * np.sin(f)])
+    s = geo["fzss"]
+    S = np.array([s * math.cos(geo["fyss"]), s * math.sin(geo["fyss"]) * math.cos(geo["fxss"]),
+                  s * math.sin(geo["fyss"]) * math.sin(geo["fxss"])])
+    exact = np.linalg.norm(P - S[:, None, None, None], axis=0) / 6.0
+    err = np.abs(t - exact)
+    assert err.max() < 0.05          # seconds; cell transit is 1/6 s
+    assert (t >= 0).all() and (t < 1e9).all()

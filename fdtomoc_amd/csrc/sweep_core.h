@@ -20,6 +20,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define FD_HD __host__ __device__
@@ -51,6 +53,7 @@ constexpr double kHeadTest = 1.e-3; // sphfd.c:360
 FD_INLINE double st_3d(const double t[8], const double s[8], const double r[8], const double sq[8],
                        const int g[8], const int n[8], const int m[8], double h, double dq, double df) {
     double slo = 0.;
+#pragma unroll
     for (int i = 0; i < 8; i++) slo += s[i];
     slo = .125 * slo;
     const double hsq = h * h, dqsq = dq * dq, dfsq = df * df;
@@ -58,18 +61,22 @@ FD_INLINE double st_3d(const double t[8], const double s[8], const double r[8], 
     const double df2sinq7 = df * dfsinq7;
     const double a = 1.0 / hsq + (1.0 / dqsq + 1. / (dfsinq7 * dfsinq7)) / (r[7] * r[7]);
     double b = 0;
+#pragma unroll
     for (int i = 0; i < 7; i++)
         b += t[i] * ((double)(g[7] * g[i]) / hsq +
                      ((double)(n[7] * n[i]) / dqsq + (double)(m[7] * m[i]) / (df2sinq7 * sq[i])) / (r[7] * r[i]));
     b = 2.0 * b;
     double c = 0.;
+#pragma unroll
     for (int i = 0; i < 7; i++) {
         const double dfsinqi = df * sq[i];
         c += t[i] * t[i] * (1. / hsq + (1. / dqsq + 1. / (dfsinqi * dfsinqi)) / (r[i] * r[i]));
     }
+#pragma unroll
     for (int i = 0; i < 6; i++) {
         double c2 = 0.;
         const double df2sinqi = dfsq * sq[i];
+#pragma unroll
         for (int j = i + 1; j < 7; j++)
             c2 += t[j] * ((double)(g[i] * g[j]) / hsq +
                           ((double)(n[i] * n[j]) / dqsq + (double)(m[i] * m[j]) / (df2sinqi * sq[j])) / (r[i] * r[j]));
@@ -82,6 +89,7 @@ FD_INLINE double st_3d(const double t[8], const double s[8], const double r[8], 
     const double x1 = (-b + sr) / (2.0 * a);
     const double x2 = (-b - sr) / (2.0 * a);
     bool bad1 = false, bad2 = false;
+#pragma unroll
     for (int i = 0; i < 7; i++) {
         bad1 |= (x1 < t[i]);
         bad2 |= (x2 < t[i]);
@@ -107,6 +115,7 @@ FD_INLINE double st_ne(const double t[8], const double s[8], double d01, double 
     const double x1 = (-b + sr) / (2 * a);
     const double x2 = (-b - sr) / (2 * a);
     bool bad1 = false, bad2 = false;
+#pragma unroll
     for (int i = 0; i < 5; i++) {
         bad1 |= (x1 < t[i]);
         bad2 |= (x2 < t[i]);
@@ -501,181 +510,233 @@ struct Geo {
 };
 
 FD_INLINE double rc(const Geo &G, int k) { return G.z0r - G.h * (k); }
-FD_INLINE double qcol(const Geo &G, int j) { return G.y0 + G.dq * (j); }
-FD_INLINE double sinq(const Geo &G, int j) { return G.sinq[j + 1]; }
 
-FD_INLINE double dist_of(const Geo &G, DS d, int j, int k) {
-    return d.form == DH ? G.h : d.form == DRDQ ? rc(G, k + d.dz) * G.dq : rc(G, k + d.dz) * sinq(G, j + d.dy) * G.df;
+// The 3x3x2 neighbourhood of a node: times and slownesses of the face plane
+// (plane 0) and of the inner layer (plane 1), indexed plane*9 + (db+1)*3 + (da+1)
+// along the face's in-plane axes (a, b).  For in-face neighbours the caller
+// stores the final value of a neighbour that precedes the node in the face
+// order and the pre-face value of one that follows it.  Entries outside the
+// grid hold arbitrary values; no trial whose bounds pass reads them.
+struct Nbhd {
+    float t[18];
+    float s[18];
+    double sq[3];   // sin(qc(j-1)), sin(qc(j)), sin(qc(j+1))
+};
+
+template <int F>
+constexpr int nb_index(int dx, int dy, int dz) {
+    using S = FaceSpec<F>;
+    const int o[3] = {dx, dy, dz};
+    const int dn = o[S::nax], da = o[S::a_ax], db = o[S::b_ax];
+    return (dn == 0 ? 0 : 9) + (db + 1) * 3 + (da + 1);
 }
-FD_INLINE double factor_of(const Geo &G, DS f, int j, int k) {
-    return f.form == XH ? G.h : f.form == XR ? rc(G, k + f.dz) : f.form == XDQ ? G.dq : f.form == XDF ? G.df : sinq(G, j + f.dy);
+
+template <int I, int N, class Fn>
+FD_INLINE void static_for(Fn &&fn) {
+    if constexpr (I < N) {
+        fn(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(fn);
+    }
 }
 
 // Result of one node's trial sequence.
 struct NodeOut {
-    float tnew;        // value to store when `stored`
+    float tnew;          // value to store when `stored`
     float tout0, tout4;  // t[0], t[4] as the node leaves them (when `sets`)
-    uint8_t stored;    // guess < old value
-    uint8_t head;      // stored and fhead > 1e-3
-    uint8_t selfset;   // t[0]/t[4] written by this node before its 2D trials
-    uint8_t sets;      // t[0]/t[4] written by this node at all
-    uint8_t used_in;   // a 2D trial ran on the carried t[0]/t[4]
+    uint8_t stored;      // guess < old value
+    uint8_t head;        // stored and fhead > 1e-3
+    uint8_t selfset;     // t[0]/t[4] written by this node before its 2D trials
+    uint8_t sets;        // t[0]/t[4] written by this node at all
+    uint8_t used_in;     // a 2D trial ran on the carried t[0]/t[4]
 };
 
-// Accessor contract:
-//   float t(dx,dy,dz)  time at a neighbour; for in-face neighbours the caller
-//                       returns the final value if it precedes the node in the
-//                       face order and the pre-face value otherwise.
-//   float s(dx,dy,dz)  slowness.
-template <int F, class Acc>
-FD_INLINE bool bounds_ok(const Trial &T, const Geo &G, const int c[3]) {
+template <int F, int TI>
+FD_INLINE bool bounds_ok(const Geo &G, int i, int j, int k) {
+    constexpr Trial T = FaceSpec<F>::trials[TI];
+    const int c[3] = {i, j, k};
     const int n3[3] = {G.nx, G.ny, G.nz};
-    if (T.bax0 >= 0 && !(T.bdir0 > 0 ? (c[T.bax0] < n3[T.bax0] - 1) : (c[T.bax0] > 0))) return false;
-    if (T.bax1 >= 0 && !(T.bdir1 > 0 ? (c[T.bax1] < n3[T.bax1] - 1) : (c[T.bax1] > 0))) return false;
-    if (T.iax0 >= 0 && !(c[T.iax0] > G.box[T.iax0][0] + 1 && c[T.iax0] < G.box[T.iax0][1] - 1)) return false;
-    if (T.iax1 >= 0 && !(c[T.iax1] > G.box[T.iax1][0] + 1 && c[T.iax1] < G.box[T.iax1][1] - 1)) return false;
+    if constexpr (T.bax0 >= 0) {
+        if (!(T.bdir0 > 0 ? (c[T.bax0] < n3[T.bax0] - 1) : (c[T.bax0] > 0))) return false;
+    }
+    if constexpr (T.bax1 >= 0) {
+        if (!(T.bdir1 > 0 ? (c[T.bax1] < n3[T.bax1] - 1) : (c[T.bax1] > 0))) return false;
+    }
+    if constexpr (T.iax0 >= 0) {
+        if (!(c[T.iax0] > G.box[T.iax0][0] + 1 && c[T.iax0] < G.box[T.iax0][1] - 1)) return false;
+    }
+    if constexpr (T.iax1 >= 0) {
+        if (!(c[T.iax1] > G.box[T.iax1][0] + 1 && c[T.iax1] < G.box[T.iax1][1] - 1)) return false;
+    }
     return true;
 }
 
-template <int F, class Acc>
-FD_INLINE bool gates_ok(const Trial &T, const Acc &A) {
-    for (int g = 0; g < T.ngate; g++) {
-        const float v = A.t(T.gate[g].dx, T.gate[g].dy, T.gate[g].dz);
-        const bool ok = (T.gtest[g] == GNZ) ? (v != 0.0f) : ((double)v < kBig);
-        if (!ok) return false;
-    }
-    return true;
+template <int F, int TI>
+FD_INLINE bool gates_ok(const Nbhd &N) {
+    constexpr Trial T = FaceSpec<F>::trials[TI];
+    bool ok = true;
+    static_for<0, T.ngate>([&](auto G) {
+        constexpr P3 g = T.gate[G];
+        const float v = N.t[nb_index<F>(g.dx, g.dy, g.dz)];
+        ok = ok && ((T.gtest[G] == GNZ) ? (v != 0.0f) : ((double)v < kBig));
+    });
+    return ok;
+}
+
+template <int F, int TI>
+FD_INLINE bool trial_open(const Geo &G, const Nbhd &N, int i, int j, int k) {
+    return bounds_ok<F, TI>(G, i, j, k) && gates_ok<F, TI>(N);
+}
+
+FD_INLINE double sinq_of(const Nbhd &N, int dy) { return N.sq[dy + 1]; }
+
+FD_INLINE double dist_of(const Geo &G, const Nbhd &N, DS d, int k) {
+    return d.form == DH ? G.h : d.form == DRDQ ? rc(G, k + d.dz) * G.dq : rc(G, k + d.dz) * sinq_of(N, d.dy) * G.df;
+}
+FD_INLINE double factor_of(const Geo &G, const Nbhd &N, DS f, int k) {
+    return f.form == XH ? G.h : f.form == XR ? rc(G, k + f.dz) : f.form == XDQ ? G.dq : f.form == XDF ? G.df
+                                                                                             : sinq_of(N, f.dy);
 }
 
 // Whether the node will run a 2D trial on carried t[0]/t[4]: it depends only
 // on gates and on the node's own pre-face value (the new-edge block runs only
 // if guess > 1e9, and with no 3D trial guess is still the pre-face value).
-template <int F, class Acc>
-FD_HD bool needs_carry(const Acc &A, const Geo &G, int i, int j, int k, float told) {
-    using S = FaceSpec<F>;
-    const int c[3] = {i, j, k};
+template <int F>
+FD_INLINE bool needs_carry(const Nbhd &N, const Geo &G, int i, int j, int k, float told) {
     bool any3 = false, anyne = false, any2 = false;
-    for (int ti = 0; ti < 22; ti++) {
-        const Trial &T = S::trials[ti];
-        if (T.kind == KNF || T.kind == K1D) continue;
-        if (!bounds_ok<F, Acc>(T, G, c) || !gates_ok<F, Acc>(T, A)) continue;
-        if (T.kind == K3D) any3 = true;
-        else if (T.kind == KNE) anyne = true;
-        else any2 = true;
-    }
+    static_for<0, 22>([&](auto TI) {
+        constexpr Trial T = FaceSpec<F>::trials[TI];
+        if constexpr (T.kind == K3D || T.kind == KNE || T.kind == K2D) {
+            if (trial_open<F, TI>(G, N, i, j, k)) {
+                if constexpr (T.kind == K3D) any3 = true;
+                else if constexpr (T.kind == KNE) anyne = true;
+                else any2 = true;
+            }
+        }
+    });
     const bool self = any3 || ((double)told > kBig && anyne);
     return any2 && !self;
 }
 
-// The node's full trial sequence (e.g. TOP sphfd.c:1035-1760).  tin0/tin4 are
-// the t[0]/t[4] left by earlier nodes; used only if out.used_in.
-template <int F, class Acc>
-FD_HD void eval_node(const Acc &A, const Geo &G, int i, int j, int k, float told, float slow, float tin0,
-                     float tin4, NodeOut &out) {
-    using S = FaceSpec<F>;
-    const int c[3] = {i, j, k};
-    double t[8];
-    double s[8];
-    t[0] = tin0;
-    t[4] = tin4;
-    bool own04 = false, self = false;
+// The node's full trial sequence (e.g. TOP sphfd.c:1035-1760), on registers.
+// tin0/tin4 are the t[0]/t[4] left by earlier nodes; used only if out.used_in.
+template <int F>
+FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float told, float tin0, float tin4,
+                     NodeOut &out) {
+    const float slow = N.s[nb_index<F>(0, 0, 0)];
+    double t0 = tin0, t4 = tin4;   // the carried registers
+    bool own04 = false, self = false, used_in = false;
     double guess = told;
     double fhead = 0.;
     bool ne_open = false;
-    out.used_in = 0;
-#pragma unroll
-    for (int ti = 0; ti < 22; ti++) {
-        const Trial &T = S::trials[ti];
-        if (ti == 4) ne_open = guess > kBig;   // the new-edge block guard is evaluated once
-        if (T.kind == KNE && !ne_open) continue;
-        if (T.kind == KNF && !(guess > kBig)) continue;
-        if (!bounds_ok<F, Acc>(T, G, c) || !gates_ok<F, Acc>(T, A)) continue;
-        double tr;
-        if (T.kind == K3D) {
-            double r[8], sq[8];
+    static_for<0, 22>([&](auto TI) {
+        constexpr Trial T = FaceSpec<F>::trials[TI];
+        if constexpr (TI == 4) ne_open = guess > kBig;   // the new-edge block guard is evaluated once
+        if constexpr (T.kind == KNE) {
+            if (!ne_open) return;
+        }
+        if constexpr (T.kind == KNF) {
+            if (!(guess > kBig)) return;
+        }
+        if (!trial_open<F, TI>(G, N, i, j, k)) return;
+        if constexpr (T.kind == K3D) {
+            double t[8], s[8], r[8], sq[8];
             int g[8], n[8], m[8];
-            int zmax = 0, ymax = 0, xmax = 0;
-            for (int p = 0; p < 7; p++) {
-                zmax = T.pt[p].dz > zmax ? T.pt[p].dz : zmax;
-                ymax = T.pt[p].dy > ymax ? T.pt[p].dy : ymax;
-                xmax = T.pt[p].dx > xmax ? T.pt[p].dx : xmax;
-            }
-            for (int p = 0; p < 8; p++) {
-                const P3 o = p < 7 ? T.pt[p] : P3{0, 0, 0};
-                if (p < 7) {
-                    t[p] = A.t(o.dx, o.dy, o.dz);
-                    s[p] = A.s(o.dx, o.dy, o.dz);
-                } else {
-                    s[p] = slow;
-                }
-                r[p] = rc(G, k + o.dz);
-                sq[p] = sinq(G, j + o.dy);
+            constexpr int zmax = (T.pt[0].dz > 0 || T.pt[1].dz > 0 || T.pt[2].dz > 0 || T.pt[3].dz > 0 ||
+                                  T.pt[4].dz > 0 || T.pt[5].dz > 0 || T.pt[6].dz > 0) ? 1 : 0;
+            constexpr int ymax = (T.pt[0].dy > 0 || T.pt[1].dy > 0 || T.pt[2].dy > 0 || T.pt[3].dy > 0 ||
+                                  T.pt[4].dy > 0 || T.pt[5].dy > 0 || T.pt[6].dy > 0) ? 1 : 0;
+            constexpr int xmax = (T.pt[0].dx > 0 || T.pt[1].dx > 0 || T.pt[2].dx > 0 || T.pt[3].dx > 0 ||
+                                  T.pt[4].dx > 0 || T.pt[5].dx > 0 || T.pt[6].dx > 0) ? 1 : 0;
+            static_for<0, 8>([&](auto P) {
+                constexpr P3 o = P < 7 ? T.pt[P] : P3{0, 0, 0};
+                constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
+                if constexpr (P < 7) t[P] = N.t[q];
+                s[P] = N.s[q];
+                r[P] = rc(G, k + o.dz);
+                sq[P] = sinq_of(N, o.dy);
                 // +1 on the cell's smaller-z layer, larger-y row, larger-x column
-                g[p] = (o.dz == zmax) ? -1 : 1;
-                n[p] = (o.dy == ymax) ? 1 : -1;
-                m[p] = (o.dx == xmax) ? 1 : -1;
-            }
-            tr = st_3d(t, s, r, sq, g, n, m, G.h, G.dq, G.df);
+                g[P] = (o.dz == zmax) ? -1 : 1;
+                n[P] = (o.dy == ymax) ? 1 : -1;
+                m[P] = (o.dx == xmax) ? 1 : -1;
+            });
+            const double tr = st_3d(t, s, r, sq, g, n, m, G.h, G.dq, G.df);
+            t0 = t[0];
+            t4 = t[4];
             own04 = true;
             self = true;
             if (tr < guess) guess = tr;
-        } else if (T.kind == KNE) {
-            for (int p = 0; p < 5; p++) {
-                t[p] = A.t(T.pt[p].dx, T.pt[p].dy, T.pt[p].dz);
-                s[p] = A.s(T.pt[p].dx, T.pt[p].dy, T.pt[p].dz);
-            }
+        } else if constexpr (T.kind == KNE) {
+            double t[8], s[8];
+            static_for<0, 5>([&](auto P) {
+                constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
+                t[P] = N.t[q];
+                s[P] = N.s[q];
+            });
             s[5] = slow;
-            tr = st_ne(t, s, dist_of(G, T.d[0], j, k), dist_of(G, T.d[1], j, k), dist_of(G, T.d[2], j, k),
-                       dist_of(G, T.d[3], j, k), dist_of(G, T.d[4], j, k));
+            const double tr = st_ne(t, s, dist_of(G, N, T.d[0], k), dist_of(G, N, T.d[1], k), dist_of(G, N, T.d[2], k),
+                                    dist_of(G, N, T.d[3], k), dist_of(G, N, T.d[4], k));
+            t0 = t[0];
+            t4 = t[4];
             own04 = true;
             self = true;
             if (tr < guess) guess = tr;
-        } else if (T.kind == K2D) {
+        } else if constexpr (T.kind == K2D) {
+            double t[8], s[8];
+            t[0] = t0;
+            t[4] = t4;
             s[0] = slow;
-            for (int p = 0; p < 3; p++) {
-                t[p + 1] = A.t(T.pt[p].dx, T.pt[p].dy, T.pt[p].dz);
-                s[p + 1] = A.s(T.pt[p].dx, T.pt[p].dy, T.pt[p].dz);
-            }
-            if (!own04) out.used_in = 1;
-            const double d01 = dist_of(G, T.d[0], j, k);
-            tr = st_2d(t, s, d01, dist_of(G, T.d[1], j, k), dist_of(G, T.d[2], j, k), dist_of(G, T.d[3], j, k));
+            static_for<0, 3>([&](auto P) {
+                constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
+                t[P + 1] = N.t[q];
+                s[P + 1] = N.s[q];
+            });
+            if (!own04) used_in = true;
+            const double d01 = dist_of(G, N, T.d[0], k);
+            const double tr = st_2d(t, s, d01, dist_of(G, N, T.d[1], k), dist_of(G, N, T.d[2], k), dist_of(G, N, T.d[3], k));
             if (tr < guess) {
-                if (T.fhead) fhead = (guess - tr) / (d01 * (double)slow);
+                if constexpr (T.fhead) fhead = (guess - tr) / (d01 * (double)slow);
                 guess = tr;
             }
-        } else if (T.kind == KNF) {
-            for (int p = 0; p < 5; p++) {
-                t[p] = A.t(T.pt[p].dx, T.pt[p].dy, T.pt[p].dz);
-                s[p] = A.s(T.pt[p].dx, T.pt[p].dy, T.pt[p].dz);
-            }
+        } else if constexpr (T.kind == KNF) {
+            double t[8], s[8];
+            static_for<0, 5>([&](auto P) {
+                constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
+                t[P] = N.t[q];
+                s[P] = N.s[q];
+            });
             s[5] = slow;
-            tr = st_nf(t, s, dist_of(G, T.d[0], j, k), dist_of(G, T.d[1], j, k), dist_of(G, T.d[2], j, k));
+            const double tr = st_nf(t, s, dist_of(G, N, T.d[0], k), dist_of(G, N, T.d[1], k), dist_of(G, N, T.d[2], k));
+            t0 = t[0];
+            t4 = t[4];
             own04 = true;
             if (tr < guess) guess = tr;
         } else {  // K1D
-            const P3 o = T.pt[0];
-            double step = .5 * (slow + A.s(o.dx, o.dy, o.dz));   // float sum, then double
+            constexpr P3 o = T.pt[0];
+            constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
+            double step = .5 * (slow + N.s[q]);   // float sum, then double
             double den = 1.0;
-            for (int f = 0; f < 3; f++) {
-                if (T.fac[f].form == XNONE) break;
-                const double fv = factor_of(G, T.fac[f], j, k);
-                step = step * fv;
-                den = (f == 0) ? fv : den * fv;
-            }
-            tr = (double)A.t(o.dx, o.dy, o.dz) + step;
+            static_for<0, 3>([&](auto Fi) {
+                if constexpr (T.fac[Fi].form != XNONE) {
+                    const double fv = factor_of(G, N, T.fac[Fi], k);
+                    step = step * fv;
+                    den = (Fi == 0) ? fv : den * fv;
+                }
+            });
+            const double tr = (double)N.t[q] + step;
             if (tr < guess) {
-                if (T.fhead) fhead = (guess - tr) / (den * (double)slow);
+                if constexpr (T.fhead) fhead = (guess - tr) / (den * (double)slow);
                 guess = tr;
             }
         }
-    }
+    });
     out.stored = guess < (double)told;
     out.tnew = out.stored ? (float)guess : told;
     out.head = out.stored && fhead > kHeadTest;
     out.selfset = self;
     out.sets = own04;
-    out.tout0 = (float)t[0];
-    out.tout4 = (float)t[4];
+    out.used_in = used_in;
+    out.tout0 = (float)t0;
+    out.tout4 = (float)t4;
 }
 
 }  // namespace fdsweep

@@ -135,42 +135,54 @@ FD_INLINE void inface(int dx, int dy, int dz, int &dn, int &da, int &db) {
     else { dn = dy; da = dx; db = dz; }
 }
 
-// Accessor: CHANGE mode reads the pre-face snapshot for in-face neighbours that
-// follow the node; LEVEL mode reads the grid (followers are untouched there).
+// Loads a node's 3x3x2 neighbourhood.  CHG: in-face neighbours that follow the
+// node in face order contribute their pre-face value from the snapshot O, the
+// others the grid (tentative or final).  LEVEL reads the grid only: there the
+// followers have not been touched yet.  Coordinates are clamped into the grid;
+// trials never use a clamped entry because their bounds checks fail first.
 template <int F, bool CHG>
-struct Acc {
-    const float *time;
-    const float *slow;
-    long long node, snode;
-    long long nx, nxy, s_sy, s_sz;
-    int a, b, p;
-    unsigned kp;
-    const FaceGeom *fg;
-    const float *O;
-    const unsigned *key;
-    FD_INLINE float t(int dx, int dy, int dz) const {
-        if (CHG) {
-            int dn, da, db;
-            inface<F>(dx, dy, dz, dn, da, db);
-            if (dn == 0) {
-                const int aa = a + da, bb = b + db;
-                if (aa >= 0 && bb >= 0 && aa < fg->wa && bb < fg->wb) {
-                    const int q = bb * fg->wa + aa;
-                    if (!key_less(key[q], q, kp, p)) return O[q];
+FD_INLINE void load_nbhd(const TableJob &J, const Geo &G, const FaceGeom &fg, const float *O, const unsigned *key,
+                         int p, int a, int b, int i, int j, int k, Nbhd &N) {
+    using S = FaceSpec<F>;
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    const unsigned kp = CHG ? key[p] : 0u;
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+        for (int db = -1; db <= 1; db++)
+#pragma unroll
+            for (int da = -1; da <= 1; da++) {
+                int o[3] = {0, 0, 0};
+                o[S::nax] = pl ? S::ndir : 0;
+                o[S::a_ax] = da;
+                o[S::b_ax] = db;
+                int ci = i + o[0], cj = j + o[1], ck = k + o[2];
+                ci = ci < 0 ? 0 : (ci >= J.nx ? J.nx - 1 : ci);
+                cj = cj < 0 ? 0 : (cj >= J.ny ? J.ny - 1 : cj);
+                ck = ck < 0 ? 0 : (ck >= J.nz ? J.nz - 1 : ck);
+                const int q = pl * 9 + (db + 1) * 3 + (da + 1);
+                float tv = J.time[(long long)ck * nxy + (long long)cj * nx + ci];
+                if (CHG && pl == 0 && (da | db)) {
+                    const int aa = a + da, bb = b + db;
+                    if (aa >= 0 && bb >= 0 && aa < fg.wa && bb < fg.wb) {
+                        const int fq = bb * fg.wa + aa;
+                        if (!key_less(key[fq], fq, kp, p)) tv = O[fq];
+                    }
                 }
+                N.t[q] = tv;
+                N.s[q] = J.slow[(long long)ck * J.s_sz + (long long)cj * J.s_sy + ci];
             }
-        }
-        return time[node + dx + dy * nx + dz * nxy];
-    }
-    FD_INLINE float s(int dx, int dy, int dz) const { return slow[snode + dx + dy * s_sy + dz * s_sz]; }
-};
+    N.sq[0] = G.sinq[j];
+    N.sq[1] = G.sinq[j + 1];
+    N.sq[2] = G.sinq[j + 2];
+}
 
 // ------------------------------------------------------------ radix sort
 
 // Stable LSD sort of F (key, val) pairs by key, 4-bit digits, skipping digits
 // on which all keys agree.  Returns true if the result is in (key2, val2).
 template <class X>
-FD_HD bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int *v1, int F) {
+FD_INLINE bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int *v1, int F) {
     const int T = x.nthr(), tid = x.tid();
     const int chunk = (F + T - 1) / T;
     const int lo = tid * chunk < F ? tid * chunk : F;
@@ -179,10 +191,9 @@ FD_HD bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int
     for (int d = 0; d < 8; d++) {
         const unsigned vary = (sh.kor ^ sh.kand) >> (4 * d);
         if ((vary & 15u) == 0) continue;
-        int cnt[16];
-        for (int q = 0; q < 16; q++) cnt[q] = 0;
-        for (int e = lo; e < hi; e++) cnt[(k0[e] >> (4 * d)) & 15u]++;
-        for (int q = 0; q < 16; q++) sh.hist[q * T + tid] = cnt[q];
+        // per-thread digit counts live in this thread's column of sh.hist
+        for (int q = 0; q < 16; q++) sh.hist[q * T + tid] = 0;
+        for (int e = lo; e < hi; e++) sh.hist[((k0[e] >> (4 * d)) & 15u) * T + tid]++;
         x.sync();
         // exclusive scan over 16*T counters in digit-major order: each thread
         // owns 16 consecutive counters
@@ -206,12 +217,10 @@ FD_HD bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int
             run += v;
         }
         x.sync();
-        int off[16];
-        for (int q = 0; q < 16; q++) off[q] = sh.hist[q * T + tid];
         for (int e = lo; e < hi; e++) {
             const unsigned kk = k0[e];
             const int dg = (kk >> (4 * d)) & 15u;
-            const int pos = off[dg]++;
+            const int pos = sh.hist[dg * T + tid]++;
             k1[pos] = kk;
             v1[pos] = v0[e];
         }
@@ -226,7 +235,7 @@ FD_HD bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int
 // ------------------------------------------------------------- one face
 
 template <int F, class X>
-FD_HD void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
+FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
                       const int *ord, int *stats) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F;
@@ -267,15 +276,12 @@ FD_HD void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo
             const int p = cur[it];
             int i, j, k, a, b;
             face_node<F>(fg, p, i, j, k, a, b);
-            Acc<F, false> A;
-            A.time = J.time; A.slow = J.slow;
-            A.node = (long long)k * nxy + (long long)j * nx + i;
-            A.snode = (long long)k * J.s_sz + (long long)j * J.s_sy + i;
-            A.nx = nx; A.nxy = nxy; A.s_sy = J.s_sy; A.s_sz = J.s_sz;
-            A.a = a; A.b = b; A.p = p; A.kp = 0; A.fg = &fg; A.O = W.O; A.key = W.key;
+            Nbhd N;
+            load_nbhd<F, false>(J, G, fg, W.O, W.key, p, a, b, i, j, k, N);
+            const long long node = (long long)k * nxy + (long long)j * nx + i;
             const float told = W.O[p];
             float tin0 = sh.cin0, tin4 = sh.cin4;
-            if (needs_carry<F>(A, G, i, j, k, told)) {
+            if (needs_carry<F>(N, G, i, j, k, told)) {
                 int r = W.rank[p] - 1;
                 while (r >= 0 && W.stat[r] == 2) r--;
                 if (r >= 0) {
@@ -289,9 +295,9 @@ FD_HD void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo
                 }
             }
             NodeOut o;
-            eval_node<F>(A, G, i, j, k, told, J.slow[A.snode], tin0, tin4, o);
+            eval_node<F>(N, G, i, j, k, told, tin0, tin4, o);
             if (o.stored) {
-                J.time[A.node] = o.tnew;
+                J.time[node] = o.tnew;
                 if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
             }
             W.rstat[p] = o.sets ? 1 : 2;
@@ -343,7 +349,7 @@ FD_HD void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo
 // CHANGE schedule.  Returns false (face restored) if LEVEL must redo the face.
 // Work list: W.list0 (nodes to evaluate), W.list2 (nodes that changed).
 template <int F, class X>
-FD_HD bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
+FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
                        int *stats) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F;
@@ -365,25 +371,22 @@ FD_HD bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Ge
             const int p = (round == 1) ? it : W.list0[it];
             int i, j, k, a, b;
             face_node<F>(fg, p, i, j, k, a, b);
-            Acc<F, true> A;
-            A.time = J.time; A.slow = J.slow;
-            A.node = (long long)k * nxy + (long long)j * nx + i;
-            A.snode = (long long)k * J.s_sz + (long long)j * J.s_sy + i;
-            A.nx = nx; A.nxy = nxy; A.s_sy = J.s_sy; A.s_sz = J.s_sz;
-            A.a = a; A.b = b; A.p = p; A.kp = W.key[p]; A.fg = &fg; A.O = W.O; A.key = W.key;
+            Nbhd N;
+            load_nbhd<F, true>(J, G, fg, W.O, W.key, p, a, b, i, j, k, N);
+            const long long node = (long long)k * nxy + (long long)j * nx + i;
             const float told = W.O[p];
             NodeOut o;
-            eval_node<F>(A, G, i, j, k, told, J.slow[A.snode], 0.0f, 0.0f, o);
+            eval_node<F>(N, G, i, j, k, told, 0.0f, 0.0f, o);
             const float nv = o.stored ? o.tnew : told;
             W.flag[p] = (unsigned char)((o.head ? 1 : 0) | (o.used_in ? 2 : 0) | (o.sets ? 4 : 0));
             W.tout0[p] = o.tout0;
             W.tout4[p] = o.tout4;
-            const float cv = J.time[A.node];
+            const float cv = J.time[node];
             unsigned un, uc;
             __builtin_memcpy(&un, &nv, 4);
             __builtin_memcpy(&uc, &cv, 4);
             if (un != uc) {
-                J.time[A.node] = nv;
+                J.time[node] = nv;
                 W.list2[x.atomic_add(&sh.n[2], 1)] = p;
             }
         }
@@ -457,7 +460,7 @@ FD_HD bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Ge
 }
 
 template <int F, class X>
-FD_HD void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, int *stats) {
+FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, int *stats) {
     const int T = x.nthr(), tid = x.tid();
     const int(*B)[2] = G.box;
     FaceGeom fg;

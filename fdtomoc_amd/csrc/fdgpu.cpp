@@ -322,7 +322,10 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         const size_t b_dist = align256(sizeof(double) * T.prep.init_dist.size());
         const size_t b_val = align256(sizeof(float) * T.prep.init_val.size());
         const size_t b_st = align256(sizeof(int) * fdsweep::kStatCount);
-        T.bytes = b_sin + b_idx + b_dist + b_val + b_st;
+        const int maxdim = std::max(T.par.nx, std::max(T.par.ny, T.par.nz));
+        const size_t b_ms = align256(sizeof(int) * (size_t)T.n);
+        const size_t b_lf = align256(sizeof(int) * 6 * (size_t)maxdim);
+        T.bytes = b_sin + b_idx + b_dist + b_val + b_st + b_ms + b_lf;
         rc = dev_alloc(c, (void **)&T.d_time, sizeof(float) * (size_t)T.n);
         if (rc) break;
         rc = dev_alloc(c, (void **)&T.d_aux, T.bytes);
@@ -343,6 +346,10 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         const float *d_val = (const float *)(T.d_aux + o);
         o += b_val;
         T.d_stats = (int *)(T.d_aux + o);
+        o += b_st;
+        int *d_ms = (int *)(T.d_aux + o);
+        o += b_ms;
+        int *d_lf = (int *)(T.d_aux + o);
         if (hipMemcpyAsync(T.d_aux, aux.data(), T.bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
             rc = FDGPU_EHIP;
             break;
@@ -375,6 +382,9 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         J.reverse = T.prep.reverse; J.headpref = T.prep.headpref; J.maxrad = T.prep.maxrad;
         J.radius0 = T.prep.radius0; J.srcwall = T.prep.srcwall;
         J.stats = T.d_stats;
+        J.mod_stamp = d_ms;
+        J.last_full = d_lf;
+        J.maxdim = maxdim;
     }
     auto unwind = [&]() {
         for (int t = 0; t < made; t++) release_table(c, tabs[(size_t)t]);

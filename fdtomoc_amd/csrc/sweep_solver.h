@@ -60,13 +60,18 @@ struct TableJob {
     int grow[6];               // kminus kplus iplus iminus jminus jplus
     int reverse, headpref, maxrad, radius0, srcwall;
     int *stats;                // kStat* counters (device)
+    int *mod_stamp;            // nx*ny*nz: face counter of each node's last change
+    int *last_full;            // 6 * maxdim: face counter at the start of the last
+                               // full-plane face per (face type, plane)
+    int maxdim;                // max(nx, ny, nz)
 };
 
 enum {
     kStatPasses = 0, kStatRestarts, kStatFacesLevel, kStatFacesChange, kStatRoundsLevel,
     kStatRoundsChange, kStatFallbacks, kStatEvals, kStatDefers, kStatError, kStatLastWall,
     kStatHeadw0,  // 7 entries (slots 0..6) of the first pass
-    kStatCount = kStatHeadw0 + 7
+    kStatSkips = kStatHeadw0 + 7,  // CHANGE first-round evaluations skipped (inputs unchanged)
+    kStatCount
 };
 
 // Per-slot scratch, sized for the largest face of any table in the launch.
@@ -84,7 +89,7 @@ struct Scratch {
     int *rstat;          // per fill index, result status of the current round
     int *list0, *list1, *list2;  // work lists
     int *stamp;          // CHANGE: last round a node was queued
-    unsigned char *flag; // CHANGE: bit0 head wave, bit1 used carried t0/t4, bit2 sets t0/t4
+    unsigned char *flag; // CHANGE: bit0 head wave, bit1 used carried t0/t4, bit2 sets t0/t4, bit3 evaluated
 };
 
 constexpr int kMaxThreads = 1024;
@@ -98,6 +103,7 @@ struct Shared {
     int best;            // CHANGE: fill index of the last node in face order
     float cin0, cin4;    // carried t[0]/t[4] entering the face
     int job;
+    int face_ctr;        // faces swept so far in this table (stamps start at 1)
     int hist[16 * 256];  // radix histograms, digit-major, up to 256 threads
     int part[kMaxThreads];
 };
@@ -177,6 +183,32 @@ FD_INLINE void load_nbhd(const TableJob &J, const Geo &G, const FaceGeom &fg, co
     N.sq[2] = G.sinq[j + 2];
 }
 
+// Latest change stamp over a node's 3x3x2 neighbourhood (its every input).
+template <int F>
+FD_INLINE int max_stamp(const TableJob &J, int i, int j, int k) {
+    using S = FaceSpec<F>;
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    int m = 0;
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+        for (int db = -1; db <= 1; db++)
+#pragma unroll
+            for (int da = -1; da <= 1; da++) {
+                int o[3] = {0, 0, 0};
+                o[S::nax] = pl ? S::ndir : 0;
+                o[S::a_ax] = da;
+                o[S::b_ax] = db;
+                int ci = i + o[0], cj = j + o[1], ck = k + o[2];
+                ci = ci < 0 ? 0 : (ci >= J.nx ? J.nx - 1 : ci);
+                cj = cj < 0 ? 0 : (cj >= J.ny ? J.ny - 1 : cj);
+                ck = ck < 0 ? 0 : (ck >= J.nz ? J.nz - 1 : ck);
+                const int v = J.mod_stamp[(long long)ck * nxy + (long long)cj * nx + ci];
+                m = v > m ? v : m;
+            }
+    return m;
+}
+
 // ------------------------------------------------------------ radix sort
 
 // Stable LSD sort of F (key, val) pairs by key, 4-bit digits, skipping digits
@@ -236,7 +268,7 @@ FD_INLINE bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1,
 
 template <int F, class X>
 FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
-                      const int *ord, int *stats) {
+                      const int *ord, int *stats, int fcur) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
@@ -298,6 +330,7 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
             eval_node<F>(N, G, i, j, k, told, tin0, tin4, o);
             if (o.stored) {
                 J.time[node] = o.tnew;
+                J.mod_stamp[node] = fcur;
                 if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
             }
             W.rstat[p] = o.sets ? 1 : 2;
@@ -350,7 +383,7 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
 // Work list: W.list0 (nodes to evaluate), W.list2 (nodes that changed).
 template <int F, class X>
 FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
-                       int *stats) {
+                       int *stats, int fcur, int fprev) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
@@ -367,10 +400,19 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             break;
         }
         // evaluate: preceding neighbours from the grid, followers from the snapshot
+        int nskip = 0;
         for (int it = tid; it < nwork; it += T) {
             const int p = (round == 1) ? it : W.list0[it];
             int i, j, k, a, b;
             face_node<F>(fg, p, i, j, k, a, b);
+            // A node whose inputs have not changed since the start of the last
+            // full-plane face of this type through its plane would reproduce
+            // that evaluation, whose result the grid already holds.
+            if (round == 1 && fprev > 0 && max_stamp<F>(J, i, j, k) < fprev) {
+                W.flag[p] = 0;
+                nskip++;
+                continue;
+            }
             Nbhd N;
             load_nbhd<F, true>(J, G, fg, W.O, W.key, p, a, b, i, j, k, N);
             const long long node = (long long)k * nxy + (long long)j * nx + i;
@@ -378,7 +420,7 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             NodeOut o;
             eval_node<F>(N, G, i, j, k, told, 0.0f, 0.0f, o);
             const float nv = o.stored ? o.tnew : told;
-            W.flag[p] = (unsigned char)((o.head ? 1 : 0) | (o.used_in ? 2 : 0) | (o.sets ? 4 : 0));
+            W.flag[p] = (unsigned char)((o.head ? 1 : 0) | (o.used_in ? 2 : 0) | (o.sets ? 4 : 0) | 8);
             W.tout0[p] = o.tout0;
             W.tout4[p] = o.tout4;
             const float cv = J.time[node];
@@ -387,9 +429,11 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             __builtin_memcpy(&uc, &cv, 4);
             if (un != uc) {
                 J.time[node] = nv;
+                J.mod_stamp[node] = fcur;
                 W.list2[x.atomic_add(&sh.n[2], 1)] = p;
             }
         }
+        if (nskip) x.atomic_add(&stats[kStatSkips], nskip);
         if (tid == 0) {
             stats[kStatRoundsChange] += 1;
             stats[kStatEvals] += nwork;
@@ -436,6 +480,21 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             if (c >= 0 && (bp < 0 || key_less(bk, bp, W.key[c], c))) { bp = c; bk = W.key[c]; }
         }
         sh.best = bp;
+        if (bp >= 0 && !(W.flag[bp] & 8)) {
+            // the last node in face order was skipped: evaluate it once for the
+            // t[0]/t[4] it leaves behind (it has no followers, all inputs final)
+            int i, j, k, a, b;
+            face_node<F>(fg, bp, i, j, k, a, b);
+            Nbhd N;
+            load_nbhd<F, false>(J, G, fg, W.O, W.key, bp, a, b, i, j, k, N);
+            NodeOut o;
+            eval_node<F>(N, G, i, j, k, W.O[bp], 0.0f, 0.0f, o);
+            if (o.stored) stats[kStatError] |= 32;   // a skipped node would have changed
+            W.flag[bp] = (unsigned char)((o.used_in ? 2 : 0) | (o.sets ? 4 : 0) | 8);
+            W.tout0[bp] = o.tout0;
+            W.tout4[bp] = o.tout4;
+            if (o.used_in) sh.flag |= 4;
+        }
         if (bp >= 0 && !(W.flag[bp] & 4)) sh.flag |= 4;
     }
     x.sync();
@@ -483,8 +542,14 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
     using S = FaceSpec<F>;
     const int di = S::nax == AXX ? S::ndir : 0, dj = S::nax == AXY ? S::ndir : 0, dk = S::nax == AXZ ? S::ndir : 0;
-    if (tid == 0) { sh.flag = 0; sh.kor = 0u; sh.kand = 0xffffffffu; }
+    if (tid == 0) { sh.flag = 0; sh.kor = 0u; sh.kand = 0xffffffffu; sh.face_ctr += 1; }
     x.sync();
+    const int fcur = sh.face_ctr;
+    const int na = FaceSpec<F>::a_ax == AXX ? J.nx : FaceSpec<F>::a_ax == AXY ? J.ny : J.nz;
+    const int nb = FaceSpec<F>::b_ax == AXX ? J.nx : FaceSpec<F>::b_ax == AXY ? J.ny : J.nz;
+    const bool full = fg.a0 == 0 && fg.wa == na && fg.b0 == 0 && fg.wb == nb;
+    int *lastf = J.last_full + F * J.maxdim + fg.fixed;
+    const int fprev = full ? *lastf : 0;
     // gather the pre-face values and the inner-layer keys
     unsigned kor = 0u, kand = 0xffffffffu;
     int fl = 0;
@@ -512,20 +577,22 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     }
     const bool try_change = !(sh.flag & 1);
     x.sync();
+    bool done = false;
     if (try_change) {
-        if (face_change<F>(x, sh, J, W, G, fg, stats)) {
-            if (tid == 0) stats[kStatFacesChange] += 1;
-            x.sync();
-            return;
-        }
+        done = face_change<F>(x, sh, J, W, G, fg, stats, fcur, fprev);
+        if (done && tid == 0) stats[kStatFacesChange] += 1;
     }
-    // LEVEL needs ranks only; the sort may leave the keys permuted
-    const bool sw = radix_sort(x, sh, W.key, W.val, W.key2, W.val2, fg.F);
-    const int *ord = sw ? W.val2 : W.val;
-    face_level<F>(x, sh, J, W, G, fg, ord, stats);
-    if (tid == 0) stats[kStatFacesLevel] += 1;
+    if (!done) {
+        // LEVEL needs ranks only; the sort may leave the keys permuted
+        const bool sw = radix_sort(x, sh, W.key, W.val, W.key2, W.val2, fg.F);
+        const int *ord = sw ? W.val2 : W.val;
+        face_level<F>(x, sh, J, W, G, fg, ord, stats, fcur);
+        if (tid == 0) stats[kStatFacesLevel] += 1;
+    }
+    if (full && tid == 0) *lastf = fcur;
     x.sync();
 }
+
 
 // ------------------------------------------------------------- one table
 
@@ -534,6 +601,8 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
     const int T = x.nthr(), tid = x.tid();
     int *stats = J.stats;
     const long long N = (long long)J.nx * J.ny * J.nz;
+    for (long long q = tid; q < N; q += T) J.mod_stamp[q] = 0;
+    for (int q = tid; q < 6 * J.maxdim; q += T) J.last_full[q] = 0;
     if (J.init_mode != 1) {
         for (long long q = tid; q < N; q += T) J.time[q] = 1.0e10f;
         x.sync();
@@ -545,6 +614,7 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
         for (int s = 0; s < kStatCount; s++) stats[s] = 0;
         sh.cin0 = 0.0f;
         sh.cin4 = 0.0f;
+        sh.face_ctr = 0;
     }
     x.sync();
     Geo G;

@@ -5,6 +5,7 @@
 // workgroup, so the schedule (LEVEL / CHANGE, carried-register resolution,
 // radix sort) can be checked against the oracle on a machine without a GPU.
 // It is not part of the product and is never loaded by it.
+#include <algorithm>
 #include <barrier>
 #include <cstring>
 #include <thread>
@@ -81,6 +82,11 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     std::memcpy(J.grow, pr.grow, sizeof J.grow);
     J.reverse = pr.reverse; J.headpref = pr.headpref; J.maxrad = pr.maxrad; J.radius0 = pr.radius0;
     J.srcwall = pr.srcwall; J.stats = stats;
+    const int maxdim = std::max(p.nx, std::max(p.ny, p.nz));
+    std::vector<int> mod_stamp((size_t)p.nx * p.ny * p.nz), last_full(6 * (size_t)maxdim);
+    J.mod_stamp = mod_stamp.data();
+    J.last_full = last_full.data();
+    J.maxdim = maxdim;
     fdsweep::Shared sh;
     std::memset(&sh, 0, sizeof sh);
     std::barrier<> bar(nthreads);

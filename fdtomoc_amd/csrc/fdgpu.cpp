@@ -27,6 +27,7 @@ struct Model {
     int nx = 0, ny = 0, nz = 0;
     float *d_slow = nullptr;
     size_t bytes = 0;
+    double vmin = 0.0, vmax = 0.0;   // velocity range (skip-safety bound)
 };
 
 struct Table {
@@ -385,6 +386,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         J.mod_stamp = d_ms;
         J.last_full = d_lf;
         J.maxdim = maxdim;
+        J.skip_ok = fdprep::skip_safe(to_values(T.par), T.prep, m.vmin, m.vmax) ? 1 : 0;
     }
     auto unwind = [&]() {
         for (int t = 0; t < made; t++) release_table(c, tabs[(size_t)t]);
@@ -503,6 +505,16 @@ int fdgpu_model_update_locked(fdgpu_ctx *c, int model, const float *vel) {
     auto it = c->models.find(model);
     if (it == c->models.end()) return FDGPU_EINPUT;
     Model &m = it->second;
+    const long long nv = (long long)m.nx * m.ny * m.nz;
+    double vmin = 1e300, vmax = -1e300;
+    for (long long q = 0; q < nv; q++) {
+        const double v = vel[q];
+        if (!(v == v)) { vmin = 0.0; break; }   // NaN: never skip
+        vmin = v < vmin ? v : vmin;
+        vmax = v > vmax ? v : vmax;
+    }
+    m.vmin = vmin;
+    m.vmax = vmax;
     float *d_vel = nullptr;
     int rc = dev_alloc(c, (void **)&d_vel, m.bytes);
     if (rc) return rc;

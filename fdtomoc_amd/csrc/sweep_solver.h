@@ -64,6 +64,7 @@ struct TableJob {
     int *last_full;            // 6 * maxdim: face counter at the start of the last
                                // full-plane face per (face type, plane)
     int maxdim;                // max(nx, ny, nz)
+    int skip_ok;               // skipping unchanged reverse-pass nodes is exact here
 };
 
 enum {
@@ -489,7 +490,8 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             load_nbhd<F, false>(J, G, fg, W.O, W.key, bp, a, b, i, j, k, N);
             NodeOut o;
             eval_node<F>(N, G, i, j, k, W.O[bp], 0.0f, 0.0f, o);
-            if (o.stored) stats[kStatError] |= 32;   // a skipped node would have changed
+            // a skipped node must reproduce its stored value and count no head wave
+            if ((o.stored && o.tnew != W.O[bp]) || o.head) stats[kStatError] |= 32;
             W.flag[bp] = (unsigned char)((o.used_in ? 2 : 0) | (o.sets ? 4 : 0) | 8);
             W.tout0[bp] = o.tout0;
             W.tout4[bp] = o.tout4;
@@ -549,7 +551,7 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     const int nb = FaceSpec<F>::b_ax == AXX ? J.nx : FaceSpec<F>::b_ax == AXY ? J.ny : J.nz;
     const bool full = fg.a0 == 0 && fg.wa == na && fg.b0 == 0 && fg.wb == nb;
     int *lastf = J.last_full + F * J.maxdim + fg.fixed;
-    const int fprev = full ? *lastf : 0;
+    const int fprev = (full && J.skip_ok) ? *lastf : 0;
     // gather the pre-face values and the inner-layer keys
     unsigned kor = 0u, kand = 0xffffffffu;
     int fl = 0;
@@ -589,7 +591,10 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
         face_level<F>(x, sh, J, W, G, fg, ord, stats, fcur);
         if (tid == 0) stats[kStatFacesLevel] += 1;
     }
-    if (full && tid == 0) *lastf = fcur;
+    // Only a CHANGE face is a valid reference for later skips: there every node
+    // started below 1e9 (no new-edge or new-face trials) and set its own
+    // t[0]/t[4]; a LEVEL face's results also depend on the carried registers.
+    if (full && done && tid == 0) *lastf = fcur;
     x.sync();
 }
 

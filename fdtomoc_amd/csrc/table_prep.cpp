@@ -29,6 +29,24 @@ double glath(double hlat, double h, double *r) {
 }
 }  // namespace
 
+bool skip_safe(const ParValues &p, const Prepared &g, double vmin, double vmax) {
+    if (!(vmin > 0.0) || !(vmax >= vmin) || !(vmax < 1.0e30)) return false;
+    const double smin = 1.0 / vmax, smax = 1.0 / vmin;
+    const double rbot = g.z0r - g.h * p.nz;
+    double sinmin = 1.0;
+    for (double s : g.sinq) sinmin = fabs(s) < sinmin ? fabs(s) : sinmin;
+    double dmin = fabs(g.h);
+    dmin = fmin(dmin, fabs(rbot * g.dq));
+    dmin = fmin(dmin, fabs(rbot * sinmin * g.df));
+    if (!(dmin > 0.0)) return false;
+    const double ext = fabs(g.h) * sqrt((double)p.nx * p.nx + (double)p.ny * p.ny + (double)p.nz * p.nz);
+    // bound on any travel time: twice the slowest straight crossing of the grid
+    const double tmax = 2.0 * smax * ext + 1.0;
+    const double ulp = ldexp(1.0, ilogb(tmax) - 23);   // float spacing at tmax
+    // fhead of a non-changing store is below ulp / (2 d s); require 8x margin to 1e-3
+    return ulp / (dmin * smin) < 2.5e-4;
+}
+
 long long max_face(int nx, int ny, int nz) {
     long long a = (long long)nx * ny, b = (long long)ny * nz, c = (long long)nx * nz;
     long long m = a > b ? a : b;

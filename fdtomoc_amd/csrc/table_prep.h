@@ -66,6 +66,15 @@ int prepare_geometry(const ParValues &p, Prepared &out);
 // For srctype 2/3 the initial grid is written to `grid` (nx*ny*nz floats).
 int prepare_source(const ParValues &p, Prepared &out, const float *preset, const float *boxvals, float *grid);
 
+// Whether reverse-pass nodes with unchanged inputs may be skipped without
+// changing head-wave counts.  A skipped node reproduces its stored float; the
+// reference would still "store" it when the double result lies below that
+// float, and would count a head wave if fhead = (improvement)/(d * slowness)
+// exceeded 1e-3 (sphfd.c:1756-1759).  The improvement is then below one float
+// ulp of the time, so skipping is exact whenever ulp(t_max)/(d_min * s_min) is
+// far below 1e-3; this bound is checked with generous margins.
+bool skip_safe(const ParValues &p, const Prepared &g, double vmin, double vmax);
+
 // Largest face area of a grid (the reference's sort buffer, sphfd.c:595-604).
 long long max_face(int nx, int ny, int nz);
 

@@ -87,6 +87,12 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     J.mod_stamp = mod_stamp.data();
     J.last_full = last_full.data();
     J.maxdim = maxdim;
+    float smin = 1e30f, smax = 0.f;
+    for (long long q = 0; q < (long long)p.nx * p.ny * p.nz; q++) {
+        smin = slow[q] < smin ? slow[q] : smin;
+        smax = slow[q] > smax ? slow[q] : smax;
+    }
+    J.skip_ok = fdprep::skip_safe(p, pr, 1.0 / smax, 1.0 / smin) ? 1 : 0;
     fdsweep::Shared sh;
     std::memset(&sh, 0, sizeof sh);
     std::barrier<> bar(nthreads);

@@ -91,7 +91,7 @@ void dev_free(fdgpu_ctx *c, void *p, size_t bytes) {
 
 size_t slot_bytes(long long F) {
     const size_t f = align256(sizeof(float) * (size_t)F);
-    return 18 * f + align256((size_t)F);
+    return 15 * f + align256((size_t)F);
 }
 
 int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
@@ -128,9 +128,6 @@ int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
         W.stat = (int *)take(f);
         W.tout0 = (float *)take(f);
         W.tout4 = (float *)take(f);
-        W.rt0 = (float *)take(f);
-        W.rt4 = (float *)take(f);
-        W.rstat = (int *)take(f);
         W.list0 = (int *)take(f);
         W.list1 = (int *)take(f);
         W.list2 = (int *)take(f);

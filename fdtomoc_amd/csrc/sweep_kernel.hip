@@ -20,6 +20,7 @@ struct DevExec {
     __device__ __forceinline__ void sync() const { __syncthreads(); }
     __device__ __forceinline__ int atomic_add(int *p, int v) const { return atomicAdd(p, v); }
     __device__ __forceinline__ int atomic_exch(int *p, int v) const { return atomicExch(p, v); }
+    __device__ __forceinline__ unsigned atomic_sub_u(unsigned *p, unsigned v) const { return atomicSub(p, v); }
     __device__ __forceinline__ unsigned atomic_or(unsigned *p, unsigned v) const { return atomicOr(p, v); }
     __device__ __forceinline__ unsigned atomic_and(unsigned *p, unsigned v) const { return atomicAnd(p, v); }
 };

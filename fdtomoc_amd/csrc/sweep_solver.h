@@ -7,6 +7,7 @@
 //   void X::sync()                         group barrier with memory ordering
 //   int  X::atomic_add(int *p, int v)      returns the old value
 //   int  X::atomic_exch(int *p, int v)     returns the old value
+//   unsigned X::atomic_sub_u(unsigned *p, unsigned v)  returns the old value
 //   unsigned X::atomic_or(unsigned*,unsigned), X::atomic_and(...)
 //
 // On the GPU X wraps threadIdx/__syncthreads/atomics (sweep_kernel.hip).
@@ -83,17 +84,20 @@ struct Scratch {
     int *val, *val2;     // sort ping-pong payload (fill index)
     int *rank;           // fill index -> rank
     int *ord;            // rank -> fill index
-    int *pend;           // preceding neighbours not yet final
-    int *stat;           // per rank: 0 not final, 1 set t0/t4, 2 passed them on
+    int *pend;           // LEVEL, faces too large for LDS: byte-packed pending counters
+    int *stat;           // per rank: 0 not final, else (round+1)<<2 | 1 set t0/t4, 2 passed them on
     float *tout0, *tout4;  // per rank (LEVEL) or fill index (CHANGE)
-    float *rt0, *rt4;    // per fill index, results of the current round
-    int *rstat;          // per fill index, result status of the current round
     int *list0, *list1, *list2;  // work lists
     int *stamp;          // CHANGE: last round a node was queued
-    unsigned char *flag; // CHANGE: bit0 head wave, bit1 used carried t0/t4, bit2 sets t0/t4, bit3 evaluated
+    unsigned char *flag; // CHANGE: bit0 head wave, bit1 used carried t0/t4, bit2 sets t0/t4, bit3 evaluated;
+                         // LEVEL, faces too large for LDS: successor masks
 };
 
-constexpr int kMaxThreads = 1024;
+constexpr int kMaxThreads = 256;
+// LDS scratch of a workgroup: radix histograms during a sort, then the LEVEL
+// schedule's byte-packed pending counters and successor masks (faces up to
+// about 70K nodes; larger faces use global scratch).
+constexpr int kArenaWords = 36000;
 
 struct Shared {
     int n[3];            // list sizes
@@ -105,8 +109,8 @@ struct Shared {
     float cin0, cin4;    // carried t[0]/t[4] entering the face
     int job;
     int face_ctr;        // faces swept so far in this table (stamps start at 1)
-    int hist[16 * 256];  // radix histograms, digit-major, up to 256 threads
-    int part[kMaxThreads];
+    int st[kStatCount];  // counters of the current table, flushed to TableJob::stats at the end
+    int arena[kArenaWords];
 };
 
 // ---------------------------------------------------------------- helpers
@@ -220,40 +224,42 @@ FD_INLINE bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1,
     const int chunk = (F + T - 1) / T;
     const int lo = tid * chunk < F ? tid * chunk : F;
     const int hi = lo + chunk < F ? lo + chunk : F;
+    int *hist = sh.arena;              // 16 * T digit-major counters
+    int *part = sh.arena + 16 * T;     // T partial sums
     bool swapped = false;
     for (int d = 0; d < 8; d++) {
         const unsigned vary = (sh.kor ^ sh.kand) >> (4 * d);
         if ((vary & 15u) == 0) continue;
-        // per-thread digit counts live in this thread's column of sh.hist
-        for (int q = 0; q < 16; q++) sh.hist[q * T + tid] = 0;
-        for (int e = lo; e < hi; e++) sh.hist[((k0[e] >> (4 * d)) & 15u) * T + tid]++;
+        // per-thread digit counts live in this thread's column of hist
+        for (int q = 0; q < 16; q++) hist[q * T + tid] = 0;
+        for (int e = lo; e < hi; e++) hist[((k0[e] >> (4 * d)) & 15u) * T + tid]++;
         x.sync();
         // exclusive scan over 16*T counters in digit-major order: each thread
         // owns 16 consecutive counters
         int s = 0;
-        for (int q = 0; q < 16; q++) s += sh.hist[tid * 16 + q];
-        sh.part[tid] = s;
+        for (int q = 0; q < 16; q++) s += hist[tid * 16 + q];
+        part[tid] = s;
         x.sync();
         if (tid == 0) {
             int run = 0;
             for (int q = 0; q < T; q++) {
-                const int v = sh.part[q];
-                sh.part[q] = run;
+                const int v = part[q];
+                part[q] = run;
                 run += v;
             }
         }
         x.sync();
-        int run = sh.part[tid];
+        int run = part[tid];
         for (int q = 0; q < 16; q++) {
-            const int v = sh.hist[tid * 16 + q];
-            sh.hist[tid * 16 + q] = run;
+            const int v = hist[tid * 16 + q];
+            hist[tid * 16 + q] = run;
             run += v;
         }
         x.sync();
         for (int e = lo; e < hi; e++) {
             const unsigned kk = k0[e];
             const int dg = (kk >> (4 * d)) & 15u;
-            const int pos = sh.hist[dg * T + tid]++;
+            const int pos = hist[dg * T + tid]++;
             k1[pos] = kk;
             v1[pos] = v0[e];
         }
@@ -267,44 +273,63 @@ FD_INLINE bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1,
 
 // ------------------------------------------------------------- one face
 
+// In-face neighbour directions of the successor masks.
+FD_INLINE int dir_da(int d) { return d == 0 || d == 3 || d == 5 ? -1 : (d == 1 || d == 6 ? 0 : 1); }
+FD_INLINE int dir_db(int d) { return d < 3 ? -1 : (d < 5 ? 0 : 1); }
+
 template <int F, class X>
 FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
-                      const int *ord, int *stats, int fcur) {
+                          const int *ord, int fcur) {
     const int T = x.nthr(), tid = x.tid();
-    const int Fn = fg.F;
+    const int Fn = fg.F, wa = fg.wa, wb = fg.wb;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
-    // ranks and pending counters
     for (int r = tid; r < Fn; r += T) W.rank[ord[r]] = r;
-    x.sync();
+    // pending counters, one byte per node packed four to a word, and successor
+    // masks (bit d: the neighbour in direction d follows the node)
+    const int nwords = (Fn + 3) / 4;
+    const bool in_lds = (long long)nwords * 4 + Fn <= (long long)kArenaWords * 4;
+    unsigned *pend = in_lds ? (unsigned *)sh.arena : (unsigned *)W.pend;
+    unsigned char *succ = in_lds ? (unsigned char *)(sh.arena + nwords) : W.flag;
+    int *lists[3] = {W.list0, W.list1, W.list2};
     if (tid == 0) { sh.n[0] = 0; sh.n[1] = 0; sh.n[2] = 0; sh.processed = 0; }
     x.sync();
-    for (int p = tid; p < Fn; p += T) {
-        const int a = p % fg.wa, b = p / fg.wa, rp = W.rank[p];
-        int c = 0;
-        for (int db = -1; db <= 1; db++)
-            for (int da = -1; da <= 1; da++) {
-                if (!da && !db) continue;
-                const int aa = a + da, bb = b + db;
-                if (aa < 0 || bb < 0 || aa >= fg.wa || bb >= fg.wb) continue;
-                c += W.rank[bb * fg.wa + aa] < rp;
+    for (int w = tid; w < nwords; w += T) {
+        unsigned word = 0;
+        for (int e = 0; e < 4; e++) {
+            const int p = 4 * w + e;
+            if (p >= Fn) break;
+            const int a = p % wa, b = p / wa, rp = W.rank[p];
+            int c = 0;
+            unsigned m = 0;
+            for (int d = 0; d < 8; d++) {
+                const int aa = a + dir_da(d), bb = b + dir_db(d);
+                if (aa < 0 || bb < 0 || aa >= wa || bb >= wb) continue;
+                if (W.rank[bb * wa + aa] < rp) c++;
+                else m |= 1u << d;
             }
-        W.pend[p] = c;
-        W.stat[rp] = 0;
-        if (c == 0) W.list0[x.atomic_add(&sh.n[0], 1)] = p;
+            word |= (unsigned)c << (8 * e);
+            succ[p] = (unsigned char)m;
+            W.stat[rp] = 0;
+            if (c == 0) lists[0][x.atomic_add(&sh.n[0], 1)] = p;
+        }
+        pend[w] = word;
     }
     x.sync();
-    int *cur = W.list0, *nxt = W.list1, *done = W.list2;
-    int icur = 0, inxt = 1;
-    // every round finalises at least the lowest-ranked open node, so F rounds
-    // always suffice; the cap only guards against a broken invariant
-    for (int guard = 0;; guard++) {
-        const int ncur = sh.n[icur];
+    // One barrier per round: round R evaluates list R%3, fills list (R+1)%3 and
+    // clears the counter of list (R+2)%3, which no one touches during round R.
+    int R = 0;
+    for (;; R++) {
+        const int lc = R % 3, ln = (R + 1) % 3, lz = (R + 2) % 3;
+        const int ncur = sh.n[lc];
         if (ncur == 0) break;
-        if (guard > 2 * Fn + 8) {
-            if (tid == 0) stats[kStatError] |= 8;
+        if (R > 2 * Fn + 8) {   // every round finalises the lowest open rank
+            if (tid == 0) sh.st[kStatError] |= 8;
             break;
         }
-        // phase B: evaluate ready nodes (statuses published by earlier rounds only)
+        if (tid == 0) sh.n[lz] = 0;
+        const int *cur = lists[lc];
+        int *nxt = lists[ln];
+        int ndone = 0, ndefer = 0;
         for (int it = tid; it < ncur; it += T) {
             const int p = cur[it];
             int i, j, k, a, b;
@@ -313,16 +338,25 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
             load_nbhd<F, false>(J, G, fg, W.O, W.key, p, a, b, i, j, k, N);
             const long long node = (long long)k * nxy + (long long)j * nx + i;
             const float told = W.O[p];
+            const int rp = W.rank[p];
             float tin0 = sh.cin0, tin4 = sh.cin4;
             if (needs_carry<F>(N, G, i, j, k, told)) {
-                int r = W.rank[p] - 1;
-                while (r >= 0 && W.stat[r] == 2) r--;
+                // the last earlier node that set t[0]/t[4]; statuses written in
+                // this round are not visible yet and count as open
+                int r = rp - 1;
+                bool open = false;
+                while (r >= 0) {
+                    const int v = W.stat[r];
+                    if (v == 0 || (v >> 2) > R) { open = true; break; }
+                    if ((v & 3) == 1) break;
+                    r--;
+                }
+                if (open) {
+                    nxt[x.atomic_add(&sh.n[ln], 1)] = p;
+                    ndefer++;
+                    continue;
+                }
                 if (r >= 0) {
-                    if (W.stat[r] == 0) {   // the node that sets them is not final yet
-                        nxt[x.atomic_add(&sh.n[inxt], 1)] = p;
-                        x.atomic_add(&stats[kStatDefers], 1);
-                        continue;
-                    }
                     tin0 = W.tout0[r];
                     tin4 = W.tout4[r];
                 }
@@ -334,48 +368,34 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
                 J.mod_stamp[node] = fcur;
                 if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
             }
-            W.rstat[p] = o.sets ? 1 : 2;
-            W.rt0[p] = o.tout0;
-            W.rt4[p] = o.tout4;
-            done[x.atomic_add(&sh.n[2], 1)] = p;
+            W.tout0[rp] = o.tout0;
+            W.tout4[rp] = o.tout4;
+            W.stat[rp] = ((R + 1) << 2) | (o.sets ? 1 : 2);
+            // release the successors; the byte cannot borrow, it is >= 1
+            const unsigned m = succ[p];
+            for (int d = 0; d < 8; d++) {
+                if (!(m & (1u << d))) continue;
+                const int q = (b + dir_db(d)) * wa + a + dir_da(d);
+                const int shf = 8 * (q & 3);
+                const unsigned old = x.atomic_sub_u(&pend[q >> 2], 1u << shf);
+                if (((old >> shf) & 255u) == 1u) nxt[x.atomic_add(&sh.n[ln], 1)] = q;
+            }
+            ndone++;
         }
-        x.sync();
-        // phase C: publish statuses and release successors
-        const int ndone = sh.n[2];
-        for (int it = tid; it < ndone; it += T) {
-            const int p = done[it];
-            const int rp = W.rank[p];
-            W.tout0[rp] = W.rt0[p];
-            W.tout4[rp] = W.rt4[p];
-            W.stat[rp] = W.rstat[p];
-            const int a = p % fg.wa, b = p / fg.wa;
-            for (int db = -1; db <= 1; db++)
-                for (int da = -1; da <= 1; da++) {
-                    if (!da && !db) continue;
-                    const int aa = a + da, bb = b + db;
-                    if (aa < 0 || bb < 0 || aa >= fg.wa || bb >= fg.wb) continue;
-                    const int q = bb * fg.wa + aa;
-                    if (W.rank[q] > rp && x.atomic_add(&W.pend[q], -1) == 1)
-                        nxt[x.atomic_add(&sh.n[inxt], 1)] = q;
-                }
+        if (ndone) {
+            x.atomic_add(&sh.processed, ndone);
+            x.atomic_add(&sh.st[kStatEvals], ndone);
         }
-        if (tid == 0) {
-            stats[kStatRoundsLevel] += 1;
-            stats[kStatEvals] += ndone;
-            sh.processed += ndone;
-        }
-        x.sync();
-        if (tid == 0) { sh.n[icur] = 0; sh.n[2] = 0; }
-        int *tl = cur; cur = nxt; nxt = tl;
-        const int ti = icur; icur = inxt; inxt = ti;
+        if (ndefer) x.atomic_add(&sh.st[kStatDefers], ndefer);
+        if (tid == 0) sh.st[kStatRoundsLevel] += 1;
         x.sync();
     }
     // t[0]/t[4] leaving the face: those of the last node in face order that set them
     if (tid == 0) {
         int r = Fn - 1;
-        while (r >= 0 && W.stat[r] == 2) r--;
+        while (r >= 0 && (W.stat[r] & 3) == 2) r--;
         if (r >= 0) { sh.cin0 = W.tout0[r]; sh.cin4 = W.tout4[r]; }
-        if (sh.processed != Fn) stats[kStatError] |= 4;
+        if (sh.processed != Fn) sh.st[kStatError] |= 4;
     }
     x.sync();
 }
@@ -384,7 +404,7 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
 // Work list: W.list0 (nodes to evaluate), W.list2 (nodes that changed).
 template <int F, class X>
 FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
-                       int *stats, int fcur, int fprev) {
+                       int fcur, int fprev) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
@@ -396,7 +416,7 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
         if (nwork == 0) break;
         // the evaluation DAG has depth <= F, so F + 1 rounds always suffice
         if (round > 2 * Fn + 8) {
-            if (tid == 0) { stats[kStatError] |= 16; sh.flag |= 4; }
+            if (tid == 0) { sh.st[kStatError] |= 16; sh.flag |= 4; }
             x.sync();
             break;
         }
@@ -434,10 +454,10 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
                 W.list2[x.atomic_add(&sh.n[2], 1)] = p;
             }
         }
-        if (nskip) x.atomic_add(&stats[kStatSkips], nskip);
+        if (nskip) x.atomic_add(&sh.st[kStatSkips], nskip);
         if (tid == 0) {
-            stats[kStatRoundsChange] += 1;
-            stats[kStatEvals] += nwork;
+            sh.st[kStatRoundsChange] += 1;
+            sh.st[kStatEvals] += nwork;
         }
         x.sync();
         if (tid == 0) sh.n[0] = 0;
@@ -471,13 +491,13 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
         if (best < 0 || key_less(bestk, best, W.key[p], p)) { best = p; bestk = W.key[p]; }
     }
     if (bad) x.atomic_or((unsigned *)&sh.flag, 4u);
-    sh.part[tid] = best;
+    sh.arena[tid] = best;
     x.sync();
     if (tid == 0) {
         int bp = -1;
         unsigned bk = 0;
         for (int q = 0; q < T; q++) {
-            const int c = sh.part[q];
+            const int c = sh.arena[q];
             if (c >= 0 && (bp < 0 || key_less(bk, bp, W.key[c], c))) { bp = c; bk = W.key[c]; }
         }
         sh.best = bp;
@@ -491,7 +511,7 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             NodeOut o;
             eval_node<F>(N, G, i, j, k, W.O[bp], 0.0f, 0.0f, o);
             // a skipped node must reproduce its stored value and count no head wave
-            if ((o.stored && o.tnew != W.O[bp]) || o.head) stats[kStatError] |= 32;
+            if ((o.stored && o.tnew != W.O[bp]) || o.head) sh.st[kStatError] |= 32;
             W.flag[bp] = (unsigned char)((o.used_in ? 2 : 0) | (o.sets ? 4 : 0) | 8);
             W.tout0[bp] = o.tout0;
             W.tout4[bp] = o.tout4;
@@ -507,7 +527,7 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
             face_node<F>(fg, p, i, j, k, a, b);
             J.time[(long long)k * nxy + (long long)j * nx + i] = W.O[p];
         }
-        if (tid == 0) stats[kStatFallbacks] += 1;
+        if (tid == 0) sh.st[kStatFallbacks] += 1;
         x.sync();
         return false;
     }
@@ -521,7 +541,7 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
 }
 
 template <int F, class X>
-FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, int *stats) {
+FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G) {
     const int T = x.nthr(), tid = x.tid();
     const int(*B)[2] = G.box;
     FaceGeom fg;
@@ -575,21 +595,21 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     if (fl) x.atomic_or((unsigned *)&sh.flag, (unsigned)fl);
     x.sync();
     if (sh.flag & 2) {
-        if (tid == 0) stats[kStatError] |= 1;   // NaN keys: the reference's qsort order is undefined
+        if (tid == 0) sh.st[kStatError] |= 1;   // NaN keys: the reference's qsort order is undefined
     }
     const bool try_change = !(sh.flag & 1);
     x.sync();
     bool done = false;
     if (try_change) {
-        done = face_change<F>(x, sh, J, W, G, fg, stats, fcur, fprev);
-        if (done && tid == 0) stats[kStatFacesChange] += 1;
+        done = face_change<F>(x, sh, J, W, G, fg, fcur, fprev);
+        if (done && tid == 0) sh.st[kStatFacesChange] += 1;
     }
     if (!done) {
         // LEVEL needs ranks only; the sort may leave the keys permuted
         const bool sw = radix_sort(x, sh, W.key, W.val, W.key2, W.val2, fg.F);
         const int *ord = sw ? W.val2 : W.val;
-        face_level<F>(x, sh, J, W, G, fg, ord, stats, fcur);
-        if (tid == 0) stats[kStatFacesLevel] += 1;
+        face_level<F>(x, sh, J, W, G, fg, ord, fcur);
+        if (tid == 0) sh.st[kStatFacesLevel] += 1;
     }
     // Only a CHANGE face is a valid reference for later skips: there every node
     // started below 1e9 (no new-edge or new-face trials) and set its own
@@ -604,7 +624,6 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
 template <class X>
 FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
     const int T = x.nthr(), tid = x.tid();
-    int *stats = J.stats;
     const long long N = (long long)J.nx * J.ny * J.nz;
     for (long long q = tid; q < N; q += T) J.mod_stamp[q] = 0;
     for (int q = tid; q < 6 * J.maxdim; q += T) J.last_full[q] = 0;
@@ -616,7 +635,7 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
             J.time[J.init_idx[q]] = J.init_mode == 0 ? (float)((double)s000 * J.init_dist[q]) : J.init_val[q];
     }
     if (tid == 0) {
-        for (int s = 0; s < kStatCount; s++) stats[s] = 0;
+        for (int s = 0; s < kStatCount; s++) sh.st[s] = 0;
         sh.cin0 = 0.0f;
         sh.cin4 = 0.0f;
         sh.face_ctr = 0;
@@ -643,17 +662,17 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
         passes++;
         while (rad0 && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
             for (int g = 1; g <= J.grow[0]; g++)
-                if (dz1) { FD_SETBOX(); sweep_face<TOP>(x, sh, J, W, G, stats); if (z1 == 0) dz1 = 0; z1--; }
+                if (dz1) { FD_SETBOX(); sweep_face<TOP>(x, sh, J, W, G); if (z1 == 0) dz1 = 0; z1--; }
             for (int g = 1; g <= J.grow[1]; g++)
-                if (dz2) { FD_SETBOX(); sweep_face<BOTTOM>(x, sh, J, W, G, stats); if (z2 == nz - 1) dz2 = 0; z2++; }
+                if (dz2) { FD_SETBOX(); sweep_face<BOTTOM>(x, sh, J, W, G); if (z2 == nz - 1) dz2 = 0; z2++; }
             for (int g = 1; g <= J.grow[2]; g++)
-                if (dx2) { FD_SETBOX(); sweep_face<EAST>(x, sh, J, W, G, stats); if (x2 == nx - 1) dx2 = 0; x2++; }
+                if (dx2) { FD_SETBOX(); sweep_face<EAST>(x, sh, J, W, G); if (x2 == nx - 1) dx2 = 0; x2++; }
             for (int g = 1; g <= J.grow[3]; g++)
-                if (dx1) { FD_SETBOX(); sweep_face<WEST>(x, sh, J, W, G, stats); if (x1 == 0) dx1 = 0; x1--; }
+                if (dx1) { FD_SETBOX(); sweep_face<WEST>(x, sh, J, W, G); if (x1 == 0) dx1 = 0; x1--; }
             for (int g = 1; g <= J.grow[4]; g++)
-                if (dy1) { FD_SETBOX(); sweep_face<NORTH>(x, sh, J, W, G, stats); if (y1 == 0) dy1 = 0; y1--; }
+                if (dy1) { FD_SETBOX(); sweep_face<NORTH>(x, sh, J, W, G); if (y1 == 0) dy1 = 0; y1--; }
             for (int g = 1; g <= J.grow[5]; g++)
-                if (dy2) { FD_SETBOX(); sweep_face<SOUTH>(x, sh, J, W, G, stats); if (y2 == ny - 1) dy2 = 0; y2++; }
+                if (dy2) { FD_SETBOX(); sweep_face<SOUTH>(x, sh, J, W, G); if (y2 == ny - 1) dy2 = 0; y2++; }
             radius++;
             if (radius == J.maxrad) rad0 = 0;
         }
@@ -661,7 +680,7 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
         int hw[7];
         for (int w = 0; w < 7; w++) hw[w] = sh.headw[w];
         if (passes == 1 && tid == 0)
-            for (int w = 0; w < 7; w++) stats[kStatHeadw0 + w] = hw[w];
+            for (int w = 0; w < 7; w++) sh.st[kStatHeadw0 + w] = hw[w];
         // sphfd.c:5570-5676
         if (hw[1] == 0 && hw[2] == 0 && hw[3] == 0 && hw[4] == 0 && hw[5] == 0 && hw[6] == 0) {
             reverse = 0;
@@ -680,15 +699,17 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
             if (srcwall == 5) z2 = 1; else { z2 = nz; dz2 = 0; }
             if (srcwall == 6) z1 = nz - 2; else { z1 = -1; dz1 = 0; }
             if (tid == 0) {
-                stats[kStatLastWall] = srcwall;
-                if (reverse > 0) stats[kStatRestarts] += 1;
+                sh.st[kStatLastWall] = srcwall;
+                if (reverse > 0) sh.st[kStatRestarts] += 1;
             }
         }
         reverse--;
         x.sync();
     }
 #undef FD_SETBOX
-    if (tid == 0) stats[kStatPasses] = passes;
+    if (tid == 0) sh.st[kStatPasses] = passes;
+    x.sync();
+    for (int q = tid; q < kStatCount; q += T) J.stats[q] = sh.st[q];
     x.sync();
 }
 

@@ -7,6 +7,7 @@
 // It is not part of the product and is never loaded by it.
 #include <algorithm>
 #include <barrier>
+#include <memory>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -23,6 +24,7 @@ struct HostExec {
     void sync() { bar->arrive_and_wait(); }
     int atomic_add(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
     int atomic_exch(int *p, int v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+    unsigned atomic_sub_u(unsigned *p, unsigned v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
     unsigned atomic_or(unsigned *p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
     unsigned atomic_and(unsigned *p, unsigned v) { return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
 };
@@ -60,15 +62,15 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     rc = fdprep::prepare_source(p, pr, preset ? extra : nullptr, (p.fill == 1) ? extra : nullptr, preset ? time : nullptr);
     if (rc) return rc;
     const long long maxF = fdprep::max_face(p.nx, p.ny, p.nz);
-    std::vector<float> O(maxF), tout0(maxF), tout4(maxF), rt0(maxF), rt4(maxF);
+    std::vector<float> O(maxF), tout0(maxF), tout4(maxF);
     std::vector<unsigned> key(maxF), key2(maxF);
-    std::vector<int> val(maxF), val2(maxF), rank(maxF), ord(maxF), pend(maxF), stat(maxF), rstat(maxF), l0(maxF),
+    std::vector<int> val(maxF), val2(maxF), rank(maxF), ord(maxF), pend(maxF), stat(maxF), l0(maxF),
         l1(maxF), l2(maxF), stamp(maxF);
     std::vector<unsigned char> flag(maxF);
     fdsweep::Scratch W;
     W.O = O.data(); W.key = key.data(); W.key2 = key2.data(); W.val = val.data(); W.val2 = val2.data();
     W.rank = rank.data(); W.ord = ord.data(); W.pend = pend.data(); W.stat = stat.data();
-    W.tout0 = tout0.data(); W.tout4 = tout4.data(); W.rt0 = rt0.data(); W.rt4 = rt4.data(); W.rstat = rstat.data();
+    W.tout0 = tout0.data(); W.tout4 = tout4.data();
     W.list0 = l0.data(); W.list1 = l1.data(); W.list2 = l2.data(); W.stamp = stamp.data(); W.flag = flag.data();
     fdsweep::TableJob J;
     std::memset(&J, 0, sizeof J);
@@ -93,7 +95,8 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
         smax = slow[q] > smax ? slow[q] : smax;
     }
     J.skip_ok = fdprep::skip_safe(p, pr, 1.0 / smax, 1.0 / smin) ? 1 : 0;
-    fdsweep::Shared sh;
+    std::unique_ptr<fdsweep::Shared> shp(new fdsweep::Shared());
+    fdsweep::Shared &sh = *shp;
     std::memset(&sh, 0, sizeof sh);
     std::barrier<> bar(nthreads);
     std::vector<std::thread> th;

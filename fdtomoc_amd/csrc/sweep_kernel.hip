@@ -23,6 +23,7 @@ struct DevExec {
     __device__ __forceinline__ unsigned atomic_sub_u(unsigned *p, unsigned v) const { return atomicSub(p, v); }
     __device__ __forceinline__ unsigned atomic_or(unsigned *p, unsigned v) const { return atomicOr(p, v); }
     __device__ __forceinline__ unsigned atomic_and(unsigned *p, unsigned v) const { return atomicAnd(p, v); }
+    __device__ __forceinline__ long long clock() const { return (long long)wall_clock64(); }
 };
 
 __global__ void __launch_bounds__(FD_SWEEP_THREADS)

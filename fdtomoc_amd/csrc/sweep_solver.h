@@ -9,6 +9,7 @@
 //   int  X::atomic_exch(int *p, int v)     returns the old value
 //   unsigned X::atomic_sub_u(unsigned *p, unsigned v)  returns the old value
 //   unsigned X::atomic_or(unsigned*,unsigned), X::atomic_and(...)
+//   long long X::clock() const             a monotonic tick counter (timing only)
 //
 // On the GPU X wraps threadIdx/__syncthreads/atomics (sweep_kernel.hip).
 //
@@ -73,6 +74,8 @@ enum {
     kStatRoundsChange, kStatFallbacks, kStatEvals, kStatDefers, kStatError, kStatLastWall,
     kStatHeadw0,  // 7 entries (slots 0..6) of the first pass
     kStatSkips = kStatHeadw0 + 7,  // CHANGE first-round evaluations skipped (inputs unchanged)
+    // time spent by the table, in X::clock() ticks (10 ns on the GPU)
+    kStatTimeGather, kStatTimeSort, kStatTimeLevel, kStatTimeChange, kStatTimeInit,
     kStatCount
 };
 
@@ -564,6 +567,7 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
     using S = FaceSpec<F>;
     const int di = S::nax == AXX ? S::ndir : 0, dj = S::nax == AXY ? S::ndir : 0, dk = S::nax == AXZ ? S::ndir : 0;
+    const long long c0 = x.clock();
     if (tid == 0) { sh.flag = 0; sh.kor = 0u; sh.kand = 0xffffffffu; sh.face_ctr += 1; }
     x.sync();
     const int fcur = sh.face_ctr;
@@ -599,17 +603,26 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     }
     const bool try_change = !(sh.flag & 1);
     x.sync();
+    const long long c1 = x.clock();
+    if (tid == 0) sh.st[kStatTimeGather] += (int)(c1 - c0);
     bool done = false;
     if (try_change) {
         done = face_change<F>(x, sh, J, W, G, fg, fcur, fprev);
         if (done && tid == 0) sh.st[kStatFacesChange] += 1;
     }
+    const long long c2 = x.clock();
+    if (tid == 0) sh.st[kStatTimeChange] += (int)(c2 - c1);
     if (!done) {
         // LEVEL needs ranks only; the sort may leave the keys permuted
         const bool sw = radix_sort(x, sh, W.key, W.val, W.key2, W.val2, fg.F);
         const int *ord = sw ? W.val2 : W.val;
+        const long long c3 = x.clock();
         face_level<F>(x, sh, J, W, G, fg, ord, fcur);
-        if (tid == 0) sh.st[kStatFacesLevel] += 1;
+        if (tid == 0) {
+            sh.st[kStatFacesLevel] += 1;
+            sh.st[kStatTimeSort] += (int)(c3 - c2);
+            sh.st[kStatTimeLevel] += (int)(x.clock() - c3);
+        }
     }
     // Only a CHANGE face is a valid reference for later skips: there every node
     // started below 1e9 (no new-edge or new-face trials) and set its own
@@ -641,6 +654,7 @@ FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
         sh.face_ctr = 0;
     }
     x.sync();
+    if (tid == 0) sh.st[kStatTimeInit] = 0;
     Geo G;
     G.h = J.h; G.dq = J.dq; G.df = J.df; G.y0 = J.y0; G.z0r = J.z0r; G.sinq = J.sinq;
     G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;

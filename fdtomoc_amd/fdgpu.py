@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(PKG, "libfdtomoc_amd.so")
 EXE_PATH = os.path.join(PKG, "bin", "sphfd")
 
 STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",
-              "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + ["skips"]
+              "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
+    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init"]
 
 BATCHED, PLAIN = 0, 1
 

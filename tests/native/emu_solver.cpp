@@ -7,6 +7,7 @@
 // It is not part of the product and is never loaded by it.
 #include <algorithm>
 #include <barrier>
+#include <chrono>
 #include <memory>
 #include <cstring>
 #include <thread>
@@ -27,6 +28,10 @@ struct HostExec {
     unsigned atomic_sub_u(unsigned *p, unsigned v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
     unsigned atomic_or(unsigned *p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
     unsigned atomic_and(unsigned *p, unsigned v) { return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
+    long long clock() const {
+        return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+                   .count() / 10;
+    }
 };
 }  // namespace
 

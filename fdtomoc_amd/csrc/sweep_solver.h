@@ -159,7 +159,10 @@ FD_INLINE void load_nbhd(const TableJob &J, const Geo &G, const FaceGeom &fg, co
                          int p, int a, int b, int i, int j, int k, Nbhd &N) {
     using S = FaceSpec<F>;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
-    const unsigned kp = CHG ? key[p] : 0u;
+    // every load below is unconditional and independent, so all of them are in
+    // flight together; the CHANGE selection happens afterwards on registers
+    unsigned kq[9];
+    float oq[9];
 #pragma unroll
     for (int pl = 0; pl < 2; pl++)
 #pragma unroll
@@ -175,20 +178,34 @@ FD_INLINE void load_nbhd(const TableJob &J, const Geo &G, const FaceGeom &fg, co
                 cj = cj < 0 ? 0 : (cj >= J.ny ? J.ny - 1 : cj);
                 ck = ck < 0 ? 0 : (ck >= J.nz ? J.nz - 1 : ck);
                 const int q = pl * 9 + (db + 1) * 3 + (da + 1);
-                float tv = J.time[(long long)ck * nxy + (long long)cj * nx + ci];
-                if (CHG && pl == 0 && (da | db)) {
-                    const int aa = a + da, bb = b + db;
-                    if (aa >= 0 && bb >= 0 && aa < fg.wa && bb < fg.wb) {
-                        const int fq = bb * fg.wa + aa;
-                        if (!key_less(key[fq], fq, kp, p)) tv = O[fq];
-                    }
-                }
-                N.t[q] = tv;
+                N.t[q] = J.time[(long long)ck * nxy + (long long)cj * nx + ci];
                 N.s[q] = J.slow[(long long)ck * J.s_sz + (long long)cj * J.s_sy + ci];
+                if (CHG && pl == 0) {
+                    int aa = a + da, bb = b + db;
+                    aa = aa < 0 ? 0 : (aa >= fg.wa ? fg.wa - 1 : aa);
+                    bb = bb < 0 ? 0 : (bb >= fg.wb ? fg.wb - 1 : bb);
+                    kq[q] = key[bb * fg.wa + aa];
+                    oq[q] = O[bb * fg.wa + aa];
+                }
             }
     N.sq[0] = G.sinq[j];
     N.sq[1] = G.sinq[j + 1];
     N.sq[2] = G.sinq[j + 2];
+    if (CHG) {
+        const unsigned kp = kq[4];
+#pragma unroll
+        for (int db = -1; db <= 1; db++)
+#pragma unroll
+            for (int da = -1; da <= 1; da++) {
+                if (!(da | db)) continue;
+                const int q = (db + 1) * 3 + (da + 1);
+                const int aa = a + da, bb = b + db;
+                const bool inside = aa >= 0 && bb >= 0 && aa < fg.wa && bb < fg.wb;
+                const int fq = bb * fg.wa + aa;
+                // a follower (later in face order) contributes its pre-face value
+                if (inside && !key_less(kq[q], fq, kp, p)) N.t[q] = oq[q];
+            }
+    }
 }
 
 // Latest change stamp over a node's 3x3x2 neighbourhood (its every input).

@@ -15,6 +15,7 @@ on a bounded sample of the same tables, whose outputs are also compared with
 the GPU tables bit for bit.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N] [--plain-sample M]
+    python bench.py --calibrate-copy MIB     # one known-size copy, for HBM counter scaling
 """
 from __future__ import annotations
 
@@ -92,6 +93,20 @@ def cpu_reference(g, vp, vs, stations, sample: int, threads: int):
         shutil.rmtree(wd, ignore_errors=True)
 
 
+def calibrate_copy(mib: int) -> None:
+    """One device-to-device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of a profiled run (each should read MIB MiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.arange(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    assert bool(torch.equal(a[-4:], b[-4:]))
+    print(json.dumps({"copy_bytes": 4 * n, "read_bytes": 4 * n, "write_bytes": 4 * n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +116,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the station tables shard across GPUs with no "
               "communication on the hot path, so multi-GPU runs are not part of it.", file=sys.stderr)
@@ -152,11 +171,13 @@ def main():
     achieved = BYTES_PER_VISIT * visits / float(np.mean(kern_s)) / 1e9
 
     # plain path: one table per launch, on a sample
+    plain_s = None
     psel = list(range(min(args.plain_sample, ntab)))
-    pids, pdt, _ = step(fdgpu.PLAIN, psel)
-    for i in pids:
-        ctx.release(i)
-    plain_s = len(psel) / pdt
+    if psel:
+        pids, pdt, _ = step(fdgpu.PLAIN, psel)
+        for i in pids:
+            ctx.release(i)
+        plain_s = len(psel) / pdt
 
     out = {
         "metric": METRIC, "value": round(stations_s, 3), "unit": "station-solves/s",
@@ -167,8 +188,8 @@ def main():
                                "+-5% 20 km checkerboard, reverse=50, one batched launch per step",
                    "tables": ntab, "nodes_per_table": n, "reverse_passes_mean": float(restarts.mean())},
         "nodes_per_s": round(nodes_s, 1),
-        "plain_station_solves_per_s": round(plain_s, 4),
-        "batched_over_plain": round(stations_s / plain_s, 2),
+        "plain_station_solves_per_s": None if plain_s is None else round(plain_s, 4),
+        "batched_over_plain": None if plain_s is None else round(stations_s / plain_s, 2),
         "latency_ms": {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(wall, 99)) * 1e3, 2),
                        "what": "batched solve of all 96 tables (submit to results resident)"},

@@ -9,6 +9,14 @@ by `make -C oracle ref`).
                 the compiled reference matches the repaired file byte for byte)
   ref_cases.npz seeded small cases (tests/cases.py) solved by the compiled
                 reference: par values, velocities, output tables and headers
+  dropin_restart.npz
+                a complete sphfd run directory for the restart sources
+                (srctype 3 old table, srctype 2 wall file) and fill=1 (box
+                file): the input files, the par files and the table files the
+                compiled reference wrote.  The reference leaves xs/ys/zs and
+                the header source uninitialised for srctype 2/3
+                (src/sphfd.c:460-538), so its live run may abort at the source
+                check on another machine; the recorded run did not.
 
 Only data is stored: velocity grids, tables and par values.
 """
@@ -67,8 +75,81 @@ def ref_cases():
     np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **out)
 
 
+def restart_inputs():
+    """Input files and par files of the restart/fill run directory."""
+    from fdtomoc_amd import synth
+    g = synth.centred_grid(17, 15, 11, 1.0, lat=23.5, lon=121.0)
+    rng = np.random.default_rng(6)
+    vp = (synth.gradient_velocity(g, 5.0, 8.0, 30.0) * synth.gaussian_anomalies(g, rng, 3, 0.08, (2.0, 6.0))
+          ).astype(np.float32)
+    head = tables.make_header(g.nx, g.ny, g.nz, g.x0, g.y0, g.z0, g.df_deg(), g.dq_deg(), g.h)
+    hbytes = np.asarray(head).tobytes()
+    st = synth.random_surface_stations(g, np.random.default_rng(3), 1)[0]
+    par = ob.Par(nx=g.nx, ny=g.ny, nz=g.nz, h=g.h, x0=g.x0, y0=g.y0, z0=g.z0, fxs=st.lon, fys=st.lat,
+                 fzs=st.depth, reverse=50)
+    t_old, _ = ob.solve(par, ob.slowness(vp))
+    nwall = max(g.nx * g.ny, g.ny * g.nz, g.nx * g.nz)
+    walls = np.full(6 * nwall, -1.0, np.float32)
+    walls[:g.ny * g.nz] = t_old[:, :, 0].ravel() + 0.5              # x = 0 wall (k outer, j inner)
+    walls[4 * nwall:4 * nwall + g.nx * g.ny] = t_old[0].ravel()     # z = 0 wall (j outer, i inner)
+    box = np.arange(125, dtype=np.float32) * 0.01 + 0.05
+    files = {
+        "VP.mod": hbytes + vp.astype("<f4").tobytes(),
+        # oldtfile is read from byte 0 (src/sphfd.c:931): a headerless grid
+        "old.ptimes": t_old.astype("<f4").tobytes(),
+        "walls.bin": walls.tobytes(),
+        "box.bin": hbytes + box.tobytes(),
+    }
+    base = f"nx={g.nx} ny={g.ny} nz={g.nz} x0={g.x0} y0={g.y0} z0=0 h=1.0 velfile=VP.mod reverse=3\n"
+    # the srctype-1 table goes first: run in one thread, the reference's
+    # srctype 2/3 calls then find the previous call's (in-model) source
+    # position in their uninitialised xs/ys/zs
+    pars = [
+        f"fxs={st.lon}\nfys={st.lat}\nfzs={st.depth}\nnx={g.nx}\nny={g.ny}\nnz={g.nz}\nx0={g.x0}\n"
+        f"y0={g.y0}\nz0=0\nh=1.0\nreverse=50\ntimefile=FB.ptimes\nvelfile=VP.mod\nfill=1\nboxfile=box.bin\n",
+        base + "srctype=3 srcwall=5 oldtfile=old.ptimes timefile=R3.ptimes\n",
+        base + "srctype=2 srcwall=1 wallfile=walls.bin timefile=R2.ptimes\n",
+    ]
+    return files, pars
+
+
+def write_run_dir(wd, files, pars):
+    """A run directory the reference's main reads: spec -> parlist -> par files."""
+    for name, data in files.items():
+        with open(os.path.join(wd, name), "wb") as f:
+            f.write(data)
+    lines = []
+    for t, txt in enumerate(pars):
+        with open(os.path.join(wd, f"t{t}.par"), "w") as f:
+            f.write(txt)
+        lines.append(f"par=t{t}.par")
+    with open(os.path.join(wd, "parlist"), "w") as f:
+        f.write("\n".join(lines) + "\n\n")
+    os.makedirs(os.path.join(wd, "out"), exist_ok=True)
+    with open(os.path.join(wd, "spec"), "w") as f:
+        f.write("# sphfd run\nparlist parlist\ntimedir out\n")
+
+
+def dropin_restart():
+    import subprocess
+    import tempfile
+    files, pars = restart_inputs()
+    wd = tempfile.mkdtemp(prefix="golden_restart_")
+    write_run_dir(wd, files, pars)
+    r = subprocess.run([ob.REF_EXE], input="spec\n", cwd=wd, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = {"pars": np.array(pars)}
+    for name, data in files.items():
+        out["in_" + name] = np.frombuffer(data, np.uint8)
+    for name in sorted(os.listdir(os.path.join(wd, "out"))):
+        out["out_" + name] = np.fromfile(os.path.join(wd, "out", name), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "dropin_restart.npz"), **out)
+
+
 if __name__ == "__main__":
     m001()
     ref_cases()
-    for f in ("m001.npz", "ref_cases.npz"):
+    dropin_restart()
+    for f in ("m001.npz", "ref_cases.npz", "dropin_restart.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -40,3 +40,114 @@ def test_kernel_matches_oracle_batched(ctx, seed):
     assert gst["error"] & ~1 == 0
     assert gst["passes"] == st["passes"]
     assert _bits_equal(t, t_ref) == 0
+
+
+def _golden_par(keys, vals):
+    ints = {"nx", "ny", "nz", "iminus", "iplus", "jminus", "jplus", "kminus", "kplus", "floatsrc", "srctype",
+            "srcwall", "xs", "ys", "zs", "ncube", "reverse", "headpref", "parse", "fill", "invh", "invq", "invf",
+            "invx0", "invy0", "invz0"}
+    return ob.Par(**{str(k): (int(v) if str(k) in ints else float(v)) for k, v in zip(keys, vals)})
+
+
+def test_golden_fixtures_one_launch(ctx):
+    """M001 (the reference's shipped verification table) and the 48 seeded
+    cases the compiled reference solved, all in ONE batched launch, against
+    the fixture times bit for bit."""
+    import os
+    from fdtomoc_amd import fdgpu
+    gdir = os.path.join(os.path.dirname(__file__), "golden")
+    z = np.load(os.path.join(gdir, "ref_cases.npz"))
+    cases = []
+    for s in sorted({k.split("_")[0] for k in z.files if k.startswith("s")}):
+        cases.append((s, _golden_par(z["par_keys"], z[f"{s}_par"]), z[f"{s}_vel"], z[f"{s}_times"]))
+    m1 = np.load(os.path.join(gdir, "m001.npz"))
+    cases.append(("m001", _golden_par(m1["par_keys"], m1["par_vals"]), m1["vel"], m1["times"]))
+    models = [ctx.model(v) for _, _, v, _ in cases]
+    specs = [fdgpu.TableSpec(par_dict(p), m) for (_, p, _, _), m in zip(cases, models)]
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    bad = []
+    for (name, _, v, t_ref), tid in zip(cases, ids):
+        t = ctx.read(tid, v.shape)
+        st = ctx.stats(tid)
+        if st["error"] & ~1 or _bits_equal(t, t_ref):
+            bad.append((name, st["error"], _bits_equal(t, t_ref)))
+        ctx.release(tid)
+    for m in models:
+        ctx.release_model(m)
+    print(f"{len(cases)} golden tables in one launch, mismatches {bad}", flush=True)
+    assert not bad
+
+
+@pytest.mark.parametrize("seed", [100, 101, 102, 103])
+def test_plain_mode_matches_oracle(ctx, seed):
+    """PLAIN launches one table at a time with the whole device (the drop-in
+    per-station call) -- same bits as BATCHED and the oracle."""
+    from fdtomoc_amd import fdgpu
+    par, v = make_case(seed, 12, 40)
+    t_ref, st = ob.solve(par, ob.slowness(v))
+    m = ctx.model(v)
+    (tid,) = ctx.solve([fdgpu.TableSpec(par_dict(par), m)], fdgpu.PLAIN)
+    t = ctx.read(tid, v.shape)
+    ctx.release(tid)
+    ctx.release_model(m)
+    assert _bits_equal(t, t_ref) == 0
+
+
+def test_windows_of_one_resident_model(ctx):
+    """Tables that cover different sub-blocks (origins) of one resident model
+    equal the oracle run on the cut-out block."""
+    from fdtomoc_amd import fdgpu, synth
+    rng = np.random.default_rng(77)
+    G = synth.centred_grid(48, 40, 24, 1.0)
+    V = (synth.gradient_velocity(G, 4.5, 7.5, 24.0) * synth.gaussian_anomalies(G, rng, 4, 0.1, (3, 8))).astype(np.float32)
+    m = ctx.model(V)
+    specs, refs = [], []
+    for w in range(6):
+        nx, ny, nz = int(rng.integers(10, 30)), int(rng.integers(10, 28)), int(rng.integers(6, 20))
+        ox, oy, oz = int(rng.integers(0, 48 - nx + 1)), int(rng.integers(0, 40 - ny + 1)), int(rng.integers(0, 24 - nz + 1))
+        sub = np.ascontiguousarray(V[oz:oz + nz, oy:oy + ny, ox:ox + nx])
+        g = synth.centred_grid(nx, ny, nz, 1.0, lat=float(rng.uniform(-40, 40)), lon=float(rng.uniform(-100, 100)))
+        lon, lat = g.node_lonlat(rng.uniform(0, nx - 1), rng.uniform(0, ny - 1))
+        par = ob.Par(nx=nx, ny=ny, nz=nz, h=1.0, x0=g.x0, y0=g.y0, z0=0.0, fxs=round(lon, 5), fys=round(lat, 5),
+                     fzs=round(float(rng.uniform(0, nz - 1)), 3), reverse=50)
+        specs.append(fdgpu.TableSpec(par_dict(par), m, (ox, oy, oz)))
+        refs.append(ob.solve(par, ob.slowness(sub))[0])
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    for tid, t_ref in zip(ids, refs):
+        t = ctx.read(tid, t_ref.shape)
+        ctx.release(tid)
+        assert _bits_equal(t, t_ref) == 0
+    ctx.release_model(m)
+
+
+def test_model_update_then_resolve(ctx):
+    """A resident model updated in place (a tomography iteration) gives the
+    oracle's tables for the new velocities."""
+    from fdtomoc_amd import fdgpu
+    par, v = make_case(200, 14, 30)
+    m = ctx.model(v)
+    for it in range(3):
+        v2 = (v * np.float32(1.0 + 0.02 * it)).astype(np.float32)
+        v2[: v.shape[0] // 2] *= np.float32(0.97)
+        ctx.update_model(m, v2)
+        (tid,) = ctx.solve([fdgpu.TableSpec(par_dict(par), m)], fdgpu.BATCHED)
+        t = ctx.read(tid, v.shape)
+        ctx.release(tid)
+        assert _bits_equal(t, ob.solve(par, ob.slowness(v2))[0]) == 0, it
+    ctx.release_model(m)
+
+
+def test_error_paths(ctx):
+    """Bad requests are refused with the documented codes, not solved."""
+    from fdtomoc_amd import fdgpu
+    par, v = make_case(3)
+    m = ctx.model(v)
+    d = par_dict(par)
+    with pytest.raises(fdgpu.FdgpuError):     # window outside the model
+        ctx.solve([fdgpu.TableSpec(d, m, (1, 0, 0))])
+    far = dict(d, fxs=d["fxs"] + 50.0)
+    with pytest.raises(fdgpu.FdgpuError, match="source"):
+        ctx.solve([fdgpu.TableSpec(far, m)])
+    with pytest.raises(fdgpu.FdgpuError):     # unknown model
+        ctx.solve([fdgpu.TableSpec(d, m + 1000)])
+    ctx.release_model(m)

@@ -216,3 +216,32 @@ def test_homogeneous_medium_close_to_chord_time():
     err = np.abs(t - exact)
     assert err.max() < 0.05          # seconds; cell transit is 1/6 s
     assert (t >= 0).all() and (t < 1e9).all()
+
+
+def test_restart_sources_match_reference_run():
+    """srctype 3 (old table), srctype 2 (six wall planes) and fill=1 (box
+    values) against the table files the compiled reference wrote
+    (tests/golden/dropin_restart.npz)."""
+    import re
+    z = np.load(os.path.join(GOLDEN, "dropin_restart.npz"))
+    vel_raw = z["in_VP.mod"].tobytes()
+    nums = {}
+    for txt in z["pars"]:
+        kv = dict(re.findall(r"(\w+)=(\S+)", str(txt)))
+        name = kv["timefile"]
+        nx, ny, nz = int(kv["nx"]), int(kv["ny"]), int(kv["nz"])
+        vel = np.frombuffer(vel_raw[232:], "<f4").reshape(nz, ny, nx)
+        kw = dict(nx=nx, ny=ny, nz=nz, h=float(kv["h"]), x0=float(kv["x0"]), y0=float(kv["y0"]),
+                  z0=float(kv["z0"]), reverse=int(kv["reverse"]))
+        if "srctype" in kv:
+            kw.update(srctype=int(kv["srctype"]), srcwall=int(kv["srcwall"]))
+            src = "old.ptimes" if kw["srctype"] == 3 else "walls.bin"
+            extra = np.frombuffer(z["in_" + src].tobytes(), "<f4")
+        else:
+            kw.update(fxs=float(kv["fxs"]), fys=float(kv["fys"]), fzs=float(kv["fzs"]), fill=1)
+            extra = np.frombuffer(z["in_box.bin"].tobytes()[232:], "<f4")
+        t, st = ob.solve(ob.Par(**kw), ob.slowness(vel), extra)
+        ref = np.frombuffer(z["out_" + name].tobytes()[232:], "<f4").reshape(nz, ny, nx)
+        nums[name] = st["passes"]
+        assert bits_diff(t, ref) == 0, name
+    assert set(nums) == {"R3.ptimes", "R2.ptimes", "FB.ptimes"}

@@ -38,6 +38,10 @@ from fdtomoc_amd import fdgpu, synth, tables  # noqa: E402
 METRIC = "stations/s & nodes/s batched vs plain at <=1e-4 s err; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VISIT = 12          # slowness read + time read + time write per node visit
+# HBM bytes (FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0, copy-calibrated) that
+# rocprofv3 counted for one launch of the default 96-table workload
+# (profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md)
+TRAFFIC_BYTES_C2 = 367.03e9
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 
 
@@ -194,7 +198,8 @@ def main():
                        "p99": round(float(np.percentile(wall, 99)) * 1e3, 2),
                        "what": "batched solve of all 96 tables (submit to results resident)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_C2,
+                     "traffic_source": "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md",
                      "kernel": "sphfd_sweep_kernel", "kernel_ms": round(float(np.mean(kern_s)) * 1e3, 2),
                      "algorithmic_bytes": BYTES_PER_VISIT * visits},
     }

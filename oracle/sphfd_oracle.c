@@ -24,6 +24,21 @@
  * t[0]/t[4] reads of fdsph2d (sphfd.c:6116-6121).
  *
  * Arithmetic: FP64 with FP32 storage; compile with -ffp-contract=off.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   oracle_solve     bit-identical to verification/cmod/M001.ptimes (the
+ *                    reference's shipped table, CR-repaired), to 48 seeded
+ *                    tables and a restart/fill run directory written by the
+ *                    reference compiled from its own sources (tests/golden/),
+ *                    and to live runs of that build (oracle/_ref) where it
+ *                    is present; homogeneous-medium chord-time bound.
+ *   oracle_fdsph3d   Cartesian-limit known answer (planar field, r -> inf)
+ *   oracle_fdsphne,  exact on planar fields with causal configurations
+ *   oracle_fdsph2d,  (hand-solved quadratic), acausal branch returns 1e11
+ *   oracle_fdsphnf
+ *   oracle_glath     closed forms at the equator and pole, and an
+ *                    independent WGS84 evaluation
+ *   oracle_geometry  the M001 header source (lon, colatitude, radius)
  */
 #include "sphfd_oracle.h"
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "serve_shared.h"
 #include "sweep_solver.h"
 
 #define FD_SWEEP_THREADS 256
@@ -10,3 +11,6 @@
 extern "C" hipError_t fd_launch_sweep(const fdsweep::TableJob *d_jobs, int njobs, const fdsweep::Scratch *d_slots,
                                       int nslots, int *d_queue, hipStream_t stream);
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream);
+extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done,
+                                      int cap, const fdsweep::Scratch *d_slots, int nslots, fdsweep::ServeDev *dev,
+                                      long long idle_ticks, hipStream_t stream);

@@ -19,54 +19,7 @@
 #include "sphfd_io.h"
 #include "table_prep.h"
 
-namespace {
-
-size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
-
-struct Model {
-    int nx = 0, ny = 0, nz = 0;
-    float *d_slow = nullptr;
-    size_t bytes = 0;
-    double vmin = 0.0, vmax = 0.0;   // velocity range (skip-safety bound)
-};
-
-struct Table {
-    fdgpu_par par{};
-    int model = -1;
-    int ox = 0, oy = 0, oz = 0;
-    long long n = 0;
-    float *d_time = nullptr;
-    char *d_aux = nullptr;     // sinq | init_idx | init_dist | init_val | stats
-    size_t bytes = 0;
-    int *d_stats = nullptr;
-    std::vector<int> stats;
-    fdprep::Prepared prep;
-};
-
-}  // namespace
-
-struct fdgpu_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    unsigned long long budget = 0, used = 0;
-    int num_cu = 0;
-    int wg_per_cu = 1;
-    std::map<int, Model> models;
-    std::map<int, Table> tables;
-    int next_id = 1;
-    // per-workgroup scratch, grown on demand
-    char *d_scratch = nullptr;
-    size_t scratch_bytes = 0;
-    fdsweep::Scratch *d_slots = nullptr;
-    int slots_cap = 0;
-    long long slot_face_cap = 0;
-    int *d_queue = nullptr;
-    double last_ms = 0.0;
-    std::mutex mu;
-};
-
-namespace {
+namespace fdgpu_detail {
 
 int hip_fail(hipError_t e) { return e == hipErrorOutOfMemory ? FDGPU_ENOMEM : FDGPU_EHIP; }
 
@@ -94,26 +47,12 @@ size_t slot_bytes(long long F) {
     return 15 * f + align256((size_t)F);
 }
 
-int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
-    if (nslots <= c->slots_cap && maxF <= c->slot_face_cap) return FDGPU_OK;
-    const int ns = std::max(nslots, c->slots_cap);
-    const long long F = std::max(maxF, c->slot_face_cap);
-    dev_free(c, c->d_scratch, c->scratch_bytes);
-    dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
-    c->d_scratch = nullptr;
-    c->d_slots = nullptr;
-    c->slots_cap = 0;
-    c->slot_face_cap = 0;
+int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots) {
     const size_t per = slot_bytes(F);
-    int rc = dev_alloc(c, (void **)&c->d_scratch, per * (size_t)ns);
-    if (rc) return rc;
-    c->scratch_bytes = per * (size_t)ns;
-    rc = dev_alloc(c, (void **)&c->d_slots, sizeof(fdsweep::Scratch) * (size_t)ns);
-    if (rc) return rc;
-    std::vector<fdsweep::Scratch> h((size_t)ns);
     const size_t f = align256(sizeof(float) * (size_t)F);
+    std::vector<fdsweep::Scratch> h((size_t)ns);
     for (int s = 0; s < ns; s++) {
-        char *b = c->d_scratch + per * (size_t)s;
+        char *b = base + per * (size_t)s;
         fdsweep::Scratch &W = h[(size_t)s];
         size_t o = 0;
         auto take = [&](size_t n) { char *p = b + o; o += n; return p; };
@@ -134,7 +73,28 @@ int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
         W.stamp = (int *)take(f);
         W.flag = (unsigned char *)take(align256((size_t)F));
     }
-    FD_HIP(hipMemcpy(c->d_slots, h.data(), sizeof(fdsweep::Scratch) * (size_t)ns, hipMemcpyHostToDevice));
+    FD_HIP(hipMemcpy(d_slots, h.data(), sizeof(fdsweep::Scratch) * (size_t)ns, hipMemcpyHostToDevice));
+    return FDGPU_OK;
+}
+
+int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
+    if (nslots <= c->slots_cap && maxF <= c->slot_face_cap) return FDGPU_OK;
+    const int ns = std::max(nslots, c->slots_cap);
+    const long long F = std::max(maxF, c->slot_face_cap);
+    dev_free(c, c->d_scratch, c->scratch_bytes);
+    dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
+    c->d_scratch = nullptr;
+    c->d_slots = nullptr;
+    c->slots_cap = 0;
+    c->slot_face_cap = 0;
+    const size_t per = slot_bytes(F);
+    int rc = dev_alloc(c, (void **)&c->d_scratch, per * (size_t)ns);
+    if (rc) return rc;
+    c->scratch_bytes = per * (size_t)ns;
+    rc = dev_alloc(c, (void **)&c->d_slots, sizeof(fdsweep::Scratch) * (size_t)ns);
+    if (rc) return rc;
+    rc = fill_slots(c->d_scratch, ns, F, c->d_slots);
+    if (rc) return rc;
     c->slots_cap = ns;
     c->slot_face_cap = F;
     return FDGPU_OK;
@@ -154,6 +114,26 @@ fdprep::ParValues to_values(const fdgpu_par &p) {
     return v;
 }
 
+int prepare_table(const fdgpu_table_desc &d, int mnx, int mny, int mnz, Table &T, std::vector<float> &preset) {
+    const fdgpu_par &p = d.par;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1 || d.ox < 0 || d.oy < 0 || d.oz < 0 || d.ox + p.nx > mnx ||
+        d.oy + p.ny > mny || d.oz + p.nz > mnz || p.ncube < 0)
+        return FDGPU_EINPUT;
+    T.par = p;
+    T.model = d.model;
+    T.ox = d.ox; T.oy = d.oy; T.oz = d.oz;
+    T.n = (long long)p.nx * p.ny * p.nz;
+    const fdprep::ParValues v = to_values(p);
+    int rc = fdprep::prepare_geometry(v, T.prep);
+    if (rc) return rc;
+    if (p.srctype != 1) {
+        if (!d.preset) return FDGPU_EINPUT;
+        preset.resize((size_t)T.n);
+    }
+    if (p.fill == 1 && !d.boxvals) return FDGPU_EINPUT;
+    return fdprep::prepare_source(v, T.prep, d.preset, d.boxvals, p.srctype != 1 ? preset.data() : nullptr);
+}
+
 void release_table(fdgpu_ctx *c, Table &t) {
     dev_free(c, t.d_time, sizeof(float) * (size_t)t.n);
     dev_free(c, t.d_aux, t.bytes);
@@ -161,7 +141,9 @@ void release_table(fdgpu_ctx *c, Table &t) {
     t.d_aux = nullptr;
 }
 
-}  // namespace
+}  // namespace fdgpu_detail
+
+using namespace fdgpu_detail;
 
 extern "C" {
 
@@ -190,7 +172,8 @@ const char *fdgpu_strerror(int code) {
     case FDGPU_EPAR: return "required parameter missing";
     case FDGPU_ESTOP: return "stop due to STOP in input";
     case FDGPU_EKERNEL: return "sweep kernel reported an internal error";
-    case FDGPU_EBUSY: return "serving queue full";
+    case FDGPU_EBUSY: return "busy: serving arena, staging or queue full, or call not allowed while serving";
+    case FDGPU_ETIMEOUT: return "timed out waiting for a request";
     }
     return "unknown error";
 }
@@ -226,6 +209,10 @@ int fdgpu_open(int device, unsigned long long budget, fdgpu_ctx **out) {
 int fdgpu_close(fdgpu_ctx *c) {
     if (!c) return FDGPU_OK;
     (void)hipSetDevice(c->device);
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        (void)serve_stop_locked(c);
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->tables) release_table(c, kv.second);
     for (auto &kv : c->models) dev_free(c, kv.second.d_slow, kv.second.bytes);
@@ -242,6 +229,7 @@ int fdgpu_close(fdgpu_ctx *c) {
 int fdgpu_model_create(fdgpu_ctx *c, int nx, int ny, int nz, const float *vel, int *model) {
     if (!c || !vel || !model || nx <= 0 || ny <= 0 || nz <= 0) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;   // no allocation while the request loop runs
     FD_HIP(hipSetDevice(c->device));
     Model m;
     m.nx = nx; m.ny = ny; m.nz = nz;
@@ -259,6 +247,7 @@ int fdgpu_model_create(fdgpu_ctx *c, int nx, int ny, int nz, const float *vel, i
 int fdgpu_model_update(fdgpu_ctx *c, int model, const float *vel) {
     if (!c || !vel) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;
     FD_HIP(hipSetDevice(c->device));
     return fdgpu_model_update_locked(c, model, vel);
 }
@@ -266,6 +255,7 @@ int fdgpu_model_update(fdgpu_ctx *c, int model, const float *vel) {
 int fdgpu_model_release(fdgpu_ctx *c, int model) {
     if (!c) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;
     auto it = c->models.find(model);
     if (it == c->models.end()) return FDGPU_EINPUT;
     (void)hipStreamSynchronize(c->stream);
@@ -277,6 +267,7 @@ int fdgpu_model_release(fdgpu_ctx *c, int model) {
 int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int *ids) {
     if (!c || (!desc && n > 0) || n < 0 || !ids) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;
     FD_HIP(hipSetDevice(c->device));
     // prepare every table on the host first: no device work on bad input
     std::vector<Table> tabs((size_t)n);
@@ -287,26 +278,9 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         auto mit = c->models.find(d.model);
         if (mit == c->models.end()) return FDGPU_EINPUT;
         const Model &m = mit->second;
-        const fdgpu_par &p = d.par;
-        if (p.nx < 1 || p.ny < 1 || p.nz < 1 || d.ox < 0 || d.oy < 0 || d.oz < 0 || d.ox + p.nx > m.nx ||
-            d.oy + p.ny > m.ny || d.oz + p.nz > m.nz || p.ncube < 0)
-            return FDGPU_EINPUT;
-        Table &T = tabs[(size_t)t];
-        T.par = p;
-        T.model = d.model;
-        T.ox = d.ox; T.oy = d.oy; T.oz = d.oz;
-        T.n = (long long)p.nx * p.ny * p.nz;
-        const fdprep::ParValues v = to_values(p);
-        int rc = fdprep::prepare_geometry(v, T.prep);
+        int rc = prepare_table(d, m.nx, m.ny, m.nz, tabs[(size_t)t], presets[(size_t)t]);
         if (rc) return rc;
-        if (p.srctype != 1) {
-            if (!d.preset) return FDGPU_EINPUT;
-            presets[(size_t)t].resize((size_t)T.n);
-        }
-        if (p.fill == 1 && !d.boxvals) return FDGPU_EINPUT;
-        rc = fdprep::prepare_source(v, T.prep, d.preset, d.boxvals, p.srctype != 1 ? presets[(size_t)t].data() : nullptr);
-        if (rc) return rc;
-        maxF = std::max(maxF, fdprep::max_face(p.nx, p.ny, p.nz));
+        maxF = std::max(maxF, fdprep::max_face(d.par.nx, d.par.ny, d.par.nz));
     }
     // device buffers per table
     std::vector<fdsweep::TableJob> jobs((size_t)n);
@@ -444,7 +418,9 @@ int fdgpu_table_read(fdgpu_ctx *c, int table, float *host, unsigned long long co
     auto it = c->tables.find(table);
     if (it == c->tables.end() || count < (unsigned long long)it->second.n) return FDGPU_EINPUT;
     FD_HIP(hipSetDevice(c->device));
-    FD_HIP(hipMemcpy(host, it->second.d_time, sizeof(float) * (size_t)it->second.n, hipMemcpyDeviceToHost));
+    FD_HIP(hipMemcpyAsync(host, it->second.d_time, sizeof(float) * (size_t)it->second.n, hipMemcpyDeviceToHost,
+                          c->stream));
+    FD_HIP(hipStreamSynchronize(c->stream));
     return FDGPU_OK;
 }
 
@@ -460,6 +436,7 @@ int fdgpu_table_stats(fdgpu_ctx *c, int table, int *stats, int nstats) {
 int fdgpu_table_release(fdgpu_ctx *c, int table) {
     if (!c) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;
     auto it = c->tables.find(table);
     if (it == c->tables.end()) return FDGPU_EINPUT;
     (void)hipSetDevice(c->device);
@@ -488,7 +465,9 @@ int fdgpu_table_write(fdgpu_ctx *c, int table, const char *path, const void *vel
         if (it == c->tables.end()) return FDGPU_EINPUT;
         host.resize((size_t)it->second.n);
         FD_HIP(hipSetDevice(c->device));
-        FD_HIP(hipMemcpy(host.data(), it->second.d_time, sizeof(float) * host.size(), hipMemcpyDeviceToHost));
+        FD_HIP(hipMemcpyAsync(host.data(), it->second.d_time, sizeof(float) * host.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        FD_HIP(hipStreamSynchronize(c->stream));
         par = it->second.par;
         prep = it->second.prep;
     }

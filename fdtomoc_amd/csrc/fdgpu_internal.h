@@ -1,7 +1,176 @@
-// fdgpu_internal.h -- declarations shared by the host translation units.
+// fdgpu_internal.h -- the device context and helpers shared by the host
+// translation units of the C ABI (fdgpu.cpp: models, batched solves, tables;
+// serve.cpp: the request loop).
 #pragma once
 
-struct fdgpu_ctx;
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fdtomoc_amd.h"
+#include "serve_shared.h"
+#include "sweep_solver.h"
+#include "table_prep.h"
+
+namespace fdgpu_detail {
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Model {
+    int nx = 0, ny = 0, nz = 0;
+    float *d_slow = nullptr;
+    size_t bytes = 0;
+    double vmin = 0.0, vmax = 0.0;   // velocity range (skip-safety bound)
+};
+
+struct Table {
+    fdgpu_par par{};
+    int model = -1;
+    int ox = 0, oy = 0, oz = 0;
+    long long n = 0;
+    float *d_time = nullptr;
+    char *d_aux = nullptr;     // sinq | init_idx | init_dist | init_val | stats | stamps | last_full
+    size_t bytes = 0;
+    int *d_stats = nullptr;
+    std::vector<int> stats;
+    fdprep::Prepared prep;
+};
+
+// First-fit allocator of byte ranges inside one block (HBM arena or pinned
+// staging area); 256-byte granules.
+class RangeAlloc {
+  public:
+    void reset(size_t bytes) {
+        free_.clear();
+        if (bytes) free_[0] = bytes;
+        total_ = bytes;
+        used_ = 0;
+    }
+    // offset of a new range, or SIZE_MAX
+    size_t take(size_t bytes) {
+        bytes = align256(bytes ? bytes : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > bytes) free_[off + bytes] = len - bytes;
+            used_ += bytes;
+            return off;
+        }
+        return SIZE_MAX;
+    }
+    void give(size_t off, size_t bytes) {
+        bytes = align256(bytes ? bytes : 1);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+    size_t total() const { return total_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t total_ = 0, used_ = 0;
+};
+
+// One admitted serving request.
+struct ServeReq {
+    unsigned long long seq = 0;
+    fdgpu_par par{};
+    long long n = 0;
+    size_t a_off = 0, a_bytes = 0;   // HBM arena block
+    size_t s_off = 0, s_bytes = 0;   // pinned staging block (freed on completion)
+    float *d_time = nullptr;
+    int *d_stats = nullptr;
+    bool done = false;
+    std::vector<int> stats;
+    fdprep::Prepared prep;
+};
+
+struct Serve {
+    bool on = false;
+    bool launched = false;             // a server kernel may be running
+    hipStream_t sstream = nullptr;     // the server kernel
+    hipStream_t cstream = nullptr;     // read-back copies while it runs
+    hipEvent_t kdone = nullptr;        // recorded after each server launch
+    fdsweep::ServeCtl *ctl = nullptr;  // pinned, coherent
+    fdsweep::ServeJob *ring = nullptr; // pinned, coherent, `cap` entries
+    unsigned long long *done = nullptr;// pinned, coherent, `cap` entries
+    int cap = 0;
+    fdsweep::ServeDev *d_dev = nullptr;
+    char *arena = nullptr;             // HBM for tables of admitted requests
+    RangeAlloc arena_alloc;
+    char *stage = nullptr;             // pinned, coherent: request inputs
+    RangeAlloc stage_alloc;
+    float *bounce = nullptr;           // pinned: read-back buffer
+    size_t bounce_bytes = 0;
+    char *d_scratch = nullptr;
+    fdsweep::Scratch *d_slots = nullptr;
+    int nslots = 0;
+    long long max_face = 0;
+    int maxdim[3] = {0, 0, 0};
+    unsigned long long tail = 0;       // requests published
+    std::map<int, ServeReq> reqs;
+    std::map<unsigned long long, int> pending;   // seq -> request id, not yet seen done
+    long long idle_ticks = 0;
+    int launches = 0;
+};
+
+}  // namespace fdgpu_detail
+
+struct fdgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long budget = 0, used = 0;
+    int num_cu = 0;
+    int wg_per_cu = 1;
+    std::map<int, fdgpu_detail::Model> models;
+    std::map<int, fdgpu_detail::Table> tables;
+    int next_id = 1;
+    // per-workgroup scratch of batched solves, grown on demand
+    char *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    fdsweep::Scratch *d_slots = nullptr;
+    int slots_cap = 0;
+    long long slot_face_cap = 0;
+    int *d_queue = nullptr;
+    double last_ms = 0.0;
+    fdgpu_detail::Serve serve;
+    std::mutex mu;
+};
+
+namespace fdgpu_detail {
+
+int hip_fail(hipError_t e);
+int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes);
+void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
+size_t slot_bytes(long long F);
+// Carves `ns` scratch slots for faces of up to F nodes out of `base` and
+// uploads their descriptors to `d_slots`.
+int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots);
+fdprep::ParValues to_values(const fdgpu_par &p);
+// Host preparation of one table request (validation, geometry, source); `preset`
+// receives the srctype 2/3 initial grid.
+int prepare_table(const fdgpu_table_desc &d, int model_nx, int model_ny, int model_nz, Table &T,
+                  std::vector<float> &preset);
+// Serving: stops the request loop and frees its memory (context lock held).
+int serve_stop_locked(fdgpu_ctx *c);
+
+}  // namespace fdgpu_detail
 
 // Uploads velocities and converts them to slowness on the device; the caller
 // holds the context lock.

@@ -3,6 +3,10 @@
 //   sphfd_sweep_kernel   persistent: each workgroup pulls whole tables from a
 //                        queue and runs the complete sweep of each
 //                        (sweep_solver.h) with 256 lanes
+//   sphfd_serve_kernel   persistent request loop for serving: workgroups claim
+//                        requests published by the host while the kernel runs
+//                        (serve_shared.h), each solved exactly like a batched
+//                        table
 //   slowness_kernel      1/v in FP32 (sphfd.c:689-690), model upload
 //
 // Compiled with -ffp-contract=off: FP64 stencil arithmetic must round
@@ -10,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fd_kernels.h"
+#include "serve_shared.h"
 #include "sweep_solver.h"
 
 namespace {
@@ -41,6 +46,78 @@ __global__ void __launch_bounds__(FD_SWEEP_THREADS)
     }
 }
 
+// Lane 0 of an idle workgroup: claim the next published request, or return -1
+// when the host asked to stop or no workgroup has had work for `idle_ticks`
+// (wall-clock ticks).  Every path of the loop re-reads the host's tail and
+// stop words, so the loop ends as soon as either allows it.
+__device__ long long serve_claim(fdsweep::ServeCtl *ctl, fdsweep::ServeDev *dev, long long idle_ticks) {
+    while (true) {
+        const unsigned long long tail = __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        unsigned long long h = __hip_atomic_load(&dev->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (h < tail) {
+            if (__hip_atomic_compare_exchange_strong(&dev->head, &h, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_fetch_add(&dev->busy, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_max(&dev->last, (long long)wall_clock64(), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                return (long long)h;
+            }
+        }
+        if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) return -1;
+        const int busy = __hip_atomic_load(&dev->busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const long long last = __hip_atomic_load(&dev->last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (busy == 0 && (long long)wall_clock64() - last > idle_ticks) return -1;
+        for (int s = 0; s < 4; s++) __builtin_amdgcn_s_sleep(127);
+    }
+}
+
+__global__ void __launch_bounds__(FD_SWEEP_THREADS)
+    sphfd_serve_kernel(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done, int cap,
+                       const fdsweep::Scratch *slots, fdsweep::ServeDev *dev, long long idle_ticks) {
+    __shared__ fdsweep::Shared sh;
+    __shared__ fdsweep::ServeJob sj;
+    __shared__ long long sjob;
+    DevExec x;
+    const int tid = (int)threadIdx.x, T = (int)blockDim.x;
+    const fdsweep::Scratch W = slots[blockIdx.x];
+    if (tid == 0)
+        __hip_atomic_fetch_max(&dev->last, (long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (true) {
+        if (tid == 0) sjob = serve_claim(ctl, dev, idle_ticks);
+        __syncthreads();
+        const long long j = sjob;
+        if (j < 0) break;
+        // the descriptor lives in host memory: one copy into LDS
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        {
+            const unsigned *src = (const unsigned *)&ring[j % cap];
+            unsigned *dst = (unsigned *)&sj;
+            for (int q = tid; q < (int)(sizeof(fdsweep::ServeJob) / 4); q += T) dst[q] = src[q];
+        }
+        __syncthreads();
+        // stage the request's host inputs into its arena block
+        for (int q = tid; q < sj.nsinq; q += T) sj.sinq_dst[q] = sj.sinq_src[q];
+        if (sj.vel_src)
+            for (long long q = tid; q < sj.nvel; q += T) sj.slow_dst[q] = 1.0f / sj.vel_src[q];
+        if (sj.preset_src) {
+            const long long n = (long long)sj.job.nx * sj.job.ny * sj.job.nz;
+            for (long long q = tid; q < n; q += T) sj.job.time[q] = sj.preset_src[q];
+        }
+        __syncthreads();
+        fdsweep::solve_table(x, sh, sj.job, W);
+        // results out to HBM before the completion word reaches the host
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            __hip_atomic_store(&done[j % cap], (unsigned long long)j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_max(&dev->last, (long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&dev->busy, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void slowness_kernel(const float *__restrict__ vel, float *__restrict__ slow, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         slow[i] = 1.0f / vel[i];
@@ -52,6 +129,14 @@ extern "C" hipError_t fd_launch_sweep(const fdsweep::TableJob *d_jobs, int njobs
                                       int nslots, int *d_queue, hipStream_t stream) {
     hipLaunchKernelGGL(sphfd_sweep_kernel, dim3(nslots), dim3(FD_SWEEP_THREADS), 0, stream, d_jobs, njobs, d_slots,
                        d_queue);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done,
+                                      int cap, const fdsweep::Scratch *d_slots, int nslots, fdsweep::ServeDev *dev,
+                                      long long idle_ticks, hipStream_t stream) {
+    hipLaunchKernelGGL(sphfd_serve_kernel, dim3(nslots), dim3(FD_SWEEP_THREADS), 0, stream, ctl, ring, done, cap,
+                       d_slots, dev, idle_ticks);
     return hipGetLastError();
 }
 

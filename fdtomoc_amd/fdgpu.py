@@ -39,6 +39,13 @@ class CPar(C.Structure):
                 ("invx0", C.c_int), ("invy0", C.c_int), ("invz0", C.c_int), ("swab", C.c_int)]
 
 
+class CServeConfig(C.Structure):
+    """fdgpu_serve_config."""
+    _fields_ = [("max_nx", C.c_int), ("max_ny", C.c_int), ("max_nz", C.c_int),
+                ("arena_bytes", C.c_ulonglong), ("staging_bytes", C.c_ulonglong),
+                ("queue_capacity", C.c_int), ("workgroups", C.c_int)]
+
+
 class CTable(C.Structure):
     _fields_ = [("par", CPar), ("model", C.c_int), ("ox", C.c_int), ("oy", C.c_int), ("oz", C.c_int),
                 ("preset", C.c_void_p), ("boxvals", C.c_void_p)]
@@ -55,7 +62,13 @@ class TableSpec:
 
 
 class FdgpuError(RuntimeError):
-    pass
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+EBUSY, ETIMEOUT = -10, -11
+SERVE_INFO = ["on", "outstanding", "arena_used", "arena_bytes", "launches", "workgroups"]
 
 
 _lib = None
@@ -85,6 +98,18 @@ def lib():
         L.fdgpu_table_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_table_device_ptr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.fdgpu_run_spec.argtypes = [C.c_char_p]
+        L.fdgpu_serve_config_defaults.argtypes = [C.POINTER(CServeConfig)]
+        L.fdgpu_serve_config_defaults.restype = None
+        L.fdgpu_serve_start.argtypes = [C.c_void_p, C.POINTER(CServeConfig)]
+        L.fdgpu_serve_submit.argtypes = [C.c_void_p, C.POINTER(CTable), C.c_void_p, C.POINTER(C.c_int)]
+        L.fdgpu_serve_poll.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.fdgpu_serve_wait.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.fdgpu_serve_read.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_ulonglong]
+        L.fdgpu_serve_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.fdgpu_serve_write.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_void_p]
+        L.fdgpu_serve_release.argtypes = [C.c_void_p, C.c_int]
+        L.fdgpu_serve_info.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.fdgpu_serve_stop.argtypes = [C.c_void_p]
         L.sphfd.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_char_p]
         _lib = L
     return _lib
@@ -92,7 +117,7 @@ def lib():
 
 def _check(rc: int, what: str):
     if rc != 0:
-        raise FdgpuError(f"{what}: {lib().fdgpu_strerror(rc).decode()} ({rc})")
+        raise FdgpuError(f"{what}: {lib().fdgpu_strerror(rc).decode()} ({rc})", rc)
 
 
 def make_par(**kw) -> CPar:
@@ -145,20 +170,24 @@ class Context:
         _check(lib().fdgpu_model_release(self._h, model), "fdgpu_model_release")
 
     # tables
+    @staticmethod
+    def _fill(dst: CTable, spec: TableSpec, keep: list):
+        dst.par = make_par(**spec.par)
+        dst.model = spec.model
+        dst.ox, dst.oy, dst.oz = spec.origin
+        for name in ("preset", "boxvals"):
+            a = getattr(spec, name)
+            if a is not None:
+                a = np.ascontiguousarray(a, dtype=np.float32)
+                keep.append(a)
+                setattr(dst, name, a.ctypes.data)
+
     def solve(self, tables: list[TableSpec], mode: int = BATCHED) -> list[int]:
         n = len(tables)
         arr = (CTable * max(n, 1))()
         keep = []
         for t, spec in enumerate(tables):
-            arr[t].par = make_par(**spec.par)
-            arr[t].model = spec.model
-            arr[t].ox, arr[t].oy, arr[t].oz = spec.origin
-            for name in ("preset", "boxvals"):
-                a = getattr(spec, name)
-                if a is not None:
-                    a = np.ascontiguousarray(a, dtype=np.float32)
-                    keep.append(a)
-                    setattr(arr[t], name, a.ctypes.data)
+            self._fill(arr[t], spec, keep)
         ids = (C.c_int * max(n, 1))()
         _check(lib().fdgpu_solve(self._h, arr, n, mode, ids), "fdgpu_solve")
         return list(ids[:n])
@@ -182,6 +211,66 @@ class Context:
 
     def release(self, table: int):
         _check(lib().fdgpu_table_release(self._h, table), "fdgpu_table_release")
+
+    # serving (continuous batching)
+    def serve_start(self, max_dims: tuple, arena_bytes: int = 0, staging_bytes: int = 0,
+                    queue_capacity: int = 0, workgroups: int = 0):
+        cfg = CServeConfig()
+        lib().fdgpu_serve_config_defaults(C.byref(cfg))
+        cfg.max_nx, cfg.max_ny, cfg.max_nz = max_dims
+        if arena_bytes:
+            cfg.arena_bytes = arena_bytes
+        if staging_bytes:
+            cfg.staging_bytes = staging_bytes
+        if queue_capacity:
+            cfg.queue_capacity = queue_capacity
+        cfg.workgroups = workgroups
+        _check(lib().fdgpu_serve_start(self._h, C.byref(cfg)), "fdgpu_serve_start")
+
+    def serve_submit(self, spec: TableSpec, velocity: np.ndarray | None = None) -> int:
+        """Admits one request; raises FdgpuError with code EBUSY when memory or
+        the queue is full."""
+        d = CTable()
+        keep = []
+        self._fill(d, spec, keep)
+        v = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float32)
+        rid = C.c_int()
+        _check(lib().fdgpu_serve_submit(self._h, C.byref(d), None if v is None else v.ctypes.data, C.byref(rid)),
+               "fdgpu_serve_submit")
+        return rid.value
+
+    def serve_poll(self, req: int) -> bool:
+        done = C.c_int()
+        _check(lib().fdgpu_serve_poll(self._h, req, C.byref(done)), "fdgpu_serve_poll")
+        return bool(done.value)
+
+    def serve_wait(self, req: int, timeout_s: float = -1.0):
+        _check(lib().fdgpu_serve_wait(self._h, req, timeout_s), "fdgpu_serve_wait")
+
+    def serve_read(self, req: int, shape: tuple) -> np.ndarray:
+        out = np.empty(shape, dtype=np.float32)
+        _check(lib().fdgpu_serve_read(self._h, req, out.ctypes.data, out.size), "fdgpu_serve_read")
+        return out
+
+    def serve_stats(self, req: int) -> dict:
+        s = np.zeros(len(STAT_NAMES), dtype=np.int32)
+        _check(lib().fdgpu_serve_stats(self._h, req, s.ctypes.data, len(STAT_NAMES)), "fdgpu_serve_stats")
+        return dict(zip(STAT_NAMES, s.tolist()))
+
+    def serve_write(self, req: int, path: str, vel_header: bytes | None):
+        hb = None if vel_header is None else C.create_string_buffer(bytes(vel_header), 232)
+        _check(lib().fdgpu_serve_write(self._h, req, path.encode(), hb), "fdgpu_serve_write")
+
+    def serve_release(self, req: int):
+        _check(lib().fdgpu_serve_release(self._h, req), "fdgpu_serve_release")
+
+    def serve_info(self) -> dict:
+        v = np.zeros(len(SERVE_INFO), dtype=np.uint64)
+        _check(lib().fdgpu_serve_info(self._h, v.ctypes.data, len(SERVE_INFO)), "fdgpu_serve_info")
+        return dict(zip(SERVE_INFO, [int(x) for x in v]))
+
+    def serve_stop(self):
+        _check(lib().fdgpu_serve_stop(self._h), "fdgpu_serve_stop")
 
     def device_ptr(self, table: int) -> int:
         p = C.c_void_p()

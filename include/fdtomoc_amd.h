@@ -34,7 +34,10 @@ enum {
     FDGPU_EPAR = -7,        /* required par key missing (src/sphfd.c:6526, exit 100) */
     FDGPU_ESTOP = -8,       /* STOP in par input (src/sphfd.c:6462-6465, exit 101) */
     FDGPU_EKERNEL = -9,     /* the sweep kernel reported an internal error */
-    FDGPU_EBUSY = -10       /* serving queue full */
+    FDGPU_EBUSY = -10,      /* serving: arena/staging/queue full (retry after releasing
+                               finished requests), or a call that allocates or frees
+                               device memory made while the request loop runs */
+    FDGPU_ETIMEOUT = -11    /* serving: request not finished within the wait's timeout */
 };
 
 /* Par-file values of one table (keys of src/sphfd.c:383-530), before the
@@ -107,6 +110,53 @@ int fdgpu_table_stats(fdgpu_ctx *ctx, int table, int *stats, int nstats);
 int fdgpu_table_release(fdgpu_ctx *ctx, int table);
 /* Device pointer of a resident table (for zero-copy consumers on the GPU). */
 int fdgpu_table_device_ptr(fdgpu_ctx *ctx, int table, void **dptr);
+
+/* ------------------------------------------------------------------ serving
+ *
+ * A device-resident request loop with continuous batching (no reference
+ * counterpart: the reference solves one table per call).  While requests are
+ * outstanding a persistent kernel runs with one workgroup per CU; each
+ * workgroup claims the next request as soon as it finishes its previous one,
+ * so a request that arrives while others run starts on the first free
+ * workgroup and overlaps with them.  Results are bit-identical to
+ * fdgpu_solve's.  Admission is by memory: each request takes a block of a
+ * fixed HBM arena (its time grid and stencil tables, plus its slowness if it
+ * carries its own velocities) and of a pinned staging area for its inputs;
+ * when either is full submit returns FDGPU_EBUSY.
+ *
+ * While serving is on, calls that allocate or free device memory
+ * (fdgpu_solve, fdgpu_model_*, fdgpu_table_release) return FDGPU_EBUSY:
+ * create the resident models first. */
+typedef struct {
+    int max_nx, max_ny, max_nz;       /* largest table to admit (sizes per-workgroup scratch) */
+    unsigned long long arena_bytes;   /* HBM arena for admitted requests (0: all the budget leaves) */
+    unsigned long long staging_bytes; /* pinned staging for request inputs (default 1 GiB) */
+    int queue_capacity;               /* requests in flight (default 4096) */
+    int workgroups;                   /* server workgroups (0: one per CU) */
+} fdgpu_serve_config;
+
+void fdgpu_serve_config_defaults(fdgpu_serve_config *cfg);
+int fdgpu_serve_start(fdgpu_ctx *ctx, const fdgpu_serve_config *cfg);
+/* Admits one table.  velocity == NULL: the table is a window of resident model
+ * d->model at (ox, oy, oz).  Otherwise `velocity` holds the request's own
+ * par.nx*ny*nz velocities (e.g. the model after an inversion update; origin
+ * must be 0); it is copied before the call returns. */
+int fdgpu_serve_submit(fdgpu_ctx *ctx, const fdgpu_table_desc *d, const float *velocity, int *request);
+int fdgpu_serve_poll(fdgpu_ctx *ctx, int request, int *done);
+/* Waits until the request is finished (timeout_s < 0: no limit; FDGPU_ETIMEOUT). */
+int fdgpu_serve_wait(fdgpu_ctx *ctx, int request, double timeout_s);
+int fdgpu_serve_read(fdgpu_ctx *ctx, int request, float *host, unsigned long long count);
+int fdgpu_serve_stats(fdgpu_ctx *ctx, int request, int *stats, int nstats);
+/* Writes a finished request's table as fdgpu_table_write does. */
+int fdgpu_serve_write(fdgpu_ctx *ctx, int request, const char *path, const void *vel_header);
+/* Returns a finished request's arena block. */
+int fdgpu_serve_release(fdgpu_ctx *ctx, int request);
+/* info[0..5]: serving on, requests outstanding, arena bytes used, arena bytes,
+ * server launches so far, server workgroups. */
+int fdgpu_serve_info(fdgpu_ctx *ctx, unsigned long long *info, int ninfo);
+/* Finishes every admitted request, stops the loop and frees its memory
+ * (results of unreleased requests are dropped). */
+int fdgpu_serve_stop(fdgpu_ctx *ctx);
 
 /* Drop-in of src/sphfd.c:260: av[1] is one parlist line ("par=STA.par").
  * Solves on device 0 and writes output_path/timefile.  Returns 0 or a

@@ -1,26 +1,40 @@
-"""Benchmark of the batched MI355X sphfd path.
+"""Benchmark of the MI355X sphfd path.
 
-Workload (default, the configuration the headline metric is quoted on): 48
-stations uniform-random on the top surface of a 161 x 161 x 81 grid at 1 km
-centred at 23.5N 121E, P and S tables (Vs = Vp/1.73) -> 96 station solves per
-step, FP32 1D gradient 5.0 -> 8.0 km/s over 0-64 km (continued below) times a
-+-5% checkerboard of 20 km cells, reverse=50 as make_par_file writes.
+Default workload (the configuration the headline metric is quoted on, C2):
+48 stations uniform-random on the top surface of a 161 x 161 x 81 grid at
+1 km centred at 23.5N 121E, P and S tables (Vs = Vp/1.73) -> 96 station
+solves per step, FP32 1D gradient 5.0 -> 8.0 km/s over 0-64 km (continued
+below) times a +-5 % checkerboard of 20 km cells, reverse=50 as make_par_file
+writes.  A step is one batched solve of all 96 tables (one persistent launch);
+the velocity models are resident in HBM before timing starts.
 
-A step is one batched solve of all 96 tables (one persistent launch); the
-velocity models are resident in HBM before timing starts.  The line reports
-station-solves/s and grid nodes/s of the batched path, the plain path (one
-table per launch) on a sample, the HBM roofline of the sweep kernel and the
-reference sphfd (compiled from its own sources, oracle/_ref) timed on the host
-on a bounded sample of the same tables, whose outputs are also compared with
-the GPU tables bit for bit.
+The line reports station-solves/s and grid nodes/s of the batched path, the
+plain path (one table per launch) on a sample, the HBM roofline of the sweep
+kernel and the reference sphfd (compiled from its own sources, oracle/_ref)
+timed on the host, one process per table, on a bounded sample of the same
+tables, whose outputs are also compared with the GPU tables bit for bit.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N] [--plain-sample M]
+Other workloads (one JSON line each, same keys where they apply):
+  --workload c1   one 65x65x33 gradient table: plain GPU latency, the
+                  reference on one core, error against the analytic
+                  linear-gradient time
+  --workload c3   256 uneven windows (41^3 .. 201x201x101) of one resident
+                  model in one launch, plain sample, reference sample
+  --workload c4   serving: a Poisson stream of requests (81^3 .. 257x257x129,
+                  each with its own updated velocities), 200 GB HBM arena,
+                  p50/p99 latency and stations/s
+  --workload c5   one GPU's share of the 1000-station P+S set on
+                  321x321x161: --tables tables per launch, resident
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
     python bench.py --calibrate-copy MIB     # one known-size copy, for HBM counter scaling
 """
 from __future__ import annotations
 
 import argparse
+import concurrent.futures as cf
 import json
+import math
 import os
 import shutil
 import subprocess
@@ -33,7 +47,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-from fdtomoc_amd import fdgpu, synth, tables  # noqa: E402
+from fdtomoc_amd import autotune, fdgpu, synth, tables, workloads  # noqa: E402
 
 METRIC = "stations/s & nodes/s batched vs plain at <=1e-4 s err; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
@@ -45,57 +59,381 @@ TRAFFIC_BYTES_C2 = 367.03e9
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 
 
-def c2_workload(seed: int = 2):
-    g = synth.centred_grid(161, 161, 81, 1.0)
-    vp = synth.gradient_velocity(g, 5.0, 8.0, 64.0) * synth.checkerboard(g, 20.0, 0.05)
-    vs = (vp / np.float32(1.73)).astype(np.float32)
-    stations = synth.random_surface_stations(g, np.random.default_rng(seed), 48)
-    return g, vp.astype(np.float32), vs, stations
+# ------------------------------------------------------------------ helpers
+
+def host_cores(requested: int) -> int:
+    return requested or min(os.cpu_count() or 1, 16)
 
 
-def table_par(g, st, reverse=50):
-    return dict(nx=g.nx, ny=g.ny, nz=g.nz, h=g.h, x0=g.x0, y0=g.y0, z0=g.z0,
-                fxs=st.lon, fys=st.lat, fzs=st.depth, reverse=reverse)
+def cpu_reference(items, cores: int):
+    """Runs the compiled reference sphfd once per table, `cores` processes at
+    a time (one process per station table).  `items` = [(par dict, velocity
+    (nz, ny, nx) float32, header or None)].  Returns ([seconds per table],
+    [times grid per table])."""
+    base = tempfile.mkdtemp(prefix="sphfd_cpu_")
 
-
-def cpu_reference(g, vp, vs, stations, sample: int, threads: int):
-    """Runs the compiled reference sphfd on `sample` tables with OpenMP threads
-    (its own parallel-for over tables, src/sphfd.c:247).  Returns (seconds,
-    {table index: times})."""
-    wd = tempfile.mkdtemp(prefix="sphfd_cpu_")
-    try:
-        head = tables.make_header(g.nx, g.ny, g.nz, g.x0, g.y0, g.z0, g.df_deg(), g.dq_deg(), g.h)
-        tables.write_grid(os.path.join(wd, "VP.mod"), head, vp)
-        tables.write_grid(os.path.join(wd, "VS.mod"), head, vs)
+    def one(k):
+        par, vel, head = items[k]
+        wd = os.path.join(base, f"t{k}")
         os.makedirs(os.path.join(wd, "out"))
-        lines = []
-        picked = []
-        for t in range(sample):
-            st = stations[t // 2]
-            phase = t % 2
-            p = table_par(g, st)
-            name = f"{st.name}.{'ptimes' if phase == 0 else 'stimes'}"
-            txt = "".join(f"{k}={v!r}\n" for k, v in p.items())
-            txt += f"timefile={name}\nvelfile={'VP.mod' if phase == 0 else 'VS.mod'}\n"
-            pf = os.path.join(wd, f"t{t}.par")
-            open(pf, "w").write(txt)
-            lines.append(f"par=t{t}.par")
-            picked.append((2 * (t // 2) + phase, name))
-        open(os.path.join(wd, "parlist"), "w").write("\n".join(lines) + "\n\n")
+        nz, ny, nx = vel.shape
+        if head is None:
+            head = tables.make_header(nx, ny, nz, par["x0"], par["y0"], par["z0"], 0.0, 0.0, par["h"])
+        tables.write_grid(os.path.join(wd, "vel.mod"), head, vel)
+        txt = "".join(f"{k2}={v!r}\n" for k2, v in par.items()) + "timefile=sta.times\nvelfile=vel.mod\n"
+        open(os.path.join(wd, "sta.par"), "w").write(txt)
+        open(os.path.join(wd, "parlist"), "w").write("par=sta.par\n\n")
         open(os.path.join(wd, "spec"), "w").write("parlist parlist\ntimedir out\n")
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        env = dict(os.environ, OMP_NUM_THREADS="1")
         t0 = time.perf_counter()
         r = subprocess.run([REF_EXE], input="spec\n", cwd=wd, env=env, capture_output=True, text=True)
         dt = time.perf_counter() - t0
         if r.returncode != 0:
             raise RuntimeError("reference sphfd failed: " + r.stderr[-1000:])
-        outs = {}
-        for idx, name in picked:
-            outs[idx] = np.array(tables.read_grid(os.path.join(wd, "out", name)).data)
-        return dt, outs
-    finally:
-        shutil.rmtree(wd, ignore_errors=True)
+        return dt, np.array(tables.read_grid(os.path.join(wd, "out", "sta.times")).data)
 
+    try:
+        with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+            res = list(ex.map(one, range(len(items))))
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+    return [r[0] for r in res], [r[1] for r in res]
+
+
+def compare(gpu: list, ref: list):
+    maxerr, ndiff = 0.0, 0
+    for a, b in zip(gpu, ref):
+        maxerr = max(maxerr, float(np.max(np.abs(a.astype(np.float64) - b))))
+        ndiff += int((a.view(np.uint32) != b.view(np.uint32)).sum())
+    return maxerr, ndiff
+
+
+def cpu_block(secs, sizes, cores, what):
+    """cpu_baseline object: throughput of `cores` concurrent single-threaded
+    reference processes = tables / (sum of per-table seconds / cores)."""
+    busy = float(np.sum(secs)) / cores
+    return {"value": round(len(secs) / busy, 5), "unit": "station-solves/s", "cores": cores, "kind": "reference",
+            "nodes_per_s": round(float(np.sum(sizes)) / busy, 1),
+            "sample": what + f"; compiled reference sphfd, one single-threaded process per table, {cores} at a "
+                             f"time; wall time per table incl. file I/O, rate = tables / (sum of times / {cores})"}
+
+
+def base_line(args, value, unit, ms_per_step, config, dtype="f64"):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded models and stations, reference binary layout)", "config": config}
+
+
+def roofline(visits, kern_s, traffic=None, source=None):
+    achieved = BYTES_PER_VISIT * visits / kern_s / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+           "kernel": "sphfd_sweep_kernel", "kernel_ms": round(kern_s * 1e3, 2),
+           "algorithmic_bytes": BYTES_PER_VISIT * visits}
+    if source:
+        out["traffic_source"] = source
+    return out
+
+
+def tune(ctx, specs, args, sample=None):
+    """Per-face schedule picked by the autotuner before timing (untimed, like
+    the warmup); the tables are identical for every choice."""
+    if args.no_autotune:
+        return {"change_budget": ctx.get_schedule(), "tuned": False}
+    best, timings = autotune.tune_change_budget(ctx, specs, sample=sample)
+    print(f"# autotune change_budget: {best} {timings}", file=sys.stderr, flush=True)
+    return {"change_budget": best, "tuned": True, "candidates_s": {str(k): round(v, 3) for k, v in timings.items()}}
+
+
+def timed_steps(ctx, specs, args, mode=fdgpu.BATCHED):
+    for _ in range(args.warmup):
+        for i in ctx.solve(specs, mode):
+            ctx.release(i)
+    wall, kern, last = [], [], None
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        ids = ctx.solve(specs, mode)
+        dt = time.perf_counter() - t0
+        wall.append(dt)
+        kern.append(ctx.last_kernel_ms() / 1e3)
+        print(f"# step {s}: {dt * 1e3:.1f} ms wall, {kern[-1] * 1e3:.1f} ms kernel", file=sys.stderr, flush=True)
+        if s < args.steps - 1:
+            for i in ids:
+                ctx.release(i)
+        else:
+            last = ids
+    return np.array(wall), np.array(kern), last
+
+
+def plain_rate(ctx, specs, sample):
+    if not sample:
+        return None
+    t0 = time.perf_counter()
+    ids = ctx.solve([specs[i] for i in sample], fdgpu.PLAIN)
+    dt = time.perf_counter() - t0
+    for i in ids:
+        ctx.release(i)
+    return len(sample) / dt
+
+
+def host_geometry(par: dict) -> dict:
+    """Geometry of a table in radians and km, as the reference derives it
+    (src/sphfd.c:415-509): geocentric colatitude and top radius from glath,
+    dq/df from h when unset, the source's lon/colatitude/radius."""
+    D = synth.DEGRAD
+    lat0, z0r = synth.glath(par["y0"] * D, par.get("z0", 0.0))
+    y0 = synth.HPI - lat0
+    h = par["h"]
+    dq = par.get("dq", 0.0) * D or h / synth.REARTH
+    df = par.get("df", 0.0) * D or abs(h / (synth.REARTH * math.sin(y0)))
+    lats, rs = synth.glath(par["fys"] * D, par["fzs"])
+    return {"h": h, "dq": dq, "df": df, "x0": par["x0"] * D, "y0": y0, "z0r": z0r,
+            "fxss": par["fxs"] * D, "fyss": synth.HPI - lats, "fzss": rs}
+
+
+# ------------------------------------------------------------------ C2 (default)
+
+def run_c2(args):
+    g, vp, vs, stations = workloads.c2()
+    ctx = fdgpu.Context(0)
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs, items = [], []
+    for st in stations:
+        for m, v in ((mp, vp), (ms, vs)):
+            p = workloads.table_par(g, st)
+            specs.append(fdgpu.TableSpec(p, m))
+            items.append((p, v, None))
+    ntab = len(specs)
+    sched = tune(ctx, specs, args)
+    wall, kern, last = timed_steps(ctx, specs, args)
+    restarts = np.array([ctx.stats(i)["restarts"] for i in last])
+    visits = int(np.sum(g.n * (1 + restarts)))
+    stations_s = ntab / float(wall.mean())
+    plain_s = plain_rate(ctx, specs, list(range(min(args.plain_sample, ntab))))
+    out = base_line(args, round(stations_s, 3), "station-solves/s", round(float(wall.mean()) * 1e3, 2),
+                    {"workload": "C2: 48 stations P+S = 96 tables, 161x161x81 at 1 km, gradient 5-8 km/s x "
+                                 "+-5% 20 km checkerboard, reverse=50, one batched launch per step",
+                     "tables": ntab, "nodes_per_table": g.n, "reverse_passes_mean": float(restarts.mean())})
+    out["nodes_per_s"] = round(ntab * g.n / float(wall.mean()), 1)
+    out["schedule"] = sched
+    out["plain_station_solves_per_s"] = None if plain_s is None else round(plain_s, 4)
+    out["batched_over_plain"] = None if plain_s is None else round(stations_s / plain_s, 2)
+    out["latency_ms"] = {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
+                         "p99": round(float(np.percentile(wall, 99)) * 1e3, 2),
+                         "what": "batched solve of all 96 tables (submit to results resident)"}
+    out["roofline"] = roofline(visits, float(kern.mean()), TRAFFIC_BYTES_C2,
+                               "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md")
+    out["cpu_baseline"] = None
+    if not args.no_cpu and os.path.exists(REF_EXE):
+        cores = host_cores(args.cpu_threads)
+        n = max(1, min(cores if args.cpu_sample < 0 else args.cpu_sample, ntab))
+        secs, refs = cpu_reference(items[:n], cores)
+        out["cpu_baseline"] = cpu_block(secs, [g.n] * n, cores, f"the first {n} of the 96 tables")
+        out["gpu_over_cpu"] = round(stations_s / out["cpu_baseline"]["value"], 2)
+        err, nd = compare([ctx.read(last[i], refs[i].shape) for i in range(n)], refs)
+        out["max_abs_err_vs_reference_s"] = err
+        out["nodes_differing_vs_reference"] = nd
+    for i in last:
+        ctx.release(i)
+    ctx.close()
+    return out
+
+
+# ------------------------------------------------------------------ C1
+
+def run_c1(args):
+    g, v, st = workloads.c1()
+    par = workloads.table_par(g, st)
+    ctx = fdgpu.Context(0)
+    m = ctx.model(v)
+    specs = [fdgpu.TableSpec(par, m)]
+    wall, kern, last = timed_steps(ctx, specs, args, fdgpu.PLAIN)
+    t = ctx.read(last[0], v.shape)
+    exact = workloads.gradient_time_flat(g, host_geometry(par), 5.0, 3.0 / 64.0)
+    err = np.abs(t.astype(np.float64) - exact)
+    out = base_line(args, round(1.0 / float(wall.mean()), 3), "station-solves/s",
+                    round(float(wall.mean()) * 1e3, 2),
+                    {"workload": "C1: one station, 65x65x33 at 1 km, Vp 5.0 + 3.0 z/64 km/s, plain launch",
+                     "tables": 1, "nodes_per_table": g.n})
+    out["latency_ms"] = {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
+                         "p99": round(float(np.percentile(wall, 99)) * 1e3, 2), "what": "one plain solve"}
+    st_ = ctx.stats(last[0])
+    out["roofline"] = roofline(g.n * (1 + st_["restarts"]), float(kern.mean()))
+    out["analytic_gradient_error_s"] = {"max": float(err.max()), "mean": float(err.mean()),
+                                        "what": "vs straight-chord flat-layer linear-gradient time (FP64)"}
+    out["cpu_baseline"] = None
+    if not args.no_cpu and os.path.exists(REF_EXE):
+        secs, refs = cpu_reference([(par, v, None)], 1)
+        out["cpu_baseline"] = cpu_block(secs, [g.n], 1, "the C1 table")
+        e, nd = compare([t], refs)
+        out["max_abs_err_vs_reference_s"] = e
+        out["nodes_differing_vs_reference"] = nd
+    ctx.release(last[0])
+    ctx.close()
+    return out
+
+
+# ------------------------------------------------------------------ C3
+
+def run_c3(args):
+    G, v, wins = workloads.c3()
+    ctx = fdgpu.Context(0)
+    m = ctx.model(v)
+    specs = [fdgpu.TableSpec(w.par, m, w.origin) for w in wins]
+    sizes = np.array([w.par["nx"] * w.par["ny"] * w.par["nz"] for w in wins])
+    sched = tune(ctx, specs, args)
+    wall, kern, last = timed_steps(ctx, specs, args)
+    restarts = np.array([ctx.stats(i)["restarts"] for i in last])
+    visits = int(np.sum(sizes * (1 + restarts)))
+    stations_s = len(specs) / float(wall.mean())
+    rng = np.random.default_rng(33)
+    psel = sorted(rng.choice(len(specs), size=min(args.plain_sample, len(specs)), replace=False).tolist())
+    plain_s = plain_rate(ctx, specs, psel)
+    # plain rate scaled to the workload's mean table size
+    plain_adj = None if plain_s is None else plain_s * float(np.mean(sizes[psel])) / float(sizes.mean())
+    out = base_line(args, round(stations_s, 3), "station-solves/s", round(float(wall.mean()) * 1e3, 2),
+                    {"workload": "C3: 256 station windows of a 401x401x101 model, radii U(40,200) km, extents "
+                                 "41^3..201x201x101, gradient x 3 Gaussian +-8% anomalies, one batched launch",
+                     "tables": len(specs), "nodes_total": int(sizes.sum()),
+                     "reverse_passes_mean": float(restarts.mean())})
+    out["nodes_per_s"] = round(float(sizes.sum()) / float(wall.mean()), 1)
+    out["schedule"] = sched
+    out["plain_station_solves_per_s"] = None if plain_adj is None else round(plain_adj, 4)
+    out["batched_over_plain"] = None if plain_adj is None else round(stations_s / plain_adj, 2)
+    out["plain_sample"] = f"{len(psel)} random tables, one launch each, rate scaled by mean size"
+    out["latency_ms"] = {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
+                         "p99": round(float(np.percentile(wall, 99)) * 1e3, 2), "what": "batched solve of 256 windows"}
+    out["roofline"] = roofline(visits, float(kern.mean()))
+    out["cpu_baseline"] = None
+    if not args.no_cpu and os.path.exists(REF_EXE):
+        cores = host_cores(args.cpu_threads)
+        n = max(1, cores if args.cpu_sample < 0 else args.cpu_sample)
+        # a size-stratified sample from the smaller three quarters keeps the CPU leg bounded
+        order = np.argsort(sizes)[: (3 * len(sizes)) // 4]
+        pick = [int(order[int(q)]) for q in np.linspace(0, len(order) - 1, n)]
+        items = []
+        for k in pick:
+            w = wins[k]
+            ox, oy, oz = w.origin
+            p = w.par
+            items.append((p, np.ascontiguousarray(v[oz:oz + p["nz"], oy:oy + p["ny"], ox:ox + p["nx"]]), None))
+        secs, refs = cpu_reference(items, cores)
+        cb = cpu_block(secs, sizes[pick], cores, f"{n} windows stratified by size from the smaller 3/4")
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(out["nodes_per_s"] / cb["nodes_per_s"], 2)
+        out["gpu_over_cpu_basis"] = "nodes/s (the tables differ in size)"
+        err, nd = compare([ctx.read(last[k], refs[q].shape) for q, k in enumerate(pick)], refs)
+        out["max_abs_err_vs_reference_s"] = err
+        out["nodes_differing_vs_reference"] = nd
+    for i in last:
+        ctx.release(i)
+    ctx.close()
+    return out
+
+
+# ------------------------------------------------------------------ C4
+
+def run_c4(args):
+    reqs = workloads.c4_requests(args.requests, args.rate)
+    vels = [workloads.c4_velocity(r) for r in reqs]
+    ctx = fdgpu.Context(0)
+    ctx.serve_start((257, 257, 129), arena_bytes=int(200e9), staging_bytes=8 << 30)
+    keep = {0, 1, 2}            # requests re-solved alone afterwards
+    lat, done_at, sizes, restarts, kept = [], [], [], [], {}
+    outstanding = {}
+    busy_retries = 0
+    i = 0
+    t0 = time.perf_counter()
+    last_print = 0.0
+    while i < len(reqs) or outstanding:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i].t_arrival <= now:
+            try:
+                rid = ctx.serve_submit(fdgpu.TableSpec(reqs[i].par, -1), vels[i])
+            except fdgpu.FdgpuError as e:
+                if e.code != fdgpu.EBUSY:
+                    raise
+                busy_retries += 1
+                break
+            outstanding[rid] = i
+            i += 1
+        for rid in list(outstanding):
+            if ctx.serve_poll(rid):
+                k = outstanding.pop(rid)
+                t = time.perf_counter() - t0
+                lat.append(t - reqs[k].t_arrival)
+                done_at.append(t)
+                st = ctx.serve_stats(rid)
+                sizes.append(int(np.prod(reqs[k].shape)))
+                restarts.append(st["restarts"])
+                if k in keep:
+                    kept[k] = ctx.serve_read(rid, reqs[k].shape)
+                ctx.serve_release(rid)
+        if now - last_print > 10.0:
+            print(f"# t={now:.0f}s submitted {i} done {len(done_at)}", file=sys.stderr, flush=True)
+            last_print = now
+        time.sleep(0.0005)
+    span = max(done_at) - reqs[0].t_arrival
+    info = ctx.serve_info()
+    ctx.serve_stop()
+    # the same requests alone: one batched launch each, compared bit for bit
+    nd = 0
+    for k, t in kept.items():
+        mm = ctx.model(vels[k])
+        (tid,) = ctx.solve([fdgpu.TableSpec(reqs[k].par, mm)])
+        nd += int((ctx.read(tid, t.shape).view(np.uint32) != t.view(np.uint32)).sum())
+        ctx.release(tid)
+        ctx.release_model(mm)
+    ctx.close()
+    lat = np.array(lat)
+    out = base_line(args, round(len(reqs) / span, 3), "station-solves/s", None,
+                    {"workload": f"C4: Poisson stream of {len(reqs)} requests at {args.rate}/s, sizes 81^3.."
+                                 f"257x257x129, each with its own +-3% Gaussian-blob velocity update, 200 GB HBM "
+                                 f"arena, continuous batching", "requests": len(reqs), "rate_per_s": args.rate})
+    out["steps"] = None
+    out["warmup"] = None
+    out["nodes_per_s"] = round(float(np.sum(sizes)) / span, 1)
+    out["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                         "p99": round(float(np.percentile(lat, 99)) * 1e3, 1),
+                         "what": "arrival to result resident (queueing + admission + solve)"}
+    out["serve"] = {"server_launches": info["launches"], "busy_retries": busy_retries,
+                    "reverse_passes_mean": float(np.mean(restarts)),
+                    "mix_vs_alone_nodes_differing": nd, "mix_vs_alone_checked": len(kept)}
+    out["cpu_baseline"] = None
+    return out
+
+
+# ------------------------------------------------------------------ C5
+
+def run_c5(args):
+    g, vp, vs, sts = workloads.c5()
+    ctx = fdgpu.Context(0)
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs = []
+    for st in sts:
+        specs.append(fdgpu.TableSpec(workloads.table_par(g, st), mp))
+        specs.append(fdgpu.TableSpec(workloads.table_par(g, st), ms))
+    specs = specs[: args.tables]
+    wall, kern, last = timed_steps(ctx, specs, args)
+    restarts = np.array([ctx.stats(i)["restarts"] for i in last])
+    stations_s = len(specs) / float(wall.mean())
+    out = base_line(args, round(stations_s, 3), "station-solves/s", round(float(wall.mean()) * 1e3, 2),
+                    {"workload": f"C5 share: {len(specs)} of the 2000 P+S tables on 321x321x161, gradient x "
+                                 f"random +-4% layers, one batched launch, tables resident",
+                     "tables": len(specs), "nodes_per_table": g.n, "reverse_passes_mean": float(restarts.mean())})
+    out["nodes_per_s"] = round(len(specs) * g.n / float(wall.mean()), 1)
+    out["roofline"] = roofline(int(np.sum(g.n * (1 + restarts))), float(kern.mean()))
+    out["c5_iteration_s_extrapolated"] = round(2000 / stations_s, 1)
+    out["resident_table_bytes"] = 4 * g.n * len(specs)
+    out["cpu_baseline"] = None
+    for i in last:
+        ctx.release(i)
+    ctx.close()
+    return out
+
+
+# ------------------------------------------------------------------ calibration
 
 def calibrate_copy(mib: int) -> None:
     """One device-to-device copy of known size, for scaling the FETCH_SIZE /
@@ -116,10 +454,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default per-face schedule")
+    ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")
+    ap.add_argument("--rate", type=float, default=3.0, help="c4: mean arrivals per second")
+    ap.add_argument("--tables", type=int, default=256, help="c5: tables per launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -129,104 +472,8 @@ def main():
         print("bench.py: this build measures one GPU; the station tables shard across GPUs with no "
               "communication on the hot path, so multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
-
-    g, vp, vs, stations = c2_workload()
-    ctx = fdgpu.Context(0)
-    mp, ms = ctx.model(vp), ctx.model(vs)
-    specs = []
-    for st in stations:
-        specs.append(fdgpu.TableSpec(table_par(g, st), mp))
-        specs.append(fdgpu.TableSpec(table_par(g, st), ms))
-    ntab = len(specs)
-
-    def step(mode=fdgpu.BATCHED, which=None):
-        sel = specs if which is None else [specs[i] for i in which]
-        t0 = time.perf_counter()
-        ids = ctx.solve(sel, mode)
-        dt = time.perf_counter() - t0
-        return ids, dt, ctx.last_kernel_ms()
-
-    for _ in range(args.warmup):
-        ids, _, _ = step()
-        for i in ids:
-            ctx.release(i)
-
-    wall, kern = [], []
-    last_ids = None
-    for s in range(args.steps):
-        ids, dt, kms = step()
-        wall.append(dt)
-        kern.append(kms)
-        print(f"# step {s}: {dt * 1e3:.1f} ms wall, {kms:.1f} ms kernel", file=sys.stderr, flush=True)
-        if s < args.steps - 1:
-            for i in ids:
-                ctx.release(i)
-        else:
-            last_ids = ids
-    stats = [ctx.stats(i) for i in last_ids]
-    restarts = np.array([s["restarts"] for s in stats])
-    n = g.n
-    visits = int(np.sum(n * (1 + restarts)))
-    wall = np.array(wall)
-    kern_s = np.array(kern) / 1e3
-    ms_per_step = float(np.mean(wall) * 1e3)
-    stations_s = ntab / float(np.mean(wall))
-    nodes_s = ntab * n / float(np.mean(wall))
-    achieved = BYTES_PER_VISIT * visits / float(np.mean(kern_s)) / 1e9
-
-    # plain path: one table per launch, on a sample
-    plain_s = None
-    psel = list(range(min(args.plain_sample, ntab)))
-    if psel:
-        pids, pdt, _ = step(fdgpu.PLAIN, psel)
-        for i in pids:
-            ctx.release(i)
-        plain_s = len(psel) / pdt
-
-    out = {
-        "metric": METRIC, "value": round(stations_s, 3), "unit": "station-solves/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded gradient x checkerboard model, random surface stations)",
-        "config": {"workload": "48 stations P+S = 96 tables, 161x161x81 at 1 km, gradient 5-8 km/s x "
-                               "+-5% 20 km checkerboard, reverse=50, one batched launch per step",
-                   "tables": ntab, "nodes_per_table": n, "reverse_passes_mean": float(restarts.mean())},
-        "nodes_per_s": round(nodes_s, 1),
-        "plain_station_solves_per_s": None if plain_s is None else round(plain_s, 4),
-        "batched_over_plain": None if plain_s is None else round(stations_s / plain_s, 2),
-        "latency_ms": {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
-                       "p99": round(float(np.percentile(wall, 99)) * 1e3, 2),
-                       "what": "batched solve of all 96 tables (submit to results resident)"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_C2,
-                     "traffic_source": "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md",
-                     "kernel": "sphfd_sweep_kernel", "kernel_ms": round(float(np.mean(kern_s)) * 1e3, 2),
-                     "algorithmic_bytes": BYTES_PER_VISIT * visits},
-    }
-
-    if not args.no_cpu and os.path.exists(REF_EXE):
-        cores = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        sample = cores if args.cpu_sample < 0 else args.cpu_sample
-        sample = max(1, min(sample, ntab))
-        dt, outs = cpu_reference(g, vp, vs, stations, sample, cores)
-        maxerr, ndiff = 0.0, 0
-        for idx, tref in outs.items():
-            tg = ctx.read(last_ids[idx], tref.shape)
-            maxerr = max(maxerr, float(np.max(np.abs(tg.astype(np.float64) - tref))))
-            ndiff += int((tg.view(np.uint32) != tref.view(np.uint32)).sum())
-        out["cpu_baseline"] = {"value": round(sample / dt, 5), "unit": "station-solves/s", "cores": cores,
-                               "kind": "reference",
-                               "sample": f"{sample} of the 96 tables, compiled reference sphfd, OpenMP "
-                                         f"parallel-for over tables with {cores} threads, wall time incl. file I/O"}
-        out["gpu_over_cpu"] = round(stations_s / (sample / dt), 2)
-        out["max_abs_err_vs_reference_s"] = maxerr
-        out["nodes_differing_vs_reference"] = ndiff
-    else:
-        out["cpu_baseline"] = None
-    for i in last_ids:
-        ctx.release(i)
-    ctx.close()
-    print(json.dumps(out), flush=True)
+    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload]
+    print(json.dumps(run(args)), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,54 @@
+"""Autotuner of the sweep's per-face schedule.
+
+Reverse-pass faces first try the CHANGE schedule (evaluate the whole face at
+once, then re-evaluate what changed); when changes cascade through a face, the
+LEVEL schedule (each node once, in dependency order) is cheaper, and the face
+is handed over after `change_budget` evaluations per node.  Which budget is
+fastest depends on the workload: many reverse passes with small changes favour
+CHANGE (C2-like batches), a few passes with large changes favour LEVEL
+(C3-like windows).  The results are bit-identical for every budget, so the
+tuner only times: one batched solve per candidate on the caller's tables (or a
+sample of them), keeping the fastest.
+
+    from fdtomoc_amd import autotune
+    best, timings = autotune.tune_change_budget(ctx, specs)
+"""
+from __future__ import annotations
+
+import time
+
+from . import fdgpu
+
+CANDIDATES = (0, 2, 4, 8, 16, 32)
+
+
+def tune_change_budget(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES, sample: int | None = None,
+                       objective: str = "wall") -> tuple[int, dict]:
+    """Times a batched solve of `specs` (the first `sample` of them if given)
+    for each candidate budget and sets the fastest on `ctx`.
+
+    objective "wall": the launch's wall time (what a batch of these tables
+    costs); "sum": the sum of the tables' own device times (throughput when
+    far more tables than workgroups are queued).  Returns (best budget,
+    {budget: seconds})."""
+    sel = specs if sample is None else specs[:sample]
+    timings = {}
+    for b in candidates:
+        ctx.set_schedule(b)
+        t0 = time.perf_counter()
+        ids = ctx.solve(sel, fdgpu.BATCHED)
+        wall = time.perf_counter() - t0
+        if objective == "sum":
+            tick = 1e-8   # device wall-clock ticks of the per-table timers (100 MHz)
+            cost = 0.0
+            for i in ids:
+                s = ctx.stats(i)
+                cost += (s["t_gather"] + s["t_sort"] + s["t_level"] + s["t_change"] + s["t_init"]) * tick
+        else:
+            cost = wall
+        for i in ids:
+            ctx.release(i)
+        timings[b] = cost
+    best = min(timings, key=timings.get)
+    ctx.set_schedule(best)
+    return best, timings

@@ -6,7 +6,9 @@
 #include "serve_shared.h"
 #include "sweep_solver.h"
 
-#define FD_SWEEP_THREADS 256
+#ifndef FD_SWEEP_THREADS
+#define FD_SWEEP_THREADS 512   // lanes per workgroup of the sweep and server kernels (2 waves per SIMD)
+#endif
 
 extern "C" hipError_t fd_launch_sweep(const fdsweep::TableJob *d_jobs, int njobs, const fdsweep::Scratch *d_slots,
                                       int nslots, int *d_queue, hipStream_t stream);

@@ -358,6 +358,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         J.last_full = d_lf;
         J.maxdim = maxdim;
         J.skip_ok = fdprep::skip_safe(to_values(T.par), T.prep, m.vmin, m.vmax) ? 1 : 0;
+        J.change_budget = c->change_budget;
     }
     auto unwind = [&]() {
         for (int t = 0; t < made; t++) release_table(c, tabs[(size_t)t]);
@@ -411,6 +412,20 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
 }
 
 double fdgpu_last_kernel_ms(fdgpu_ctx *c) { return c ? c->last_ms : 0.0; }
+
+int fdgpu_set_schedule(fdgpu_ctx *c, int change_budget) {
+    if (!c || change_budget < 0) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->change_budget = change_budget;
+    return FDGPU_OK;
+}
+
+int fdgpu_get_schedule(fdgpu_ctx *c, int *change_budget) {
+    if (!c || !change_budget) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *change_budget = c->change_budget;
+    return FDGPU_OK;
+}
 
 int fdgpu_table_read(fdgpu_ctx *c, int table, float *host, unsigned long long count) {
     if (!c || !host) return FDGPU_EINPUT;

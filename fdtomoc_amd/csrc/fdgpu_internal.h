@@ -149,6 +149,7 @@ struct fdgpu_ctx {
     long long slot_face_cap = 0;
     int *d_queue = nullptr;
     double last_ms = 0.0;
+    int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     fdgpu_detail::Serve serve;
     std::mutex mu;
 };

@@ -336,6 +336,7 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
     J.last_full = (int *)(blk + DL.lastf);
     J.maxdim = std::max(p.nx, std::max(p.ny, p.nz));
     J.skip_ok = fdprep::skip_safe(to_values(p), T.prep, vmin, vmax) ? 1 : 0;
+    J.change_budget = c->change_budget;
     job.vel_src = own ? (const float *)(st + SL.vel) : nullptr;
     job.slow_dst = own ? (float *)(blk + DL.slow) : nullptr;
     job.nvel = own ? T.n : 0;

@@ -67,6 +67,8 @@ struct TableJob {
                                // full-plane face per (face type, plane)
     int maxdim;                // max(nx, ny, nz)
     int skip_ok;               // skipping unchanged reverse-pass nodes is exact here
+    int change_budget;         // CHANGE evaluations per face node before a face goes to
+                               // LEVEL (0: never); a schedule choice, results are identical
 };
 
 enum {
@@ -96,7 +98,6 @@ struct Scratch {
                          // LEVEL, faces too large for LDS: successor masks
 };
 
-constexpr int kMaxThreads = 256;
 // LDS scratch of a workgroup: radix histograms during a sort, then the LEVEL
 // schedule's byte-packed pending counters and successor masks (faces up to
 // about 70K nodes; larger faces use global scratch).
@@ -431,9 +432,20 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
     if (tid == 0) { sh.n[0] = 0; sh.n[2] = 0; sh.flag &= ~4; sh.best = 0; }
     for (int p = tid; p < Fn; p += T) W.stamp[p] = 0;
     x.sync();
+    long long spent = 0;
     for (int round = 1;; round++) {
         const int nwork = (round == 1) ? Fn : sh.n[0];
         if (nwork == 0) break;
+        // When changes keep cascading through the face (typically the first
+        // reverse pass), every round re-evaluates most of it; LEVEL evaluates
+        // each node once, so past change_budget evaluations per node the face
+        // is restored and handed to LEVEL.
+        if (round > 1 && J.change_budget > 0 && spent + nwork > (long long)J.change_budget * Fn) {
+            if (tid == 0) sh.flag |= 4;
+            x.sync();
+            break;
+        }
+        spent += nwork;
         // the evaluation DAG has depth <= F, so F + 1 rounds always suffice
         if (round > 2 * Fn + 8) {
             if (tid == 0) { sh.st[kStatError] |= 16; sh.flag |= 4; }

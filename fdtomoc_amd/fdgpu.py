@@ -98,6 +98,8 @@ def lib():
         L.fdgpu_table_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_table_device_ptr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.fdgpu_run_spec.argtypes = [C.c_char_p]
+        L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
+        L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_serve_config_defaults.argtypes = [C.POINTER(CServeConfig)]
         L.fdgpu_serve_config_defaults.restype = None
         L.fdgpu_serve_start.argtypes = [C.c_void_p, C.POINTER(CServeConfig)]
@@ -191,6 +193,15 @@ class Context:
         ids = (C.c_int * max(n, 1))()
         _check(lib().fdgpu_solve(self._h, arr, n, mode, ids), "fdgpu_solve")
         return list(ids[:n])
+
+    def set_schedule(self, change_budget: int):
+        """Per-face schedule knob (fdgpu_set_schedule); results do not depend on it."""
+        _check(lib().fdgpu_set_schedule(self._h, int(change_budget)), "fdgpu_set_schedule")
+
+    def get_schedule(self) -> int:
+        b = C.c_int()
+        _check(lib().fdgpu_get_schedule(self._h, C.byref(b)), "fdgpu_get_schedule")
+        return b.value
 
     def last_kernel_ms(self) -> float:
         return float(lib().fdgpu_last_kernel_ms(self._h))

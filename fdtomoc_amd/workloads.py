@@ -1,0 +1,178 @@
+"""Synthetic workloads of the sphfd path (seeded; written in the reference's
+layout when files are needed).
+
+  C1  one station, 65 x 65 x 33 at 1 km, linear gradient Vp 5.0 -> 8.0 km/s
+      over 0-64 km, source at a seeded surface node
+  C2  48 stations x (P, S) = 96 tables of 161 x 161 x 81 at 1 km, gradient x
+      +-5 % checkerboard of 20 km cells, Vs = Vp / 1.73
+  C3  256 station solves whose boxes are clipped to per-station radii
+      r ~ U(40, 200) km: windows (r+1) x (r+1) x (min(r,100)+1) of one
+      401 x 401 x 101 model (gradient x three +-8 % Gaussian anomalies of
+      sigma 10-30 km), each window its own table
+  C4  a Poisson stream of requests, sizes 81^3 to 257 x 257 x 129, each with
+      its own velocity model (the gradient times +-3 % Gaussian blobs, as an
+      inversion update would make)
+  C5  1000 stations x (P, S) on 321 x 321 x 161, gradient x random layers
+      (+-4 %), tables resident across inversion iterations
+
+Grids are centred at 23.5N 121E.  Station par values follow make_par_file's
+format (fxs/fys at 1e-5 deg, fzs at 1e-3 km; reverse=50).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import synth
+
+
+def table_par(g: synth.GridSpec, st: synth.Station, reverse: int = 50) -> dict:
+    return dict(nx=g.nx, ny=g.ny, nz=g.nz, h=g.h, x0=g.x0, y0=g.y0, z0=g.z0,
+                fxs=st.lon, fys=st.lat, fzs=st.depth, reverse=reverse)
+
+
+# ------------------------------------------------------------------ C1
+
+def c1(seed: int = 0):
+    """(grid, velocity, station) of C1: Vp(z) = 5 + 3 z / 64 km."""
+    g = synth.centred_grid(65, 65, 33, 1.0)
+    v = synth.gradient_velocity(g, 5.0, 8.0, 64.0)
+    st = synth.surface_node_station(g, np.random.default_rng(seed), "C1", margin=4)
+    return g, np.ascontiguousarray(v, dtype=np.float32), st
+
+
+def gradient_time_flat(g: synth.GridSpec, geom: dict, v0: float, grad: float) -> np.ndarray:
+    """FP64 travel times from the source in a medium whose velocity grows
+    linearly with depth below the grid top, v = v0 + grad * d, taking the
+    straight chord between source and node for the distance and depths
+    measured along the radius (flat-layer approximation of the spherical
+    grid): t = arccosh(1 + grad^2 L^2 / (2 v_s v_n)) / grad."""
+    k, j, i = np.meshgrid(np.arange(g.nz), np.arange(g.ny), np.arange(g.nx), indexing="ij")
+    r = geom["z0r"] - geom["h"] * k
+    q = geom["y0"] + geom["dq"] * j
+    f = geom["x0"] + geom["df"] * i
+    P = np.stack([r * np.cos(q), r * np.sin(q) * np.cos(f), r * np.sin(q) * np.sin(f)])
+    rs = geom["fzss"]
+    S = np.array([rs * math.cos(geom["fyss"]), rs * math.sin(geom["fyss"]) * math.cos(geom["fxss"]),
+                  rs * math.sin(geom["fyss"]) * math.sin(geom["fxss"])])
+    L = np.linalg.norm(P - S[:, None, None, None], axis=0)
+    vs = v0 + grad * (geom["z0r"] - rs)
+    vn = v0 + grad * (geom["z0r"] - r)
+    return np.arccosh(1.0 + grad * grad * L * L / (2.0 * vs * vn)) / grad
+
+
+# ------------------------------------------------------------------ C2
+
+def c2(seed: int = 2):
+    """(grid, Vp, Vs, stations) of C2."""
+    g = synth.centred_grid(161, 161, 81, 1.0)
+    vp = synth.gradient_velocity(g, 5.0, 8.0, 64.0) * synth.checkerboard(g, 20.0, 0.05)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    stations = synth.random_surface_stations(g, np.random.default_rng(seed), 48)
+    return g, vp.astype(np.float32), vs, stations
+
+
+# ------------------------------------------------------------------ C3
+
+@dataclasses.dataclass
+class WindowTable:
+    par: dict
+    origin: tuple        # model node of table node (0, 0, 0)
+    radius: float
+
+
+def c3(count: int = 256, seed: int = 3):
+    """(model grid, Vp, [WindowTable]) of C3."""
+    rng = np.random.default_rng(seed)
+    G = synth.centred_grid(401, 401, 101, 1.0)
+    v = synth.gradient_velocity(G, 5.0, 8.0, 64.0) * synth.gaussian_anomalies(G, rng, 3, 0.08, (10.0, 30.0))
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    dq, df = G.dq_deg(), G.df_deg()
+    out = []
+    for s in range(count):
+        r = float(rng.uniform(40.0, 200.0))
+        n_h = int(r) + 1
+        n_z = min(int(r), 100) + 1
+        # station anywhere on the surface; its box is clipped to the model
+        ci, cj = float(rng.uniform(0, G.nx - 1)), float(rng.uniform(0, G.ny - 1))
+        ox = int(min(max(round(ci - n_h / 2), 0), G.nx - 1))
+        oy = int(min(max(round(cj - n_h / 2), 0), G.ny - 1))
+        nx = min(n_h, G.nx - ox)
+        ny = min(n_h, G.ny - oy)
+        nz = n_z
+        # the station sits inside its window, two nodes from a clipped edge (the
+        # geodetic-to-geocentric conversion moves it by a fraction of a node)
+        ci = min(max(ci, ox + 2.0), ox + nx - 3.0)
+        cj = min(max(cj, oy + 2.0), oy + ny - 3.0)
+        x0 = round(G.x0 + ox * df, 6)
+        y0 = round(G.y0 - oy * dq, 6)
+        lon = round(G.x0 + ci * df, 5)
+        lat = round(G.y0 - cj * dq, 5)
+        par = dict(nx=nx, ny=ny, nz=nz, h=1.0, x0=x0, y0=y0, z0=0.0, dq=dq, df=df,
+                   fxs=lon, fys=lat, fzs=0.0, reverse=50)
+        out.append(WindowTable(par, (ox, oy, 0), r))
+    return G, v, out
+
+
+# ------------------------------------------------------------------ C4
+
+@dataclasses.dataclass
+class Request:
+    t_arrival: float
+    par: dict
+    shape: tuple         # (nz, ny, nx)
+    seed: int
+
+
+def c4_requests(count: int, rate: float, seed: int = 4) -> list[Request]:
+    """A Poisson stream: exponential inter-arrival times of mean 1/rate s,
+    sizes from 81^3 to 257 x 257 x 129 (nx = ny and nz interpolated)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        s = float(rng.uniform(0.0, 1.0))
+        nxy = int(round(81 + s * (257 - 81)))
+        nz = int(round(81 + s * (129 - 81)))
+        g = synth.centred_grid(nxy, nxy, nz, 1.0)
+        st = synth.random_surface_stations(g, rng, 1, prefix=f"R{k:04d}")[0]
+        out.append(Request(t, table_par(g, st), (nz, nxy, nxy), int(rng.integers(1 << 31))))
+    return out
+
+
+def c4_velocity(req: Request) -> np.ndarray:
+    """The request's velocities: gradient times 1 + three +-3 % separable
+    Gaussian blobs (sigma 10-30 km) -- cheap outer products, so the stream
+    can be generated at submission time."""
+    nz, ny, nx = req.shape
+    rng = np.random.default_rng(req.seed)
+    z = np.arange(nz, dtype=np.float32)
+    y = np.arange(ny, dtype=np.float32)
+    x = np.arange(nx, dtype=np.float32)
+    v = (5.0 + 3.0 * z / 64.0).astype(np.float32)[:, None, None] * np.ones((1, ny, nx), np.float32)
+    f = np.ones((nz, ny, nx), np.float32)
+    for _ in range(3):
+        cz, cy, cx = rng.uniform(0, nz), rng.uniform(0, ny), rng.uniform(0, nx)
+        sg = float(rng.uniform(10.0, 30.0))
+        a = np.float32(0.03 if rng.random() < 0.5 else -0.03)
+        gz = np.exp(-((z - cz) ** 2) / (2 * sg * sg)).astype(np.float32)
+        gy = np.exp(-((y - cy) ** 2) / (2 * sg * sg)).astype(np.float32)
+        gx = np.exp(-((x - cx) ** 2) / (2 * sg * sg)).astype(np.float32)
+        f += a * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    return np.ascontiguousarray(v * f, dtype=np.float32)
+
+
+# ------------------------------------------------------------------ C5
+
+def c5(stations: int = 1000, seed: int = 5):
+    """(grid, Vp, Vs, stations) of C5."""
+    rng = np.random.default_rng(seed)
+    g = synth.centred_grid(321, 321, 161, 1.0)
+    vp = synth.gradient_velocity(g, 5.0, 8.0, 64.0) * synth.layered_perturbation(g, rng, 0.04, 2.0, 12.0)
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    sts = synth.random_surface_stations(g, rng, stations)
+    return g, vp, vs, sts

@@ -96,6 +96,15 @@ int fdgpu_solve(fdgpu_ctx *ctx, const fdgpu_table_desc *tables, int n, int mode,
  * measured with HIP events on the launch stream. */
 double fdgpu_last_kernel_ms(fdgpu_ctx *ctx);
 
+/* Per-face schedule knob (no reference counterpart; results do not depend on
+ * it).  Reverse-pass faces first try the CHANGE schedule (evaluate the whole
+ * face at once, then re-evaluate what changed); when changes cascade through a
+ * face it is handed to the LEVEL schedule (each node once, in dependency
+ * order) after `change_budget` evaluations per face node (0: never).  Default
+ * 8; fdtomoc_amd/autotune.py picks it per workload by timing a sample. */
+int fdgpu_set_schedule(fdgpu_ctx *ctx, int change_budget);
+int fdgpu_get_schedule(fdgpu_ctx *ctx, int *change_budget);
+
 /* Copies a resident table (nx*ny*nz floats) to host memory. */
 int fdgpu_table_read(fdgpu_ctx *ctx, int table, float *host, unsigned long long count);
 /* Writes a resident table as the reference does (src/sphfd.c:5702-5803):

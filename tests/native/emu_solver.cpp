@@ -49,7 +49,7 @@ struct EmuPar {
 };
 
 extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const float *extra, int nthreads,
-                         int *stats) {
+                         int *stats, int change_budget) {
     fdprep::ParValues p;
     p.nx = pc->nx; p.ny = pc->ny; p.nz = pc->nz;
     p.h = pc->h; p.dq = pc->dq; p.df = pc->df; p.x0 = pc->x0; p.y0 = pc->y0; p.z0 = pc->z0;
@@ -100,6 +100,7 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
         smax = slow[q] > smax ? slow[q] : smax;
     }
     J.skip_ok = fdprep::skip_safe(p, pr, 1.0 / smax, 1.0 / smin) ? 1 : 0;
+    J.change_budget = change_budget;
     std::unique_ptr<fdsweep::Shared> shp(new fdsweep::Shared());
     fdsweep::Shared &sh = *shp;
     std::memset(&sh, 0, sizeof sh);

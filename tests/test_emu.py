@@ -26,7 +26,10 @@ def test_emulated_kernel_matches_oracle(emu, seed):
     s = ob.slowness(v)
     t_ref, st = ob.solve(par, s)
     nthreads = [1, 2, 3, 4, 8][seed % 5]
-    t, est = emu.solve(par, s, nthreads)
+    # the per-face schedule knob must not change a bit: 0 (CHANGE never gives
+    # up), 1 (most reverse-pass faces handed to LEVEL), 8 (default)
+    budget = [0, 1, 8][seed % 3]
+    t, est = emu.solve(par, s, nthreads, change_budget=budget)
     assert est["error"] == 0
     assert est["passes"] == st["passes"] and est["restarts"] == st["restarts"]
     assert int((t.view(np.uint32) != t_ref.view(np.uint32)).sum()) == 0
@@ -34,13 +37,18 @@ def test_emulated_kernel_matches_oracle(emu, seed):
 
 def test_schedules_exercised(emu):
     """Across a handful of cases both schedules run, carried registers are
-    resolved by deferral, and no CHANGE face needed the fallback."""
-    tot = {}
+    resolved by deferral, and with the budget switched off CHANGE faces
+    converge in few rounds; a budget of 1 hands faces to LEVEL."""
+    tot, tot1 = {}, {}
     for seed in range(100, 130):
         par, v = make_case(seed)
-        _, est = emu.solve(par, ob.slowness(v), 4)
+        _, est = emu.solve(par, ob.slowness(v), 4, change_budget=0)
         for k, x in est.items():
             tot[k] = tot.get(k, 0) + x
+        _, est = emu.solve(par, ob.slowness(v), 4, change_budget=1)
+        for k, x in est.items():
+            tot1[k] = tot1.get(k, 0) + x
     assert tot["faces_level"] > 0 and tot["faces_change"] > 0
     assert tot["defers"] > 0
     assert tot["rounds_change"] < tot["faces_change"] * 64
+    assert tot1["fallbacks"] > tot["fallbacks"]

@@ -151,3 +151,22 @@ def test_error_paths(ctx):
     with pytest.raises(fdgpu.FdgpuError):     # unknown model
         ctx.solve([fdgpu.TableSpec(d, m + 1000)])
     ctx.release_model(m)
+
+
+@pytest.mark.parametrize("budget", [0, 1, 32])
+def test_schedule_knob_does_not_change_results(ctx, budget):
+    """Every per-face schedule choice gives the oracle's bits (budget 1 hands
+    nearly every reverse-pass face to LEVEL, 0 never does)."""
+    from fdtomoc_amd import fdgpu
+    cases = [make_case(s) for s in range(300, 312)]
+    models = [ctx.model(v) for _, v in cases]
+    ctx.set_schedule(budget)
+    try:
+        ids = ctx.solve([fdgpu.TableSpec(par_dict(p), m) for (p, _), m in zip(cases, models)])
+    finally:
+        ctx.set_schedule(8)
+    for (p, v), tid in zip(cases, ids):
+        assert _bits_equal(ctx.read(tid, v.shape), ob.solve(p, ob.slowness(v))[0]) == 0
+        ctx.release(tid)
+    for m in models:
+        ctx.release_model(m)

@@ -245,3 +245,20 @@ def test_restart_sources_match_reference_run():
         nums[name] = st["passes"]
         assert bits_diff(t, ref) == 0, name
     assert set(nums) == {"R3.ptimes", "R2.ptimes", "FB.ptimes"}
+
+
+def test_linear_gradient_close_to_analytic():
+    """C1's medium (v = 5 + 3 z/64 km/s): FD times against the FP64
+    straight-chord flat-layer linear-gradient time, within the FD
+    discretisation error (stated: max 0.06 s, mean 0.013 s for this grid)."""
+    from fdtomoc_amd import workloads
+    g, v, st = workloads.c1()
+    p = workloads.table_par(g, st)
+    par = ob.Par(**{k: p[k] for k in ("nx", "ny", "nz", "h", "x0", "y0", "z0", "fxs", "fys", "fzs", "reverse")})
+    t, _ = ob.solve(par, ob.slowness(v))
+    exact = workloads.gradient_time_flat(g, ob.geometry(par), 5.0, 3.0 / 64.0)
+    err = np.abs(t - exact)
+    assert err.max() < 0.08 and err.mean() < 0.02
+    # a wrong gradient (a dropped term) is far outside that band
+    wrong = workloads.gradient_time_flat(g, ob.geometry(par), 5.0, 0.0 + 1e-9)
+    assert np.abs(t - wrong).max() > 0.3

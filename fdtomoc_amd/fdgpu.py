@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libfdtomoc_amd.so")
+LIB_PATH = os.environ.get("FDTOMOC_LIB") or os.path.join(PKG, "libfdtomoc_amd.so")   # env: another build
 EXE_PATH = os.path.join(PKG, "bin", "sphfd")
 
 STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",

@@ -55,7 +55,7 @@ BYTES_PER_VISIT = 12          # slowness read + time read + time write per node 
 # HBM bytes (FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0, copy-calibrated) that
 # rocprofv3 counted for one launch of the default 96-table workload
 # (profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md)
-TRAFFIC_BYTES_C2 = 367.03e9
+TRAFFIC_BYTES_C2 = 478.16e9
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 
 
@@ -139,6 +139,9 @@ def roofline(visits, kern_s, traffic=None, source=None):
 def tune(ctx, specs, args, sample=None):
     """Per-face schedule picked by the autotuner before timing (untimed, like
     the warmup); the tables are identical for every choice."""
+    if args.schedule >= 0:
+        ctx.set_schedule(args.schedule)
+        return {"change_budget": args.schedule, "tuned": False}
     if args.no_autotune:
         return {"change_budget": ctx.get_schedule(), "tuned": False}
     best, timings = autotune.tune_change_budget(ctx, specs, sample=sample)
@@ -460,6 +463,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-face schedule")
+    ap.add_argument("--schedule", type=int, default=-1, help="set the per-face schedule (CHANGE budget) instead of tuning")
     ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=3.0, help="c4: mean arrivals per second")
     ap.add_argument("--tables", type=int, default=256, help="c5: tables per launch")

@@ -27,9 +27,7 @@ ARCH = os.environ.get("FDTOMOC_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-Wno-unknown-pragmas"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-# lanes per sweep workgroup (FD_SWEEP_THREADS in csrc/fd_kernels.h); the host
-# side reads it back from the kernel through the occupancy query
-THREADS = os.environ.get("FDTOMOC_THREADS")
+
 SOURCES = [("sweep_kernel.hip", True), ("fdgpu.cpp", False), ("sphfd_io.cpp", False),
            ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False)]
 
@@ -51,8 +49,7 @@ def build(verbose: bool = False) -> str:
         src, dev = item
         obj = os.path.join(BUILD, src + ".o")
         host = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROCM, 'include')}"]
-        extra = [f"-DFD_SWEEP_THREADS={THREADS}"] if THREADS else []
-        cmd = [HIPCC] + COMMON + extra + (DEVICE if dev else host) + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + COMMON + (DEVICE if dev else host) + ["-c", os.path.join(CSRC, src), "-o", obj]
         _run(cmd, verbose)
         return obj
 

@@ -6,12 +6,20 @@
 #include "serve_shared.h"
 #include "sweep_solver.h"
 
-#ifndef FD_SWEEP_THREADS
-#define FD_SWEEP_THREADS 512   // lanes per workgroup of the sweep and server kernels (2 waves per SIMD)
-#endif
+// Two builds of the sweep kernel, both at two wavefronts per SIMD:
+//   wide    512 lanes, one workgroup per CU (LDS arena of kArenaWords words):
+//           the fastest per table, used while there are no more tables than CUs
+//   narrow  256 lanes, two workgroups per CU (LDS arena of kNarrowArenaWords):
+//           two tables share a CU, used for larger batches
+// The server kernel is built wide.
+enum { FD_WIDE = 0, FD_NARROW = 1 };
+constexpr int kWideThreads = 512, kNarrowThreads = 256;
+constexpr int kNarrowArenaWords = 20000;
 
-extern "C" hipError_t fd_launch_sweep(const fdsweep::TableJob *d_jobs, int njobs, const fdsweep::Scratch *d_slots,
-                                      int nslots, int *d_queue, hipStream_t stream);
+extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
+                                      const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream);
+// Kernel address of a sweep build, for occupancy queries.
+extern "C" const void *fd_sweep_kernel_ptr(int variant);
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream);
 extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done,
                                       int cap, const fdsweep::Scratch *d_slots, int nslots, fdsweep::ServeDev *dev,

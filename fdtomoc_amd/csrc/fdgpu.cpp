@@ -192,10 +192,13 @@ int fdgpu_open(int device, unsigned long long budget, fdgpu_ctx **out) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) { delete c; return FDGPU_EHIP; }
     c->budget = budget ? budget : (unsigned long long)(0.9 * (double)freeb);
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fd_sweep_kernel_ptr(), FD_SWEEP_THREADS, 0) == hipSuccess &&
-        per_cu > 0)
-        c->wg_per_cu = per_cu;
+    for (int v = 0; v < 2; v++) {
+        int per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fd_sweep_kernel_ptr(v),
+                                                         v == FD_NARROW ? kNarrowThreads : kWideThreads, 0) == hipSuccess &&
+            per_cu > 0)
+            c->wg_per_cu[v] = per_cu;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipMalloc(&c->d_queue, sizeof(int)) != hipSuccess) {
@@ -303,7 +306,10 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         rc = dev_alloc(c, (void **)&T.d_aux, T.bytes);
         if (rc) { dev_free(c, T.d_time, sizeof(float) * (size_t)T.n); T.d_time = nullptr; break; }
         made++;
-        std::vector<char> aux(T.bytes, 0);
+        // only the leading parts come from the host; the stamps and last-face
+        // marks behind them are initialised by the kernel
+        const size_t b_host = b_sin + b_idx + b_dist + b_val + b_st;
+        std::vector<char> aux(b_host, 0);
         size_t o = 0;
         memcpy(aux.data() + o, T.prep.sinq.data(), sizeof(double) * T.prep.sinq.size());
         const double *d_sin = (const double *)(T.d_aux + o);
@@ -322,7 +328,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         int *d_ms = (int *)(T.d_aux + o);
         o += b_ms;
         int *d_lf = (int *)(T.d_aux + o);
-        if (hipMemcpyAsync(T.d_aux, aux.data(), T.bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        if (hipMemcpyAsync(T.d_aux, aux.data(), b_host, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
             rc = FDGPU_EHIP;
             break;
         }
@@ -371,7 +377,13 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tabs[(size_t)a].n > tabs[(size_t)b].n; });
     std::vector<fdsweep::TableJob> qjobs((size_t)n);
     for (int t = 0; t < n; t++) qjobs[(size_t)t] = jobs[(size_t)order[(size_t)t]];
-    int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu) : 1;
+    // wide workgroups while every table gets a CU of its own, else narrow ones
+    int variant = FD_WIDE;
+    if (mode == FDGPU_BATCHED) {
+        if (c->lanes == kNarrowThreads) variant = FD_NARROW;
+        else if (c->lanes == 0 && n > c->num_cu * c->wg_per_cu[FD_WIDE]) variant = FD_NARROW;
+    }
+    int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu[variant]) : 1;
     if (nslots < 1) nslots = 1;
     rc = ensure_scratch(c, nslots, maxF);
     if (rc) { unwind(); return rc; }
@@ -382,11 +394,11 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
     if (mode == FDGPU_BATCHED) {
         ok = ok && hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream) == hipSuccess;
-        ok = ok && fd_launch_sweep(d_jobs, n, c->d_slots, nslots, c->d_queue, c->stream) == hipSuccess;
+        ok = ok && fd_launch_sweep(variant, d_jobs, n, c->d_slots, nslots, c->d_queue, c->stream) == hipSuccess;
     } else {
         for (int t = 0; t < n && ok; t++) {
             ok = hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream) == hipSuccess;
-            ok = ok && fd_launch_sweep(d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
+            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
         }
     }
     ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
@@ -417,6 +429,13 @@ int fdgpu_set_schedule(fdgpu_ctx *c, int change_budget) {
     if (!c || change_budget < 0) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
     c->change_budget = change_budget;
+    return FDGPU_OK;
+}
+
+int fdgpu_set_workgroup(fdgpu_ctx *c, int lanes) {
+    if (!c || (lanes != 0 && lanes != kNarrowThreads && lanes != kWideThreads)) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->lanes = lanes;
     return FDGPU_OK;
 }
 

@@ -137,7 +137,8 @@ struct fdgpu_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long budget = 0, used = 0;
     int num_cu = 0;
-    int wg_per_cu = 1;
+    int wg_per_cu[2] = {1, 1};   // resident workgroups per CU of the wide / narrow sweep kernel
+    int lanes = 0;               // workgroup choice (fdgpu_set_workgroup): 0 automatic, 256, 512
     std::map<int, fdgpu_detail::Model> models;
     std::map<int, fdgpu_detail::Table> tables;
     int next_id = 1;
@@ -177,5 +178,3 @@ int serve_stop_locked(fdgpu_ctx *c);
 // holds the context lock.
 int fdgpu_model_update_locked(fdgpu_ctx *c, int model, const float *vel);
 
-// Address of the sweep kernel, for occupancy queries.
-extern "C" const void *fd_sweep_kernel_ptr();

@@ -193,7 +193,7 @@ int fdgpu_serve_start(fdgpu_ctx *c, const fdgpu_serve_config *cfg) {
     S = Serve();
     S.maxdim[0] = cfg->max_nx; S.maxdim[1] = cfg->max_ny; S.maxdim[2] = cfg->max_nz;
     S.max_face = fdprep::max_face(cfg->max_nx, cfg->max_ny, cfg->max_nz);
-    const int full = c->num_cu * c->wg_per_cu;
+    const int full = c->num_cu * c->wg_per_cu[FD_WIDE];
     S.nslots = cfg->workgroups ? std::min(cfg->workgroups, full) : full;
     S.cap = cfg->queue_capacity ? cfg->queue_capacity : 4096;
     const size_t stage_bytes = cfg->staging_bytes ? (size_t)cfg->staging_bytes : (size_t)1 << 30;

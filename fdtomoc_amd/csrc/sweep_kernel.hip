@@ -2,7 +2,8 @@
 //
 //   sphfd_sweep_kernel   persistent: each workgroup pulls whole tables from a
 //                        queue and runs the complete sweep of each
-//                        (sweep_solver.h) with 256 lanes
+//                        (sweep_solver.h); built wide (512 lanes, one
+//                        workgroup per CU) and narrow (256 lanes, two per CU)
 //   sphfd_serve_kernel   persistent request loop for serving: workgroups claim
 //                        requests published by the host while the kernel runs
 //                        (serve_shared.h), each solved exactly like a batched
@@ -31,9 +32,10 @@ struct DevExec {
     __device__ __forceinline__ long long clock() const { return (long long)wall_clock64(); }
 };
 
-__global__ void __launch_bounds__(FD_SWEEP_THREADS)
+template <int THREADS, int ARENA>
+__global__ void __launch_bounds__(THREADS, 2)
     sphfd_sweep_kernel(const fdsweep::TableJob *jobs, int njobs, const fdsweep::Scratch *slots, int *queue) {
-    __shared__ fdsweep::Shared sh;
+    __shared__ fdsweep::SharedT<ARENA> sh;
     DevExec x;
     const fdsweep::Scratch W = slots[blockIdx.x];
     while (true) {
@@ -71,7 +73,7 @@ __device__ long long serve_claim(fdsweep::ServeCtl *ctl, fdsweep::ServeDev *dev,
     }
 }
 
-__global__ void __launch_bounds__(FD_SWEEP_THREADS)
+__global__ void __launch_bounds__(kWideThreads, 2)
     sphfd_serve_kernel(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done, int cap,
                        const fdsweep::Scratch *slots, fdsweep::ServeDev *dev, long long idle_ticks) {
     __shared__ fdsweep::Shared sh;
@@ -125,22 +127,29 @@ __global__ void slowness_kernel(const float *__restrict__ vel, float *__restrict
 
 }  // namespace
 
-extern "C" hipError_t fd_launch_sweep(const fdsweep::TableJob *d_jobs, int njobs, const fdsweep::Scratch *d_slots,
-                                      int nslots, int *d_queue, hipStream_t stream) {
-    hipLaunchKernelGGL(sphfd_sweep_kernel, dim3(nslots), dim3(FD_SWEEP_THREADS), 0, stream, d_jobs, njobs, d_slots,
-                       d_queue);
+extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
+                                      const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream) {
+    if (variant == FD_NARROW)
+        hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords>), dim3(nslots), dim3(kNarrowThreads),
+                           0, stream, d_jobs, njobs, d_slots, d_queue);
+    else
+        hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords>), dim3(nslots), dim3(kWideThreads), 0,
+                           stream, d_jobs, njobs, d_slots, d_queue);
     return hipGetLastError();
 }
 
 extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done,
                                       int cap, const fdsweep::Scratch *d_slots, int nslots, fdsweep::ServeDev *dev,
                                       long long idle_ticks, hipStream_t stream) {
-    hipLaunchKernelGGL(sphfd_serve_kernel, dim3(nslots), dim3(FD_SWEEP_THREADS), 0, stream, ctl, ring, done, cap,
+    hipLaunchKernelGGL(sphfd_serve_kernel, dim3(nslots), dim3(kWideThreads), 0, stream, ctl, ring, done, cap,
                        d_slots, dev, idle_ticks);
     return hipGetLastError();
 }
 
-extern "C" const void *fd_sweep_kernel_ptr() { return (const void *)sphfd_sweep_kernel; }
+extern "C" const void *fd_sweep_kernel_ptr(int variant) {
+    return variant == FD_NARROW ? (const void *)sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords>
+                                : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords>;
+}
 
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream) {
     const int threads = 256;

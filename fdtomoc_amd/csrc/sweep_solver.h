@@ -103,7 +103,9 @@ struct Scratch {
 // about 70K nodes; larger faces use global scratch).
 constexpr int kArenaWords = 36000;
 
-struct Shared {
+template <int AW>
+struct SharedT {
+    static constexpr int kArena = AW;
     int n[3];            // list sizes
     int processed;
     int headw[7];
@@ -114,8 +116,11 @@ struct Shared {
     int job;
     int face_ctr;        // faces swept so far in this table (stamps start at 1)
     int st[kStatCount];  // counters of the current table, flushed to TableJob::stats at the end
-    int arena[kArenaWords];
+    int arena[AW];
 };
+
+// Workgroup state of the one-workgroup-per-CU kernel (and of host builds).
+using Shared = SharedT<kArenaWords>;
 
 // ---------------------------------------------------------------- helpers
 
@@ -239,8 +244,8 @@ FD_INLINE int max_stamp(const TableJob &J, int i, int j, int k) {
 
 // Stable LSD sort of F (key, val) pairs by key, 4-bit digits, skipping digits
 // on which all keys agree.  Returns true if the result is in (key2, val2).
-template <class X>
-FD_INLINE bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1, int *v1, int F) {
+template <class X, class SH>
+FD_INLINE bool radix_sort(X &x, SH &sh, unsigned *k0, int *v0, unsigned *k1, int *v1, int F) {
     const int T = x.nthr(), tid = x.tid();
     const int chunk = (F + T - 1) / T;
     const int lo = tid * chunk < F ? tid * chunk : F;
@@ -298,8 +303,8 @@ FD_INLINE bool radix_sort(X &x, Shared &sh, unsigned *k0, int *v0, unsigned *k1,
 FD_INLINE int dir_da(int d) { return d == 0 || d == 3 || d == 5 ? -1 : (d == 1 || d == 6 ? 0 : 1); }
 FD_INLINE int dir_db(int d) { return d < 3 ? -1 : (d < 5 ? 0 : 1); }
 
-template <int F, class X>
-FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
+template <int F, class X, class SH>
+FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
                           const int *ord, int fcur) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F, wa = fg.wa, wb = fg.wb;
@@ -308,7 +313,7 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
     // pending counters, one byte per node packed four to a word, and successor
     // masks (bit d: the neighbour in direction d follows the node)
     const int nwords = (Fn + 3) / 4;
-    const bool in_lds = (long long)nwords * 4 + Fn <= (long long)kArenaWords * 4;
+    const bool in_lds = (long long)nwords * 4 + Fn <= (long long)SH::kArena * 4;
     unsigned *pend = in_lds ? (unsigned *)sh.arena : (unsigned *)W.pend;
     unsigned char *succ = in_lds ? (unsigned char *)(sh.arena + nwords) : W.flag;
     int *lists[3] = {W.list0, W.list1, W.list2};
@@ -423,8 +428,8 @@ FD_INLINE void face_level(X &x, Shared &sh, const TableJob &J, const Scratch &W,
 
 // CHANGE schedule.  Returns false (face restored) if LEVEL must redo the face.
 // Work list: W.list0 (nodes to evaluate), W.list2 (nodes that changed).
-template <int F, class X>
-FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
+template <int F, class X, class SH>
+FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G, const FaceGeom &fg,
                        int fcur, int fprev) {
     const int T = x.nthr(), tid = x.tid();
     const int Fn = fg.F;
@@ -572,8 +577,8 @@ FD_INLINE bool face_change(X &x, Shared &sh, const TableJob &J, const Scratch &W
     return true;
 }
 
-template <int F, class X>
-FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W, Geo &G) {
+template <int F, class X, class SH>
+FD_INLINE void sweep_face(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G) {
     const int T = x.nthr(), tid = x.tid();
     const int(*B)[2] = G.box;
     FaceGeom fg;
@@ -663,8 +668,8 @@ FD_INLINE void sweep_face(X &x, Shared &sh, const TableJob &J, const Scratch &W,
 
 // ------------------------------------------------------------- one table
 
-template <class X>
-FD_HD void solve_table(X &x, Shared &sh, const TableJob &J, const Scratch &W) {
+template <class X, class SH>
+FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
     const int T = x.nthr(), tid = x.tid();
     const long long N = (long long)J.nx * J.ny * J.nz;
     for (long long q = tid; q < N; q += T) J.mod_stamp[q] = 0;

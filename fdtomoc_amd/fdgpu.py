@@ -100,6 +100,7 @@ def lib():
         L.fdgpu_run_spec.argtypes = [C.c_char_p]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_serve_config_defaults.argtypes = [C.POINTER(CServeConfig)]
         L.fdgpu_serve_config_defaults.restype = None
         L.fdgpu_serve_start.argtypes = [C.c_void_p, C.POINTER(CServeConfig)]
@@ -197,6 +198,11 @@ class Context:
     def set_schedule(self, change_budget: int):
         """Per-face schedule knob (fdgpu_set_schedule); results do not depend on it."""
         _check(lib().fdgpu_set_schedule(self._h, int(change_budget)), "fdgpu_set_schedule")
+
+    def set_workgroup(self, lanes: int):
+        """Workgroup shape of batched launches: 512 (one table per CU), 256 (two
+        per CU) or 0 (automatic); results do not depend on it."""
+        _check(lib().fdgpu_set_workgroup(self._h, int(lanes)), "fdgpu_set_workgroup")
 
     def get_schedule(self) -> int:
         b = C.c_int()

@@ -104,6 +104,10 @@ double fdgpu_last_kernel_ms(fdgpu_ctx *ctx);
  * 8; fdtomoc_amd/autotune.py picks it per workload by timing a sample. */
 int fdgpu_set_schedule(fdgpu_ctx *ctx, int change_budget);
 int fdgpu_get_schedule(fdgpu_ctx *ctx, int *change_budget);
+/* Workgroup shape of batched launches (no reference counterpart; results do
+ * not depend on it): 512 lanes and one workgroup (table) per CU, or 256 lanes
+ * and two per CU.  0 (default): 512 while the tables fit one per CU, else 256. */
+int fdgpu_set_workgroup(fdgpu_ctx *ctx, int lanes);
 
 /* Copies a resident table (nx*ny*nz floats) to host memory. */
 int fdgpu_table_read(fdgpu_ctx *ctx, int table, float *host, unsigned long long count);

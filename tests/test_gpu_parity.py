@@ -170,3 +170,24 @@ def test_schedule_knob_does_not_change_results(ctx, budget):
         ctx.release(tid)
     for m in models:
         ctx.release_model(m)
+
+
+@pytest.mark.parametrize("lanes", [256, 512])
+def test_workgroup_shapes_give_the_same_bits(ctx, lanes):
+    """Both sweep builds (256 lanes, two tables per CU; 512 lanes, one per CU)
+    reproduce the oracle."""
+    from fdtomoc_amd import fdgpu
+    cases = [make_case(s) for s in range(400, 416)]
+    models = [ctx.model(v) for _, v in cases]
+    ctx.set_workgroup(lanes)
+    try:
+        ids = ctx.solve([fdgpu.TableSpec(par_dict(p), m) for (p, _), m in zip(cases, models)])
+    finally:
+        ctx.set_workgroup(0)
+    for (p, v), tid in zip(cases, ids):
+        st = ctx.stats(tid)
+        assert st["error"] & ~1 == 0
+        assert _bits_equal(ctx.read(tid, v.shape), ob.solve(p, ob.slowness(v))[0]) == 0
+        ctx.release(tid)
+    for m in models:
+        ctx.release_model(m)

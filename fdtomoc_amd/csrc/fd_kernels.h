@@ -20,6 +20,8 @@ extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jo
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream);
 // Kernel address of a sweep build, for occupancy queries.
 extern "C" const void *fd_sweep_kernel_ptr(int variant);
+// Sets the table queue counter to 0 (a kernel, so that it can sit in a graph).
+extern "C" hipError_t fd_launch_reset(int *d_queue, hipStream_t stream);
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream);
 extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done,
                                       int cap, const fdsweep::Scratch *d_slots, int nslots, fdsweep::ServeDev *dev,

@@ -81,6 +81,8 @@ int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
     if (nslots <= c->slots_cap && maxF <= c->slot_face_cap) return FDGPU_OK;
     const int ns = std::max(nslots, c->slots_cap);
     const long long F = std::max(maxF, c->slot_face_cap);
+    (void)hipStreamSynchronize(c->stream);
+    clear_graphs(c);
     dev_free(c, c->d_scratch, c->scratch_bytes);
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
     c->d_scratch = nullptr;
@@ -112,6 +114,52 @@ fdprep::ParValues to_values(const fdgpu_par &p) {
     v.invh = p.invh; v.invq = p.invq; v.invf = p.invf; v.invx0 = p.invx0; v.invy0 = p.invy0; v.invz0 = p.invz0;
     v.swab = p.swab;
     return v;
+}
+
+void clear_graphs(fdgpu_ctx *c) {
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
+
+// Device array of table descriptors, kept between solves so that captured
+// launches stay valid; grown on demand.
+int ensure_jobs(fdgpu_ctx *c, int n) {
+    if (n <= c->jobs_cap) return FDGPU_OK;
+    (void)hipStreamSynchronize(c->stream);
+    clear_graphs(c);
+    dev_free(c, c->d_jobs, sizeof(fdsweep::TableJob) * (size_t)c->jobs_cap);
+    c->d_jobs = nullptr;
+    c->jobs_cap = 0;
+    const int cap = std::max(n, 2 * c->jobs_cap);
+    int rc = dev_alloc(c, (void **)&c->d_jobs, sizeof(fdsweep::TableJob) * (size_t)cap);
+    if (rc) return rc;
+    c->jobs_cap = cap;
+    return FDGPU_OK;
+}
+
+// The batched launch as a graph (a queue-reset kernel, then the persistent
+// sweep kernel), captured once per (build, table count, workgroups) and
+// replayed; descriptors and scratch pointers are fixed for its lifetime.
+int sweep_graph(fdgpu_ctx *c, int variant, int n, int nslots, hipGraphExec_t *out) {
+    const long long key = ((long long)variant << 48) | ((long long)nslots << 24) | (long long)n;
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end()) { *out = it->second; return FDGPU_OK; }
+    hipGraph_t graph = nullptr;
+    FD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = fd_launch_reset(c->d_queue, c->stream);
+    hipError_t e2 = fd_launch_sweep(variant, c->d_jobs, n, c->d_slots, nslots, c->d_queue, c->stream);
+    hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return FDGPU_EHIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e4 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e4 != hipSuccess) return FDGPU_EHIP;
+    c->graphs[key] = exec;
+    *out = exec;
+    return FDGPU_OK;
 }
 
 int prepare_table(const fdgpu_table_desc &d, int mnx, int mny, int mnz, Table &T, std::vector<float> &preset) {
@@ -219,6 +267,8 @@ int fdgpu_close(fdgpu_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->tables) release_table(c, kv.second);
     for (auto &kv : c->models) dev_free(c, kv.second.d_slow, kv.second.bytes);
+    clear_graphs(c);
+    dev_free(c, c->d_jobs, sizeof(fdsweep::TableJob) * (size_t)c->jobs_cap);
     dev_free(c, c->d_scratch, c->scratch_bytes);
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
     if (c->d_queue) (void)hipFree(c->d_queue);
@@ -387,17 +437,20 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     if (nslots < 1) nslots = 1;
     rc = ensure_scratch(c, nslots, maxF);
     if (rc) { unwind(); return rc; }
-    fdsweep::TableJob *d_jobs = nullptr;
-    rc = dev_alloc(c, (void **)&d_jobs, sizeof(fdsweep::TableJob) * (size_t)std::max(n, 1));
+    rc = ensure_jobs(c, n);
     if (rc) { unwind(); return rc; }
-    bool ok = hipMemcpy(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice) == hipSuccess;
+    fdsweep::TableJob *d_jobs = c->d_jobs;
+    bool ok = hipMemcpyAsync(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess;
     ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
     if (mode == FDGPU_BATCHED) {
-        ok = ok && hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream) == hipSuccess;
-        ok = ok && fd_launch_sweep(variant, d_jobs, n, c->d_slots, nslots, c->d_queue, c->stream) == hipSuccess;
+        // one captured graph per launch shape: queue reset + persistent sweep
+        hipGraphExec_t g = nullptr;
+        ok = ok && sweep_graph(c, variant, n, nslots, &g) == FDGPU_OK;
+        ok = ok && hipGraphLaunch(g, c->stream) == hipSuccess;
     } else {
         for (int t = 0; t < n && ok; t++) {
-            ok = hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream) == hipSuccess;
+            ok = fd_launch_reset(c->d_queue, c->stream) == hipSuccess;
             ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
         }
     }
@@ -405,7 +458,6 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
     float ms = 0.f;
     if (ok && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
-    dev_free(c, d_jobs, sizeof(fdsweep::TableJob) * (size_t)std::max(n, 1));
     if (!ok) { unwind(); return FDGPU_EHIP; }
     int kerr = 0;
     for (int t = 0; t < n; t++) {

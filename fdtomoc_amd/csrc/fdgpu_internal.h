@@ -149,6 +149,9 @@ struct fdgpu_ctx {
     int slots_cap = 0;
     long long slot_face_cap = 0;
     int *d_queue = nullptr;
+    fdsweep::TableJob *d_jobs = nullptr;   // descriptors of the current batch
+    int jobs_cap = 0;
+    std::map<long long, hipGraphExec_t> graphs;   // captured batched launches
     double last_ms = 0.0;
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     fdgpu_detail::Serve serve;
@@ -161,6 +164,8 @@ int hip_fail(hipError_t e);
 int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes);
 void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
 size_t slot_bytes(long long F);
+// Destroys the captured batched launches (their pointers are about to change).
+void clear_graphs(fdgpu_ctx *c);
 // Carves `ns` scratch slots for faces of up to F nodes out of `base` and
 // uploads their descriptors to `d_slots`.
 int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots);

@@ -120,6 +120,10 @@ __global__ void __launch_bounds__(kWideThreads, 2)
     }
 }
 
+__global__ void reset_kernel(int *q) {
+    if (threadIdx.x == 0) q[0] = 0;
+}
+
 __global__ void slowness_kernel(const float *__restrict__ vel, float *__restrict__ slow, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         slow[i] = 1.0f / vel[i];
@@ -149,6 +153,11 @@ extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::Ser
 extern "C" const void *fd_sweep_kernel_ptr(int variant) {
     return variant == FD_NARROW ? (const void *)sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords>
                                 : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords>;
+}
+
+extern "C" hipError_t fd_launch_reset(int *d_queue, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, d_queue);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream) {

@@ -191,3 +191,25 @@ def test_workgroup_shapes_give_the_same_bits(ctx, lanes):
         ctx.release(tid)
     for m in models:
         ctx.release_model(m)
+
+
+def test_captured_launch_replays(ctx):
+    """Batched solves replay one captured graph per launch shape: the same
+    batch solved three times, with other work in between, gives the oracle's
+    bits every time."""
+    from fdtomoc_amd import fdgpu
+    cases = [make_case(s) for s in range(500, 506)]
+    refs = [ob.solve(p, ob.slowness(v))[0] for p, v in cases]
+    models = [ctx.model(v) for _, v in cases]
+    specs = [fdgpu.TableSpec(par_dict(p), m) for (p, _), m in zip(cases, models)]
+    for rep in range(3):
+        ids = ctx.solve(specs)
+        for tid, t_ref in zip(ids, refs):
+            assert _bits_equal(ctx.read(tid, t_ref.shape), t_ref) == 0, rep
+            ctx.release(tid)
+        # eager work of another shape between the replays
+        (other,) = ctx.solve(specs[rep:rep + 1])
+        assert _bits_equal(ctx.read(other, refs[rep].shape), refs[rep]) == 0
+        ctx.release(other)
+    for m in models:
+        ctx.release_model(m)

@@ -435,7 +435,13 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
     const int Fn = fg.F;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
     if (tid == 0) { sh.n[0] = 0; sh.n[2] = 0; sh.flag &= ~4; sh.best = 0; }
-    for (int p = tid; p < Fn; p += T) W.stamp[p] = 0;
+    // followers queued in the current round: a bit per node in LDS when the
+    // face fits, else a round stamp per node in scratch
+    const int nbits = (Fn + 31) / 32;
+    const bool qlds = nbits <= SH::kArena;
+    unsigned *qbits = (unsigned *)sh.arena;
+    if (!qlds)
+        for (int p = tid; p < Fn; p += T) W.stamp[p] = 0;
     x.sync();
     long long spent = 0;
     for (int round = 1;; round++) {
@@ -498,6 +504,8 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
         }
         x.sync();
         if (tid == 0) sh.n[0] = 0;
+        if (qlds)
+            for (int w = tid; w < nbits; w += T) qbits[w] = 0u;
         x.sync();
         // queue the followers of every node that changed, once each
         const int nch = sh.n[2];
@@ -512,7 +520,14 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
                     if (aa < 0 || bb < 0 || aa >= fg.wa || bb >= fg.wb) continue;
                     const int q = bb * fg.wa + aa;
                     if (!key_less(kp, p, W.key[q], q)) continue;
-                    if (x.atomic_exch(&W.stamp[q], round) != round) W.list0[x.atomic_add(&sh.n[0], 1)] = q;
+                    bool first;
+                    if (qlds) {
+                        const unsigned bit = 1u << (q & 31);
+                        first = !(x.atomic_or(&qbits[q >> 5], bit) & bit);
+                    } else {
+                        first = x.atomic_exch(&W.stamp[q], round) != round;
+                    }
+                    if (first) W.list0[x.atomic_add(&sh.n[0], 1)] = q;
                 }
         }
         x.sync();
@@ -528,15 +543,27 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
         if (best < 0 || key_less(bestk, best, W.key[p], p)) { best = p; bestk = W.key[p]; }
     }
     if (bad) x.atomic_or((unsigned *)&sh.flag, 4u);
-    sh.arena[tid] = best;
+    // the last node in face order: tree reduction of (key, index) in LDS
+    int *rb = sh.arena;
+    unsigned *rk = (unsigned *)(sh.arena + T);
+    rb[tid] = best;
+    rk[tid] = bestk;
+    for (int n = T; n > 1;) {   // any lane count: fold the upper half onto the lower
+        const int half = (n + 1) / 2;
+        x.sync();
+        if (tid < n - half) {
+            const int b2 = rb[tid + half];
+            const unsigned k2 = rk[tid + half];
+            if (b2 >= 0 && (rb[tid] < 0 || key_less(rk[tid], rb[tid], k2, b2))) {
+                rb[tid] = b2;
+                rk[tid] = k2;
+            }
+        }
+        n = half;
+    }
     x.sync();
     if (tid == 0) {
-        int bp = -1;
-        unsigned bk = 0;
-        for (int q = 0; q < T; q++) {
-            const int c = sh.arena[q];
-            if (c >= 0 && (bp < 0 || key_less(bk, bp, W.key[c], c))) { bp = c; bk = W.key[c]; }
-        }
+        const int bp = rb[0];
         sh.best = bp;
         if (bp >= 0 && !(W.flag[bp] & 8)) {
             // the last node in face order was skipped: evaluate it once for the

@@ -14,7 +14,7 @@
 // The server kernel is built wide.
 enum { FD_WIDE = 0, FD_NARROW = 1 };
 constexpr int kWideThreads = 512, kNarrowThreads = 256;
-constexpr int kNarrowArenaWords = 20000;
+constexpr int kNarrowArenaWords = 13600;
 
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream);
@@ -22,6 +22,8 @@ extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jo
 extern "C" const void *fd_sweep_kernel_ptr(int variant);
 // Sets the table queue counter to 0 (a kernel, so that it can sit in a graph).
 extern "C" hipError_t fd_launch_reset(int *d_queue, hipStream_t stream);
+// Fills the geometry table of one grid (p->sinq is a device pointer).
+extern "C" hipError_t fd_launch_geo(const fdsweep::GeoParams *p, double *d_out, hipStream_t stream);
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream);
 extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::ServeJob *ring, unsigned long long *done,
                                       int cap, const fdsweep::Scratch *d_slots, int nslots, fdsweep::ServeDev *dev,

@@ -116,6 +116,56 @@ fdprep::ParValues to_values(const fdgpu_par &p) {
     return v;
 }
 
+void clear_geos(fdgpu_ctx *c) {
+    for (auto &kv : c->geos) dev_free(c, kv.second.d, kv.second.bytes);
+    c->geos.clear();
+}
+
+namespace {
+
+unsigned long long dbits(double v) {
+    unsigned long long u;
+    memcpy(&u, &v, 8);
+    return u;
+}
+
+// The geometry table of a table's grid from the cache, computed on the
+// device on first use (J.sinq must already be on the device).  When the
+// budget is short, cached tables that `keep` does not name are dropped first.
+int geometry_for(fdgpu_ctx *c, const fdsweep::TableJob &J, const std::vector<double> &sinq,
+                 std::map<std::vector<unsigned long long>, int> &keep, const double **out) {
+    std::vector<unsigned long long> key = {(unsigned long long)J.ny, (unsigned long long)J.nz, dbits(J.h), dbits(J.dq),
+                                           dbits(J.df), dbits(J.z0r)};
+    for (double v : sinq) key.push_back(dbits(v));
+    auto it = c->geos.find(key);
+    if (it == c->geos.end()) {
+        GeoBuf g;
+        g.bytes = sizeof(double) * (size_t)fdsweep::geo_doubles(J.ny, J.nz);
+        int rc = dev_alloc(c, (void **)&g.d, g.bytes);
+        if (rc == FDGPU_ENOMEM) {
+            (void)hipStreamSynchronize(c->stream);
+            for (auto g2 = c->geos.begin(); g2 != c->geos.end();) {
+                if (keep.count(g2->first)) { ++g2; continue; }
+                dev_free(c, g2->second.d, g2->second.bytes);
+                g2 = c->geos.erase(g2);
+            }
+            rc = dev_alloc(c, (void **)&g.d, g.bytes);
+        }
+        if (rc) return rc;
+        const fdsweep::GeoParams gp{J.ny, J.nz, J.h, J.dq, J.df, J.z0r, J.sinq};
+        if (fd_launch_geo(&gp, g.d, c->stream) != hipSuccess) {
+            dev_free(c, g.d, g.bytes);
+            return FDGPU_EHIP;
+        }
+        it = c->geos.emplace(key, g).first;
+    }
+    keep[key] = 1;
+    *out = it->second.d;
+    return FDGPU_OK;
+}
+
+}  // namespace
+
 void clear_graphs(fdgpu_ctx *c) {
     for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     c->graphs.clear();
@@ -268,6 +318,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     for (auto &kv : c->tables) release_table(c, kv.second);
     for (auto &kv : c->models) dev_free(c, kv.second.d_slow, kv.second.bytes);
     clear_graphs(c);
+    clear_geos(c);
     dev_free(c, c->d_jobs, sizeof(fdsweep::TableJob) * (size_t)c->jobs_cap);
     dev_free(c, c->d_scratch, c->scratch_bytes);
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
@@ -337,6 +388,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     }
     // device buffers per table
     std::vector<fdsweep::TableJob> jobs((size_t)n);
+    std::map<std::vector<unsigned long long>, int> geo_keep;
     int rc = FDGPU_OK;
     int made = 0;
     for (int t = 0; t < n && rc == FDGPU_OK; t++) {
@@ -415,6 +467,8 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         J.maxdim = maxdim;
         J.skip_ok = fdprep::skip_safe(to_values(T.par), T.prep, m.vmin, m.vmax) ? 1 : 0;
         J.change_budget = c->change_budget;
+        rc = geometry_for(c, J, T.prep.sinq, geo_keep, &J.geo);
+        if (rc) break;
     }
     auto unwind = [&]() {
         for (int t = 0; t < made; t++) release_table(c, tabs[(size_t)t]);

@@ -86,6 +86,12 @@ class RangeAlloc {
     size_t total_ = 0, used_ = 0;
 };
 
+// Device geometry tables of one grid (fdsweep::geo_fill_item layout).
+struct GeoBuf {
+    double *d = nullptr;
+    size_t bytes = 0;
+};
+
 // One admitted serving request.
 struct ServeReq {
     unsigned long long seq = 0;
@@ -152,6 +158,9 @@ struct fdgpu_ctx {
     fdsweep::TableJob *d_jobs = nullptr;   // descriptors of the current batch
     int jobs_cap = 0;
     std::map<long long, hipGraphExec_t> graphs;   // captured batched launches
+    // geometry tables of the grids solved so far, keyed by the bits of
+    // (ny, nz, h, dq, df, z0r, sin(qc) table); shared by every table of a grid
+    std::map<std::vector<unsigned long long>, fdgpu_detail::GeoBuf> geos;
     double last_ms = 0.0;
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     fdgpu_detail::Serve serve;
@@ -170,6 +179,8 @@ void clear_graphs(fdgpu_ctx *c);
 // uploads their descriptors to `d_slots`.
 int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots);
 fdprep::ParValues to_values(const fdgpu_par &p);
+// Frees every cached geometry table (stream idle).
+void clear_geos(fdgpu_ctx *c);
 // Host preparation of one table request (validation, geometry, source); `preset`
 // receives the srctype 2/3 initial grid.
 int prepare_table(const fdgpu_table_desc &d, int model_nx, int model_ny, int model_nz, Table &T,

@@ -46,9 +46,9 @@ namespace {
 
 constexpr double kIdleSeconds = 0.25;
 
-// Per-request device block: time | sinq | stats | stamps | last_full | slowness
+// Per-request device block: time | sinq | stats | stamps | last_full | geometry table | slowness
 struct DevLayout {
-    size_t time, sinq, stats, stamp, lastf, slow, total;
+    size_t time, sinq, stats, stamp, lastf, geo, slow, total;
 };
 
 DevLayout dev_layout(const fdgpu_par &p, bool own_vel) {
@@ -61,6 +61,7 @@ DevLayout dev_layout(const fdgpu_par &p, bool own_vel) {
     L.stats = o; o += align256(sizeof(int) * fdsweep::kStatCount);
     L.stamp = o; o += align256(sizeof(int) * (size_t)n);
     L.lastf = o; o += align256(sizeof(int) * 6 * (size_t)maxdim);
+    L.geo = o; o += align256(sizeof(double) * (size_t)fdsweep::geo_doubles(p.ny, p.nz));
     L.slow = o; if (own_vel) o += align256(sizeof(float) * (size_t)n);
     L.total = o;
     return L;
@@ -337,6 +338,7 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
     J.maxdim = std::max(p.nx, std::max(p.ny, p.nz));
     J.skip_ok = fdprep::skip_safe(to_values(p), T.prep, vmin, vmax) ? 1 : 0;
     J.change_budget = c->change_budget;
+    J.geo_fill = (double *)(blk + DL.geo);   // filled by the workgroup that takes the request
     job.vel_src = own ? (const float *)(st + SL.vel) : nullptr;
     job.slow_dst = own ? (float *)(blk + DL.slow) : nullptr;
     job.nvel = own ? T.n : 0;
@@ -431,6 +433,12 @@ int fdgpu_serve_write(fdgpu_ctx *c, int request, const char *path, const void *v
         prep = R->prep;
     }
     return fdio::write_table(path, par, prep, vel_header, host.data());
+}
+
+int fdgpu_serve_request_bytes(const fdgpu_par *par, int own_velocity, unsigned long long *bytes) {
+    if (!par || !bytes || par->nx <= 0 || par->ny <= 0 || par->nz <= 0) return FDGPU_EINPUT;
+    *bytes = (unsigned long long)dev_layout(*par, own_velocity != 0).total;
+    return FDGPU_OK;
 }
 
 int fdgpu_serve_release(fdgpu_ctx *c, int request) {

@@ -46,74 +46,69 @@ constexpr double kBig = 1.0e9;      // "computed" threshold used by every gate
 constexpr double kAcausal = 1.e11;  // stencil rejection value
 constexpr double kHeadTest = 1.e-3; // sphfd.c:360
 
-// ---------------------------------------------------------------- stencils
-
-// 8-point transmission stencil.  s holds 8 slownesses, r/q radii and
-// colatitudes, sq[] = sin(q[]) (host libm values), g/n/m the +-1 signs.
-FD_INLINE double st_3d(const double t[8], const double s[8], const double r[8], const double sq[8],
-                       const int g[8], const int n[8], const int m[8], double h, double dq, double df) {
-    double slo = 0.;
-#pragma unroll
-    for (int i = 0; i < 8; i++) slo += s[i];
-    slo = .125 * slo;
-    const double hsq = h * h, dqsq = dq * dq, dfsq = df * df;
-    const double dfsinq7 = df * sq[7];
-    const double df2sinq7 = df * dfsinq7;
-    const double a = 1.0 / hsq + (1.0 / dqsq + 1. / (dfsinq7 * dfsinq7)) / (r[7] * r[7]);
-    double b = 0;
-#pragma unroll
-    for (int i = 0; i < 7; i++)
-        b += t[i] * ((double)(g[7] * g[i]) / hsq +
-                     ((double)(n[7] * n[i]) / dqsq + (double)(m[7] * m[i]) / (df2sinq7 * sq[i])) / (r[7] * r[i]));
-    b = 2.0 * b;
-    double c = 0.;
-#pragma unroll
-    for (int i = 0; i < 7; i++) {
-        const double dfsinqi = df * sq[i];
-        c += t[i] * t[i] * (1. / hsq + (1. / dqsq + 1. / (dfsinqi * dfsinqi)) / (r[i] * r[i]));
+// Calls fn(integral_constant<int, I>) for I = I0 .. N-1.
+template <int I, int N, class Fn>
+FD_INLINE void static_for(Fn &&fn) {
+    if constexpr (I < N) {
+        fn(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(fn);
     }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        double c2 = 0.;
-        const double df2sinqi = dfsq * sq[i];
-#pragma unroll
-        for (int j = i + 1; j < 7; j++)
-            c2 += t[j] * ((double)(g[i] * g[j]) / hsq +
-                          ((double)(n[i] * n[j]) / dqsq + (double)(m[i] * m[j]) / (df2sinqi * sq[j])) / (r[i] * r[j]));
-        c += 2.0 * c2 * t[i];
-    }
-    c = c - 16.0 * slo * slo;
-    const double rad = b * b - 4.0 * a * c;
-    if (!(rad >= 0.)) return kAcausal;
-    const double sr = sqrt(rad);
-    const double x1 = (-b + sr) / (2.0 * a);
-    const double x2 = (-b - sr) / (2.0 * a);
-    bool bad1 = false, bad2 = false;
-#pragma unroll
-    for (int i = 0; i < 7; i++) {
-        bad1 |= (x1 < t[i]);
-        bad2 |= (x2 < t[i]);
-    }
-    if (bad1) return bad2 ? kAcausal : x2;
-    if (bad2) return x1;
-    return (x1 < x2) ? x1 : x2;
 }
 
-// New-edge stencil: points 0..4 known, point 5 the node.
-FD_INLINE double st_ne(const double t[8], const double s[8], double d01, double d12, double d14, double d25,
-                       double d45) {
+// ---------------------------------------------------------------- division
+
+// Correctly rounded n / d from y = RN(1/d) (Markstein: with y the correctly
+// rounded reciprocal and q0 = RN(n*y) within one ulp of n/d, the residual
+// d*q0 - n is exact and q0 - residual*y rounds to the correctly rounded
+// quotient, barring overflow and underflow).  Three FP64 operations instead
+// of a full division.  Special operands (infinite or NaN numerator, zero or
+// infinite divisor) get the IEEE result: on the GPU from v_div_fixup_f64, the
+// special-case step of the hardware division sequence, which otherwise
+// passes the quotient through; on the host from the division itself.  Every
+// divisor passed here is a grid distance or a geometric constant with a
+// precomputed reciprocal; the operand magnitudes of the stencils (float
+// times and slownesses, kilometre distances) keep the residual far from
+// underflow, so results are bit-identical to `n / d`.
+FD_INLINE double xdiv(double n, double d, double y) {
+    const double q0 = n * y;
+    const double e = fma(d, q0, -n);
+    const double q = fma(-e, y, q0);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_div_fixup(q, d, n);
+#else
+    return __builtin_isfinite(q) && __builtin_isfinite(d) && d != 0.0 ? q : n / d;
+#endif
+}
+
+// ---------------------------------------------------------------- stencils
+//
+// The stencils of sphfd.c with every sub-expression that depends on the grid
+// geometry only (not on times or slownesses) taken from a per-table geometry
+// table (below), where it is computed with the same IEEE operations in the
+// same order; divisions by grid distances use xdiv.  The remaining arithmetic
+// follows the reference's expression trees operation by operation.
+
+// A grid distance with its correctly rounded reciprocal.
+struct Dist {
+    double d, y;
+};
+
+// New-edge stencil (fdsphne, sphfd.c:6016-6074): points 0..4 known, point 5
+// the node; a = 1/d25^2 + 1/d45^2 and ya = RN(1/(2a)) from the table.
+FD_INLINE double st_ne(const double t[8], const double s[8], Dist d01, Dist d12, Dist d14, Dist d25, Dist d45,
+                       double a, double ya) {
     const double slo = .25 * (s[1] + s[2] + s[4] + s[5]);
-    const double dt1 = (t[4] - t[1]) / d14 - t[2] / d25;
-    const double dt2 = (t[2] - t[1]) / d12 - t[4] / d45;
-    const double dt3 = (t[0] - t[3]) / d01;
-    const double a = 1. / (d25 * d25) + 1. / (d45 * d45);
-    const double b = 2 * (dt1 / d25 + dt2 / d45);
+    const double dt1 = xdiv(t[4] - t[1], d14.d, d14.y) - xdiv(t[2], d25.d, d25.y);
+    const double dt2 = xdiv(t[2] - t[1], d12.d, d12.y) - xdiv(t[4], d45.d, d45.y);
+    const double dt3 = xdiv(t[0] - t[3], d01.d, d01.y);
+    const double b = 2 * (xdiv(dt1, d25.d, d25.y) + xdiv(dt2, d45.d, d45.y));
     const double c = dt1 * dt1 + dt2 * dt2 + dt3 * dt3 - 4 * slo * slo;
     const double rad = b * b - 4.0 * a * c;
     if (!(rad >= 0.)) return kAcausal;
     const double sr = sqrt(rad);
-    const double x1 = (-b + sr) / (2 * a);
-    const double x2 = (-b - sr) / (2 * a);
+    const double a2 = 2 * a;
+    const double x1 = xdiv(-b + sr, a2, ya);
+    const double x2 = xdiv(-b - sr, a2, ya);
     bool bad1 = false, bad2 = false;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -125,23 +120,24 @@ FD_INLINE double st_ne(const double t[8], const double s[8], double d01, double 
     return (x2 < x1) ? x2 : x1;
 }
 
-// 2D face stencil.  Causality is tested against t[0] and t[4] as well, which
-// the caller carries over from earlier trials (the reference's 2D call sites
-// never set them).
-FD_INLINE double st_2d(const double t[8], const double s[8], double d01, double d02, double d13, double d23) {
+// 2D face stencil (fdsph2d, sphfd.c:6077-6143).  Causality is tested against
+// t[0] and t[4] as well, which the caller carries over from earlier trials
+// (the reference's 2D call sites never set them); a = 1/d01^2 + 1/d02^2 and
+// ya = RN(1/(2a)) from the table.
+FD_INLINE double st_2d(const double t[8], const double s[8], Dist d01, Dist d02, Dist d13, Dist d23, double a,
+                       double ya) {
     const double slo = .25 * (s[0] + s[1] + s[2] + s[3]);
-    const double d01sq = d01 * d01, d02sq = d02 * d02;
     const double t1mt3 = t[1] - t[3], t2mt3 = t[2] - t[3];
-    const double t1t3t2 = t1mt3 / d13 - t[2] / d02;
-    const double t2t3t1 = t2mt3 / d23 - t[1] / d01;
-    const double a = (1.0 / d01sq + 1. / d02sq);
-    const double b = 2.0 * (t1t3t2 / d02 + t2t3t1 / d01);
+    const double t1t3t2 = xdiv(t1mt3, d13.d, d13.y) - xdiv(t[2], d02.d, d02.y);
+    const double t2t3t1 = xdiv(t2mt3, d23.d, d23.y) - xdiv(t[1], d01.d, d01.y);
+    const double b = 2.0 * (xdiv(t1t3t2, d02.d, d02.y) + xdiv(t2t3t1, d01.d, d01.y));
     const double c = t1t3t2 * t1t3t2 + t2t3t1 * t2t3t1 - 4 * slo * slo;
     const double rad = b * b - 4.0 * a * c;
     if (!(rad >= 0.)) return kAcausal;
     const double sr = sqrt(rad);
-    const double x1 = (-b + sr) / (2 * a);
-    const double x2 = (-b - sr) / (2 * a);
+    const double a2 = 2 * a;
+    const double x1 = xdiv(-b + sr, a2, ya);
+    const double x2 = xdiv(-b - sr, a2, ya);
     const bool bad1 = (x1 < t[1]) | (x1 < t[2]) | (x1 < t[3]) | (x1 < t[4]) | (x1 < t[0]);
     const bool bad2 = (x2 < t[1]) | (x2 < t[2]) | (x2 < t[3]) | (x2 < t[4]) | (x2 < t[0]);
     if (bad1) return bad2 ? kAcausal : x2;
@@ -149,11 +145,11 @@ FD_INLINE double st_2d(const double t[8], const double s[8], double d01, double 
     return (x2 < x1) ? x2 : x1;
 }
 
-// New-face stencil: explicit step from the inner layer.
-FD_INLINE double st_nf(const double t[8], const double s[8], double d02, double d12, double d25) {
+// New-face stencil (fdsphnf, sphfd.c:6146-6184): explicit step from the inner layer.
+FD_INLINE double st_nf(const double t[8], const double s[8], Dist d02, Dist d12, double d25) {
     const double slo = .5 * (s[5] + s[2]);
-    const double t4t0 = (t[4] - t[0]) / d02;
-    const double t1t3 = (t[1] - t[3]) / d12;
+    const double t4t0 = xdiv(t[4] - t[0], d02.d, d02.y);
+    const double t1t3 = xdiv(t[1] - t[3], d12.d, d12.y);
     double x = slo * slo - 0.25 * (t4t0 * t4t0 + t1t3 * t1t3);
     if (x >= 0.) {
         x = t[2] + d25 * sqrt(x);
@@ -499,15 +495,165 @@ struct FaceSpec<SOUTH> {
     };
 };
 
+// --------------------------------------------------------- geometry tables
+
+// fdsph3d (sphfd.c:5944-5978) at node (j, k) of a cell corner: point 7 is the
+// node, points 0..6 sit at offsets (dx, dy, dz) in the cell, and the signs
+// g7*gi, n7*ni, m7*mi are +1 exactly when the point shares the node's layer,
+// row or column (gi*gj, ni*nj, mi*mj likewise for two points).  Every
+// coefficient of the quadratic is then
+//     +-1/h^2 + (+-1/dq^2 + X) / R
+// with X a reciprocal that depends on rows only and R = r_a * r_b a product
+// of two layer radii.  X, R and RN(1/R) come from per-table row and layer
+// tables, the division by R from xdiv, so each coefficient costs an add and
+// one fast division and equals the reference's bit for bit.  Values that
+// depend on both row and layer (the reciprocal grid distances of fdsph2d,
+// fdsphne and fdsphnf and the `a` terms of the quadratics) come from a cell
+// table.  Every entry is computed with the reference's operations in the
+// reference's order.
+//
+// Row entry j (j = -1..ny), with sq(j) = sin(qc(j)), dfsq = df*df:
+//   kRowIS2        1 / ((df sq(j)) (df sq(j)))               (D terms)
+//   kRowZ + dy+1   1 / ((df (df sq(j))) sq(j+dy)), dy=-1..1  (b terms)
+//   kRowX0         1 / ((dfsq sq(j)) sq(j))                  (pairs, same row)
+//   kRowXp         1 / ((dfsq sq(j)) sq(j+1))                (pairs, rows j, j+1)
+//   kRowXq         1 / ((dfsq sq(j+1)) sq(j))                (pairs, rows j+1, j)
+// Layer entry k (k = -1..nz), r(k) = z0r - h k:
+//   kLayR00, kLayY00   r(k) r(k) and its reciprocal
+//   kLayR01, kLayY01   r(k) r(k+1) and its reciprocal
+//   kLayYQ             RN(1 / (r(k) dq))
+// Cell entry (j, k):
+//   kCellRS + (dz+1)*3 + (dy+1)   RN(1 / S(j+dy, k+dz)), S(j, k) = (r(k) sq(j)) df
+//   kCellA + p, kCellY + p        a = 1/d^2 + 1/e^2 of fdsph2d/fdsphne for the
+//                                 node's distance pairs p = {S,H}, {Q,H}, {S,Q}
+//                                 (H = h, Q = r(k) dq) and RN(1 / (2 a))
+//   kCellY3                       RN(1 / (2 a)) of fdsph3d
+constexpr int kRowIS2 = 0, kRowZ = 1, kRowX0 = 4, kRowXp = 5, kRowXq = 6, kRowStride = 7;
+constexpr int kLayR00 = 0, kLayY00 = 1, kLayR01 = 2, kLayY01 = 3, kLayYQ = 4, kLayStride = 5;
+constexpr int kCellRS = 0, kCellA = 9, kCellY = 12, kCellY3 = 15, kCellStride = 16;
+
+struct GeoParams {
+    int ny, nz;
+    double h, dq, df, z0r;
+    const double *sinq;   // sin(qc(j)) for j = -1..ny, at [j + 1]
+};
+
+// Buffer layout: rows | layers | cells.
+FD_INLINE long long geo_rows_doubles(int ny) { return (long long)(ny + 2) * kRowStride; }
+FD_INLINE long long geo_layers_doubles(int nz) { return (long long)(nz + 2) * kLayStride; }
+FD_INLINE long long geo_doubles(int ny, int nz) {
+    return geo_rows_doubles(ny) + geo_layers_doubles(nz) + (long long)(ny + 2) * (nz + 2) * kCellStride;
+}
+// Work items of a fill: one per row, layer and cell entry.
+FD_INLINE long long geo_items(int ny, int nz) { return (long long)(ny + 2) + (nz + 2) + (long long)(ny + 2) * (nz + 2); }
+
+// Fills work item `it` of the geometry buffer `out`.
+FD_INLINE void geo_fill_item(const GeoParams &G, long long it, double *out) {
+    auto rcz = [&](int kk) { return G.z0r - G.h * (kk); };
+    auto sq = [&](int jj) { return G.sinq[jj + 1]; };
+    const double hsq = G.h * G.h, dqsq = G.dq * G.dq, dfsq = G.df * G.df;
+    const long long nr = G.ny + 2, nl = G.nz + 2;
+    if (it < nr) {
+        const int j = (int)it - 1;
+        double *E = out + it * kRowStride;
+        for (int q = 0; q < kRowStride; q++) E[q] = 0.0;
+        const double dfsinq = G.df * sq(j);
+        E[kRowIS2] = 1. / (dfsinq * dfsinq);
+        E[kRowX0] = 1. / (dfsq * sq(j) * sq(j));
+        if (j + 1 <= G.ny) {
+            E[kRowXp] = 1. / (dfsq * sq(j) * sq(j + 1));
+            E[kRowXq] = 1. / (dfsq * sq(j + 1) * sq(j));
+        }
+        if (j >= 0 && j < G.ny) {
+            const double df2sinq = G.df * dfsinq;
+            for (int dy = -1; dy <= 1; dy++) E[kRowZ + dy + 1] = 1. / (df2sinq * sq(j + dy));
+        }
+        return;
+    }
+    it -= nr;
+    if (it < nl) {
+        const int k = (int)it - 1;
+        double *E = out + nr * kRowStride + it * kLayStride;
+        for (int q = 0; q < kLayStride; q++) E[q] = 0.0;
+        E[kLayR00] = rcz(k) * rcz(k);
+        E[kLayY00] = 1.0 / E[kLayR00];
+        E[kLayR01] = rcz(k) * rcz(k + 1);
+        E[kLayY01] = 1.0 / E[kLayR01];
+        E[kLayYQ] = 1.0 / (rcz(k) * G.dq);
+        return;
+    }
+    it -= nl;
+    const int j = (int)(it % nr) - 1, k = (int)(it / nr) - 1;
+    double *E = out + nr * kRowStride + nl * kLayStride + it * kCellStride;
+    for (int q = 0; q < kCellStride; q++) E[q] = 0.0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const int jj = j + dy, kk = k + dz;
+            if (jj < -1 || jj > G.ny || kk < -1 || kk > G.nz) continue;
+            E[kCellRS + (dz + 1) * 3 + (dy + 1)] = 1.0 / (rcz(kk) * sq(jj) * G.df);
+        }
+    if (j < 0 || j >= G.ny || k < 0 || k >= G.nz) return;
+    const double S = rcz(k) * sq(j) * G.df, Q = rcz(k) * G.dq, H = G.h;
+    const double aSH = 1.0 / (S * S) + 1.0 / (H * H);
+    const double aQH = 1.0 / (Q * Q) + 1.0 / (H * H);
+    const double aSQ = 1.0 / (S * S) + 1.0 / (Q * Q);
+    E[kCellA + 0] = aSH;
+    E[kCellA + 1] = aQH;
+    E[kCellA + 2] = aSQ;
+    E[kCellY + 0] = 1.0 / (2 * aSH);
+    E[kCellY + 1] = 1.0 / (2 * aQH);
+    E[kCellY + 2] = 1.0 / (2 * aSQ);
+    const double dfsinq7 = G.df * sq(j), r7 = rcz(k);
+    const double a3 = 1.0 / hsq + (1.0 / dqsq + 1. / (dfsinq7 * dfsinq7)) / (r7 * r7);
+    E[kCellY3] = 1.0 / (2.0 * a3);
+}
+
+// index of the distance pair {f, g} of the node (forms DH, DRDQ, DRSDF)
+constexpr int geo_pair(int f, int g) {
+    return ((f == DRSDF && g == DH) || (f == DH && g == DRSDF)) ? 0
+           : ((f == DRDQ && g == DH) || (f == DH && g == DRDQ)) ? 1
+                                                                : 2;
+}
+
 // --------------------------------------------------------- node evaluation
 
 // Per-table geometry shared by all nodes (sphfd.c:151-156, 437-455).
 struct Geo {
     double h, dq, df, y0, z0r;
-    const double *sinq;   // sin(qc(j)) for j = -1..ny, stored at [j + 1]
+    double rh, ihsq, idqsq;   // RN(1/h), 1/(h h), 1/(dq dq)
+    const double *sinq;       // sin(qc(j)) for j = -1..ny, stored at [j + 1]
+    const double *rowt;       // geometry row table (entry j at [(j + 1) * kRowStride])
+    const double *layt;       // geometry layer table (entry k at [(k + 1) * kLayStride])
+    const double *cellt;      // geometry cell table
     int nx, ny, nz;
-    int box[3][2];        // current box sides x1 x2 / y1 y2 / z1 z2
+    int box[3][2];            // current box sides x1 x2 / y1 y2 / z1 z2
 };
+
+FD_INLINE const double *row_at(const Geo &G, int j) { return G.rowt + (j + 1) * kRowStride; }
+FD_INLINE const double *lay_at(const Geo &G, int k) { return G.layt + (k + 1) * kLayStride; }
+FD_INLINE const double *cell_at(const Geo &G, int j, int k) {
+    return G.cellt + ((long long)(k + 1) * (G.ny + 2) + (j + 1)) * kCellStride;
+}
+// r(k+a) r(k+b) and its reciprocal (|a - b| <= 1); the product is commutative.
+template <int A, int B>
+FD_INLINE void layer_pair(const Geo &G, int k, double &R, double &Y) {
+    if constexpr (A == B) {
+        const double *L = lay_at(G, k + A);
+        R = L[kLayR00];
+        Y = L[kLayY00];
+    } else {
+        const double *L = lay_at(G, k + (A < B ? A : B));
+        R = L[kLayR01];
+        Y = L[kLayY01];
+    }
+}
+// 1 / ((dfsq sq(j+a)) sq(j+b)) (|a - b| <= 1)
+template <int A, int B>
+FD_INLINE double row_pair(const Geo &G, int j) {
+    if constexpr (A == B) return row_at(G, j + A)[kRowX0];
+    else if constexpr (B == A + 1) return row_at(G, j + A)[kRowXp];
+    else return row_at(G, j + B)[kRowXq];
+}
 
 FD_INLINE double rc(const Geo &G, int k) { return G.z0r - G.h * (k); }
 
@@ -529,14 +675,6 @@ constexpr int nb_index(int dx, int dy, int dz) {
     const int o[3] = {dx, dy, dz};
     const int dn = o[S::nax], da = o[S::a_ax], db = o[S::b_ax];
     return (dn == 0 ? 0 : 9) + (db + 1) * 3 + (da + 1);
-}
-
-template <int I, int N, class Fn>
-FD_INLINE void static_for(Fn &&fn) {
-    if constexpr (I < N) {
-        fn(std::integral_constant<int, I>{});
-        static_for<I + 1, N>(fn);
-    }
 }
 
 // Result of one node's trial sequence.
@@ -592,9 +730,79 @@ FD_INLINE double sinq_of(const Nbhd &N, int dy) { return N.sq[dy + 1]; }
 FD_INLINE double dist_of(const Geo &G, const Nbhd &N, DS d, int k) {
     return d.form == DH ? G.h : d.form == DRDQ ? rc(G, k + d.dz) * G.dq : rc(G, k + d.dz) * sinq_of(N, d.dy) * G.df;
 }
+// A distance and its reciprocal; ce is the node's cell entry.
+FD_INLINE Dist dist_y(const Geo &G, const Nbhd &N, const double *ce, DS d, int k) {
+    Dist r;
+    r.d = dist_of(G, N, d, k);
+    r.y = d.form == DH ? G.rh : d.form == DRDQ ? lay_at(G, k + d.dz)[kLayYQ] : ce[kCellRS + (d.dz + 1) * 3 + (d.dy + 1)];
+    return r;
+}
 FD_INLINE double factor_of(const Geo &G, const Nbhd &N, DS f, int k) {
     return f.form == XH ? G.h : f.form == XR ? rc(G, k + f.dz) : f.form == XDQ ? G.dq : f.form == XDF ? G.df
                                                                                              : sinq_of(N, f.dy);
+}
+
+// fdsph3d of trial TI of face F at node (j, k) (sphfd.c:5935-6010), every
+// geometric sub-expression from the geometry tables; ce is the cell entry.
+template <int F, int TI>
+FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j, int k, const double *ce) {
+    constexpr Trial T = FaceSpec<F>::trials[TI];
+    double slo = 0.;
+#pragma unroll
+    for (int i = 0; i < 8; i++) slo += s[i];
+    slo = .125 * slo;
+    // b: (g7 gi)/h^2 + ((n7 ni)/dq^2 + (m7 mi)/(df (df sq7) sqi)) / (r7 ri)
+    double b = 0;
+    static_for<0, 7>([&](auto I) {
+        constexpr P3 o = T.pt[I];
+        double R, Y;
+        layer_pair<0, o.dz>(G, k, R, Y);
+        const double z = row_at(G, j)[kRowZ + o.dy + 1];
+        const double w = (o.dy == 0 ? G.idqsq : -G.idqsq) + (o.dx == 0 ? z : -z);
+        b += t[I] * ((o.dz == 0 ? G.ihsq : -G.ihsq) + xdiv(w, R, Y));
+    });
+    b = 2.0 * b;
+    // c: t_i^2 (1/h^2 + (1/dq^2 + 1/(df sqi)^2) / ri^2) ...
+    double c = 0.;
+    static_for<0, 7>([&](auto I) {
+        constexpr P3 o = T.pt[I];
+        const double *L = lay_at(G, k + o.dz);
+        const double w = G.idqsq + row_at(G, j + o.dy)[kRowIS2];
+        c += t[I] * t[I] * (G.ihsq + xdiv(w, L[kLayR00], L[kLayY00]));
+    });
+    // ... + 2 t_i sum_j t_j ((gi gj)/h^2 + ((ni nj)/dq^2 + (mi mj)/((df^2 sqi) sqj)) / (ri rj))
+    static_for<0, 6>([&](auto I) {
+        constexpr P3 u = T.pt[I];
+        double c2 = 0.;
+        static_for<decltype(I)::value + 1, 7>([&](auto J) {
+            constexpr P3 v = T.pt[J];
+            double R, Y;
+            layer_pair<u.dz, v.dz>(G, k, R, Y);
+            const double x = row_pair<u.dy, v.dy>(G, j);
+            const double w = (u.dy == v.dy ? G.idqsq : -G.idqsq) + (u.dx == v.dx ? x : -x);
+            c2 += t[J] * ((u.dz == v.dz ? G.ihsq : -G.ihsq) + xdiv(w, R, Y));
+        });
+        c += 2.0 * c2 * t[I];
+    });
+    // a: the node's own diagonal term
+    const double *L7 = lay_at(G, k);
+    const double a = G.ihsq + xdiv(G.idqsq + row_at(G, j)[kRowIS2], L7[kLayR00], L7[kLayY00]);
+    c = c - 16.0 * slo * slo;
+    const double rad = b * b - 4.0 * a * c;
+    if (!(rad >= 0.)) return kAcausal;
+    const double sr = sqrt(rad);
+    const double a2 = 2.0 * a, ya = ce[kCellY3];
+    const double x1 = xdiv(-b + sr, a2, ya);
+    const double x2 = xdiv(-b - sr, a2, ya);
+    bool bad1 = false, bad2 = false;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        bad1 |= (x1 < t[i]);
+        bad2 |= (x2 < t[i]);
+    }
+    if (bad1) return bad2 ? kAcausal : x2;
+    if (bad2) return x1;
+    return (x1 < x2) ? x1 : x2;
 }
 
 // Whether the node will run a 2D trial on carried t[0]/t[4]: it depends only
@@ -623,6 +831,7 @@ template <int F>
 FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float told, float tin0, float tin4,
                      NodeOut &out) {
     const float slow = N.s[nb_index<F>(0, 0, 0)];
+    const double *ce = cell_at(G, j, k);
     double t0 = tin0, t4 = tin4;   // the carried registers
     bool own04 = false, self = false, used_in = false;
     double guess = told;
@@ -639,27 +848,14 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
         }
         if (!trial_open<F, TI>(G, N, i, j, k)) return;
         if constexpr (T.kind == K3D) {
-            double t[8], s[8], r[8], sq[8];
-            int g[8], n[8], m[8];
-            constexpr int zmax = (T.pt[0].dz > 0 || T.pt[1].dz > 0 || T.pt[2].dz > 0 || T.pt[3].dz > 0 ||
-                                  T.pt[4].dz > 0 || T.pt[5].dz > 0 || T.pt[6].dz > 0) ? 1 : 0;
-            constexpr int ymax = (T.pt[0].dy > 0 || T.pt[1].dy > 0 || T.pt[2].dy > 0 || T.pt[3].dy > 0 ||
-                                  T.pt[4].dy > 0 || T.pt[5].dy > 0 || T.pt[6].dy > 0) ? 1 : 0;
-            constexpr int xmax = (T.pt[0].dx > 0 || T.pt[1].dx > 0 || T.pt[2].dx > 0 || T.pt[3].dx > 0 ||
-                                  T.pt[4].dx > 0 || T.pt[5].dx > 0 || T.pt[6].dx > 0) ? 1 : 0;
+            double t[7], s[8];
             static_for<0, 8>([&](auto P) {
                 constexpr P3 o = P < 7 ? T.pt[P] : P3{0, 0, 0};
                 constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
                 if constexpr (P < 7) t[P] = N.t[q];
                 s[P] = N.s[q];
-                r[P] = rc(G, k + o.dz);
-                sq[P] = sinq_of(N, o.dy);
-                // +1 on the cell's smaller-z layer, larger-y row, larger-x column
-                g[P] = (o.dz == zmax) ? -1 : 1;
-                n[P] = (o.dy == ymax) ? 1 : -1;
-                m[P] = (o.dx == xmax) ? 1 : -1;
             });
-            const double tr = st_3d(t, s, r, sq, g, n, m, G.h, G.dq, G.df);
+            const double tr = st_3d<F, TI>(t, s, G, j, k, ce);
             t0 = t[0];
             t4 = t[4];
             own04 = true;
@@ -673,8 +869,11 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
                 s[P] = N.s[q];
             });
             s[5] = slow;
-            const double tr = st_ne(t, s, dist_of(G, N, T.d[0], k), dist_of(G, N, T.d[1], k), dist_of(G, N, T.d[2], k),
-                                    dist_of(G, N, T.d[3], k), dist_of(G, N, T.d[4], k));
+            static_assert(T.d[3].dz == 0 && T.d[3].dy == 0 && T.d[4].dz == 0 && T.d[4].dy == 0, "node distances");
+            constexpr int pa = geo_pair(T.d[3].form, T.d[4].form);
+            const double tr = st_ne(t, s, dist_y(G, N, ce, T.d[0], k), dist_y(G, N, ce, T.d[1], k),
+                                    dist_y(G, N, ce, T.d[2], k), dist_y(G, N, ce, T.d[3], k),
+                                    dist_y(G, N, ce, T.d[4], k), ce[kCellA + pa], ce[kCellY + pa]);
             t0 = t[0];
             t4 = t[4];
             own04 = true;
@@ -691,8 +890,12 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
                 s[P + 1] = N.s[q];
             });
             if (!own04) used_in = true;
-            const double d01 = dist_of(G, N, T.d[0], k);
-            const double tr = st_2d(t, s, d01, dist_of(G, N, T.d[1], k), dist_of(G, N, T.d[2], k), dist_of(G, N, T.d[3], k));
+            static_assert(T.d[0].dz == 0 && T.d[0].dy == 0 && T.d[1].dz == 0 && T.d[1].dy == 0, "node distances");
+            constexpr int pa = geo_pair(T.d[0].form, T.d[1].form);
+            const Dist D01 = dist_y(G, N, ce, T.d[0], k);
+            const double d01 = D01.d;
+            const double tr = st_2d(t, s, D01, dist_y(G, N, ce, T.d[1], k), dist_y(G, N, ce, T.d[2], k),
+                                    dist_y(G, N, ce, T.d[3], k), ce[kCellA + pa], ce[kCellY + pa]);
             if (tr < guess) {
                 if constexpr (T.fhead) fhead = (guess - tr) / (d01 * (double)slow);
                 guess = tr;
@@ -705,7 +908,7 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
                 s[P] = N.s[q];
             });
             s[5] = slow;
-            const double tr = st_nf(t, s, dist_of(G, N, T.d[0], k), dist_of(G, N, T.d[1], k), dist_of(G, N, T.d[2], k));
+            const double tr = st_nf(t, s, dist_y(G, N, ce, T.d[0], k), dist_y(G, N, ce, T.d[1], k), dist_of(G, N, T.d[2], k));
             t0 = t[0];
             t4 = t[4];
             own04 = true;

@@ -120,6 +120,12 @@ __global__ void __launch_bounds__(kWideThreads, 2)
     }
 }
 
+// Geometry tables of one grid (fdsweep::geo_fill_item per row, layer and cell).
+__global__ void geo_kernel(fdsweep::GeoParams p, double *out, long long n) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        fdsweep::geo_fill_item(p, e, out);
+}
+
 __global__ void reset_kernel(int *q) {
     if (threadIdx.x == 0) q[0] = 0;
 }
@@ -139,6 +145,13 @@ extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jo
     else
         hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords>), dim3(nslots), dim3(kWideThreads), 0,
                            stream, d_jobs, njobs, d_slots, d_queue);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fd_launch_geo(const fdsweep::GeoParams *p, double *d_out, hipStream_t stream) {
+    const long long n = fdsweep::geo_items(p->ny, p->nz);
+    const int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(geo_kernel, dim3(blocks), dim3(256), 0, stream, *p, d_out, n);
     return hipGetLastError();
 }
 

@@ -69,6 +69,8 @@ struct TableJob {
     int skip_ok;               // skipping unchanged reverse-pass nodes is exact here
     int change_budget;         // CHANGE evaluations per face node before a face goes to
                                // LEVEL (0: never); a schedule choice, results are identical
+    const double *geo;         // geometry tables of this grid (geo_doubles(ny, nz)), or
+    double *geo_fill;          // if set, a buffer the workgroup fills first and then uses
 };
 
 enum {
@@ -100,12 +102,18 @@ struct Scratch {
 
 // LDS scratch of a workgroup: radix histograms during a sort, then the LEVEL
 // schedule's byte-packed pending counters and successor masks (faces up to
-// about 70K nodes; larger faces use global scratch).
-constexpr int kArenaWords = 36000;
+// about 94K nodes; larger faces use global scratch).  The geometry row and
+// layer tables of the table being solved sit in LDS beside it when they fit
+// (7 (ny + 2) + 5 (nz + 2) <= kGeoLdsDoubles, e.g. 321 x 161 grids); else
+// they are read from HBM.
+constexpr int kArenaWords = 29600;
+constexpr int kGeoLdsDoubles = 3200;
 
-template <int AW>
+template <int AW, int GL = kGeoLdsDoubles>
 struct SharedT {
     static constexpr int kArena = AW;
+    static constexpr int kGeoLds = GL;
+    double geol[GL];     // geometry row and layer tables
     int n[3];            // list sizes
     int processed;
     int headw[7];
@@ -708,6 +716,18 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
         for (int q = tid; q < J.ninit; q += T)
             J.time[J.init_idx[q]] = J.init_mode == 0 ? (float)((double)s000 * J.init_dist[q]) : J.init_val[q];
     }
+    if (J.geo_fill) {
+        const GeoParams gp{J.ny, J.nz, J.h, J.dq, J.df, J.z0r, J.sinq};
+        const long long ne = geo_items(J.ny, J.nz);
+        for (long long e = tid; e < ne; e += T) geo_fill_item(gp, e, J.geo_fill);
+        x.sync();
+    }
+    // the row and layer tables into LDS when they fit
+    const double *gbuf = J.geo_fill ? J.geo_fill : J.geo;
+    const long long nrl = geo_rows_doubles(J.ny) + geo_layers_doubles(J.nz);
+    const bool geo_lds = nrl <= SH::kGeoLds;
+    if (geo_lds)
+        for (long long q = tid; q < nrl; q += T) sh.geol[q] = gbuf[q];
     if (tid == 0) {
         for (int s = 0; s < kStatCount; s++) sh.st[s] = 0;
         sh.cin0 = 0.0f;
@@ -718,6 +738,12 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
     if (tid == 0) sh.st[kStatTimeInit] = 0;
     Geo G;
     G.h = J.h; G.dq = J.dq; G.df = J.df; G.y0 = J.y0; G.z0r = J.z0r; G.sinq = J.sinq;
+    G.rh = 1.0 / J.h;
+    G.ihsq = 1. / (J.h * J.h);
+    G.idqsq = 1. / (J.dq * J.dq);
+    G.rowt = geo_lds ? sh.geol : gbuf;
+    G.layt = G.rowt + geo_rows_doubles(J.ny);
+    G.cellt = gbuf + nrl;
     G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;
     int x1 = J.box[0][0], x2 = J.box[0][1], y1 = J.box[1][0], y2 = J.box[1][1], z1 = J.box[2][0], z2 = J.box[2][1];
     int dx1 = J.dflag[0], dx2 = J.dflag[1], dy1 = J.dflag[2], dy2 = J.dflag[3], dz1 = J.dflag[4], dz2 = J.dflag[5];

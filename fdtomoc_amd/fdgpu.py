@@ -98,6 +98,7 @@ def lib():
         L.fdgpu_table_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_table_device_ptr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.fdgpu_run_spec.argtypes = [C.c_char_p]
+        L.fdgpu_serve_request_bytes.argtypes = [C.POINTER(CPar), C.c_int, C.POINTER(C.c_ulonglong)]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -131,6 +132,14 @@ def make_par(**kw) -> CPar:
             raise KeyError(k)
         setattr(p, k, v)
     return p
+
+
+def serve_request_bytes(own_velocity: bool = False, **par) -> int:
+    """HBM arena bytes one serving request of these sizes takes."""
+    b = C.c_ulonglong()
+    _check(lib().fdgpu_serve_request_bytes(C.byref(make_par(**par)), int(own_velocity), C.byref(b)),
+           "fdgpu_serve_request_bytes")
+    return int(b.value)
 
 
 class Context:

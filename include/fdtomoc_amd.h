@@ -162,6 +162,9 @@ int fdgpu_serve_read(fdgpu_ctx *ctx, int request, float *host, unsigned long lon
 int fdgpu_serve_stats(fdgpu_ctx *ctx, int request, int *stats, int nstats);
 /* Writes a finished request's table as fdgpu_table_write does. */
 int fdgpu_serve_write(fdgpu_ctx *ctx, int request, const char *path, const void *vel_header);
+/* HBM arena bytes one request with these sizes takes (own_velocity: it
+ * carries its own velocities), for sizing cfg->arena_bytes. */
+int fdgpu_serve_request_bytes(const fdgpu_par *par, int own_velocity, unsigned long long *bytes);
 /* Returns a finished request's arena block. */
 int fdgpu_serve_release(fdgpu_ctx *ctx, int request);
 /* info[0..5]: serving on, requests outstanding, arena bytes used, arena bytes,

@@ -101,6 +101,8 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     }
     J.skip_ok = fdprep::skip_safe(p, pr, 1.0 / smax, 1.0 / smin) ? 1 : 0;
     J.change_budget = change_budget;
+    std::vector<double> geo((size_t)fdsweep::geo_doubles(p.ny, p.nz));
+    J.geo_fill = geo.data();
     std::unique_ptr<fdsweep::Shared> shp(new fdsweep::Shared());
     fdsweep::Shared &sh = *shp;
     std::memset(&sh, 0, sizeof sh);

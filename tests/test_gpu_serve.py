@@ -102,11 +102,13 @@ def test_admission_by_arena_and_busy_calls(ctx):
     m = ctx.model(V)
     par, origin, sub = _window_request(rng, V, 24, 25)
     n = par.nx * par.ny * par.nz
-    # one request's arena block: time grid, sin table, counters, stamps, last
-    # full-plane faces (256-byte aligned parts); room for two and a half
+    # one request's arena block (time grid, sin table, counters, stamps, last
+    # full-plane faces, geometry table): at least the time grid and stamps;
+    # room for two and a half
     a = lambda b: (b + 255) // 256 * 256
-    per = a(4 * n) + a(8 * (par.ny + 2)) + a(4 * len(fdgpu.STAT_NAMES)) + a(4 * n) + \
-        a(24 * max(par.nx, par.ny, par.nz))
+    per = fdgpu.serve_request_bytes(nx=par.nx, ny=par.ny, nz=par.nz)
+    assert per >= a(4 * n) + a(4 * n)
+    assert fdgpu.serve_request_bytes(True, nx=par.nx, ny=par.ny, nz=par.nz) >= per + 4 * n
     ctx.serve_start((56, 48, 28), arena_bytes=a(int(2.5 * per)))
     try:
         spec = fdgpu.TableSpec(par_dict(par), m, origin)

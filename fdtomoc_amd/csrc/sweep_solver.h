@@ -80,6 +80,11 @@ enum {
     kStatSkips = kStatHeadw0 + 7,  // CHANGE first-round evaluations skipped (inputs unchanged)
     // time spent by the table, in X::clock() ticks (10 ns on the GPU)
     kStatTimeGather, kStatTimeSort, kStatTimeLevel, kStatTimeChange, kStatTimeInit,
+    // LEVEL rounds, as seen by lane 0 (which always holds the first ready node):
+    // ticks from the round's start to its evaluation, ticks of the evaluation
+    kStatTimeLevelPre, kStatTimeLevelEval,
+    kStatWalks,         // LEVEL: rank-order steps walked back to find carried t[0]/t[4]
+    kStatTimeChange1,   // CHANGE: ticks of the first round (every node of the face)
     kStatCount
 };
 
@@ -363,7 +368,8 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
         if (tid == 0) sh.n[lz] = 0;
         const int *cur = lists[lc];
         int *nxt = lists[ln];
-        int ndone = 0, ndefer = 0;
+        int ndone = 0, ndefer = 0, nwalk = 0;
+        const long long tr0 = tid == 0 ? x.clock() : 0;
         for (int it = tid; it < ncur; it += T) {
             const int p = cur[it];
             int i, j, k, a, b;
@@ -381,6 +387,7 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
                 bool open = false;
                 while (r >= 0) {
                     const int v = W.stat[r];
+                    nwalk++;
                     if (v == 0 || (v >> 2) > R) { open = true; break; }
                     if ((v & 3) == 1) break;
                     r--;
@@ -396,7 +403,12 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
                 }
             }
             NodeOut o;
+            const long long te0 = it == 0 ? x.clock() : 0;
             eval_node<F>(N, G, i, j, k, told, tin0, tin4, o);
+            if (it == 0) {
+                sh.st[kStatTimeLevelPre] += (int)(te0 - tr0);
+                sh.st[kStatTimeLevelEval] += (int)(x.clock() - te0);
+            }
             if (o.stored) {
                 J.time[node] = o.tnew;
                 J.mod_stamp[node] = fcur;
@@ -421,6 +433,7 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
             x.atomic_add(&sh.st[kStatEvals], ndone);
         }
         if (ndefer) x.atomic_add(&sh.st[kStatDefers], ndefer);
+        if (nwalk) x.atomic_add(&sh.st[kStatWalks], nwalk);
         if (tid == 0) sh.st[kStatRoundsLevel] += 1;
         x.sync();
     }
@@ -452,6 +465,7 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
         for (int p = tid; p < Fn; p += T) W.stamp[p] = 0;
     x.sync();
     long long spent = 0;
+    const long long tc0 = x.clock();
     for (int round = 1;; round++) {
         const int nwork = (round == 1) ? Fn : sh.n[0];
         if (nwork == 0) break;
@@ -509,6 +523,10 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
         if (tid == 0) {
             sh.st[kStatRoundsChange] += 1;
             sh.st[kStatEvals] += nwork;
+        }
+        if (round == 1) {
+            x.sync();
+            if (tid == 0) sh.st[kStatTimeChange1] += (int)(x.clock() - tc0);
         }
         x.sync();
         if (tid == 0) sh.n[0] = 0;

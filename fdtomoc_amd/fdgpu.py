@@ -18,7 +18,7 @@ EXE_PATH = os.path.join(PKG, "bin", "sphfd")
 
 STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",
               "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
-    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init"]
+    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1"]
 
 BATCHED, PLAIN = 0, 1
 

@@ -13,7 +13,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EMU_SO = os.path.join(REPO, "tests", "native", "libemu_solver.so")
 STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",
               "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
-    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init"]
+    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1"]
 
 _lib = None
 

@@ -2,8 +2,8 @@
 
     python -m fdtomoc_amd.build
 
-produces ``fdtomoc_amd/libfdtomoc_amd.so`` (gfx950 kernels + C ABI of
-include/fdtomoc_amd.h) and ``fdtomoc_amd/bin/sphfd`` (the drop-in
+produces ``fdtomoc_amd/libfdtomoc_amd.so`` (gfx950 kernels -- the sphfd sweep
+and c2f's coarse-to-fine interpolation -- + C ABI of include/fdtomoc_amd.h) and ``fdtomoc_amd/bin/sphfd`` (the drop-in
 executable).  All code is compiled without floating-point contraction: the
 FP64 stencils must round like the reference's x86-64 build.
 """
@@ -28,8 +28,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
           "-Wno-unused-function", "-Wno-unknown-pragmas"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = [("sweep_kernel.hip", True), ("fdgpu.cpp", False), ("sphfd_io.cpp", False),
-           ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False)]
+SOURCES = [("sweep_kernel.hip", True), ("coarse_kernel.hip", True), ("fdgpu.cpp", False), ("sphfd_io.cpp", False),
+           ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False), ("coarse.cpp", False)]
 
 
 def _run(cmd, verbose):

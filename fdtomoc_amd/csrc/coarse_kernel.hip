@@ -1,0 +1,143 @@
+// coarse_kernel.hip -- gfx950 kernels of the coarse-to-fine model
+// interpolation of FDtomoC's c2f (src/c2f.c:549-575, find_vel 673-727): the
+// fine velocity grids that sphfd consumes, computed on the device from an
+// inversion's coarse model and written straight into resident models.
+//
+//   c2f_prep_kernel  coarse velocities -> slowness (a double quotient rounded
+//                    to float, c2f.c:549-552); per fine coordinate of each
+//                    axis the coarse cell and the float coordinate
+//   c2f_fine_kernel  one fine node per lane, P and S: trilinear slowness in
+//                    the reference's float operation order, velocity 1/v,
+//                    the model's slowness 1/velocity (sphfd.c:689-690), and
+//                    the velocity range of each phase
+//
+// Compiled with -ffp-contract=off: every float multiply and add rounds
+// separately, as in the reference's x86-64 build.
+#include <hip/hip_runtime.h>
+
+#include "coarse_kernels.h"
+
+namespace {
+
+// The coarse cell holding v on axis g (c2f.c:675-683): the first node
+// strictly beyond v, clamped to the last interval, minus one.
+__device__ int coarse_cell(const float *g, int n, float v) {
+    int i;
+    for (i = 1; i < n; i++)
+        if (g[i] > v) break;
+    if (i == n) i--;
+    return i - 1;
+}
+
+__global__ void c2f_prep_kernel(const C2fGrid grid, float *coarse, long long ncoarse, C2fAxis *axis) {
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long q = t0; q < ncoarse; q += stride) coarse[q] = (float)(1. / (double)coarse[q]);
+    const long long nax = (long long)grid.nx + grid.ny + grid.nz;
+    for (long long q = t0; q < nax; q += stride) {
+        // axis 0: x (c2f.c:566), 1: y (c2f.c:563), 2: z (c2f.c:560)
+        const int a = q < grid.nx ? 0 : q < grid.nx + grid.ny ? 1 : 2;
+        const int i = (int)(a == 0 ? q : a == 1 ? q - grid.nx : q - grid.nx - grid.ny);
+        const double o = a == 0 ? grid.x0 : a == 1 ? grid.y0 : grid.z0;
+        const float *g = a == 0 ? grid.gx : a == 1 ? grid.gy : grid.gz;
+        const int nc = a == 0 ? grid.nxc : a == 1 ? grid.nyc : grid.nzc;
+        const float v = (float)(o + i * grid.h);
+        C2fAxis e;
+        e.cell = coarse_cell(g, nc, v);
+        e.v = v;
+        axis[q] = e;
+    }
+}
+
+// Block-wide min/max of positive finite velocities (as ordered bit patterns)
+// and a flag for any other value, folded into range[4 * phase + 0..2].
+__device__ void fold_range(unsigned lo, unsigned hi, unsigned bad, unsigned *range) {
+    __shared__ unsigned slo[64], shi[64], sbad[64];
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (unsigned)__shfl_xor((int)lo, off));
+        hi = max(hi, (unsigned)__shfl_xor((int)hi, off));
+        bad |= (unsigned)__shfl_xor((int)bad, off);
+    }
+    const int w = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
+    if ((threadIdx.x & 63) == 0) { slo[w] = lo; shi[w] = hi; sbad[w] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < nw; q++) { lo = min(lo, slo[q]); hi = max(hi, shi[q]); bad |= sbad[q]; }
+        atomicMin(&range[0], lo);
+        atomicMax(&range[1], hi);
+        if (bad) atomicOr(&range[2], 1u);
+    }
+    __syncthreads();
+}
+
+__global__ void c2f_fine_kernel(const C2fGrid grid, const float *__restrict__ sl, const C2fAxis *__restrict__ axis,
+                                float *__restrict__ vel_out, float *__restrict__ slow_p, float *__restrict__ slow_s,
+                                unsigned *range) {
+    const long long nrow = (long long)grid.ny * grid.nz, n = nrow * grid.nx;
+    const long long nxyzc = (long long)grid.nxc * grid.nyc * grid.nzc;
+    unsigned lo[2] = {0xffffffffu, 0xffffffffu}, hi[2] = {0u, 0u}, bad[2] = {0u, 0u};
+    // one fine row (j, k) per block row, x across the lanes: coalesced writes
+    for (long long r = blockIdx.y; r < nrow; r += gridDim.y) {
+        const int j = (int)(r % grid.ny), k = (int)(r / grid.ny);
+        const C2fAxis ey = axis[grid.nx + j], ez = axis[grid.nx + grid.ny + k];
+        const int cj = ey.cell, ck = ez.cell;
+        const float yj = grid.gy[cj], hy = grid.gy[cj + 1] - yj;
+        const float zk = grid.gz[ck], hz = grid.gz[ck + 1] - zk;
+        const int nk = grid.nxc * grid.nyc * ck;
+        const int nj = grid.nxc * cj;
+        const int nk2 = grid.nxc * grid.nyc * (ck + 1);
+        const int nj2 = grid.nxc * (cj + 1);
+        const float fy = (ey.v - yj) / hy;
+        const float fz = (ez.v - zk) / hz;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < grid.nx; i += gridDim.x * blockDim.x) {
+            const long long q = r * grid.nx + i;
+            const C2fAxis ex = axis[i];
+            const int ci = ex.cell;
+            const float xi = grid.gx[ci], hx = grid.gx[ci + 1] - xi;
+            const float fx = (ex.v - xi) / hx;
+#pragma unroll
+            for (int ph = 0; ph < 2; ph++) {
+                const float *s = sl + ph * nxyzc + ci;
+                float tv = 0.f;
+                tv = (1.f - fx) * (1.f - fy) * (1.f - fz) * s[nk + nj];
+                tv = tv + fx * (1.f - fy) * (1.f - fz) * s[nk + nj + 1];
+                tv = tv + (1.f - fx) * fy * (1.f - fz) * s[nk + nj2];
+                tv = tv + (1.f - fx) * (1.f - fy) * fz * s[nk2 + nj];
+                tv = tv + fx * fy * (1.f - fz) * s[nk + nj2 + 1];
+                tv = tv + fx * (1.f - fy) * fz * s[nk2 + nj + 1];
+                tv = tv + (1.f - fx) * fy * fz * s[nk2 + nj2];
+                tv = tv + fx * fy * fz * s[nk2 + nj2 + 1];
+                const float vel = 1.f / tv;                     // c2f.c:570
+                if (vel_out) vel_out[ph * n + q] = vel;
+                (ph == 0 ? slow_p : slow_s)[q] = 1.0f / vel;    // sphfd.c:689-690
+                unsigned u;
+                __builtin_memcpy(&u, &vel, 4);
+                if (vel > 0.f && vel < __builtin_inff()) {
+                    lo[ph] = min(lo[ph], u);
+                    hi[ph] = max(hi[ph], u);
+                } else {
+                    bad[ph] = 1u;
+                }
+            }
+        }
+    }
+    fold_range(lo[0], hi[0], bad[0], range);
+    fold_range(lo[1], hi[1], bad[1], range + 4);
+}
+
+}  // namespace
+
+extern "C" hipError_t fd_launch_c2f(const C2fGrid *grid, float *d_coarse, C2fAxis *d_axis, float *d_vel_out,
+                                    float *d_slow_p, float *d_slow_s, unsigned *d_range, hipStream_t stream) {
+    const long long ncoarse = 2LL * grid->nxc * grid->nyc * grid->nzc;
+    const long long nax = (long long)grid->nx + grid->ny + grid->nz;
+    const long long m = ncoarse > nax ? ncoarse : nax;
+    int blocks = (int)((m + 255) / 256);
+    hipLaunchKernelGGL(c2f_prep_kernel, dim3(blocks), dim3(256), 0, stream, *grid, d_coarse, ncoarse, d_axis);
+    const long long nrow = (long long)grid->ny * grid->nz;
+    const int bx = (grid->nx + 255) / 256;
+    const int by = (int)(nrow < 65535 ? nrow : 65535);
+    hipLaunchKernelGGL(c2f_fine_kernel, dim3(bx < 16 ? bx : 16, by), dim3(256), 0, stream, *grid,
+                       (const float *)d_coarse, (const C2fAxis *)d_axis, d_vel_out, d_slow_p, d_slow_s, d_range);
+    return hipGetLastError();
+}

@@ -51,6 +51,51 @@ class CTable(C.Structure):
                 ("preset", C.c_void_p), ("boxvals", C.c_void_p)]
 
 
+class CCoarse(C.Structure):
+    """fdgpu_coarse: a coarse model as c2f reads it (src/c2f.c)."""
+    _fields_ = [("nxc", C.c_int), ("nyc", C.c_int), ("nzc", C.c_int),
+                ("igridx", C.c_void_p), ("igridy", C.c_void_p), ("igridz", C.c_void_p),
+                ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double), ("h", C.c_double),
+                ("velocity", C.c_void_p)]
+
+
+@dataclasses.dataclass
+class Coarse:
+    """A coarse model: node counts, fine intervals per coarse interval, the
+    spec's origin and fine spacing, and 2*nxc*nyc*nzc velocities (P then S)."""
+    nxc: int
+    nyc: int
+    nzc: int
+    igridx: np.ndarray
+    igridy: np.ndarray
+    igridz: np.ndarray
+    x0: float
+    y0: float
+    z0: float
+    h: float
+    velocity: np.ndarray
+
+    def to_c(self, keep: list) -> CCoarse:
+        c = CCoarse()
+        c.nxc, c.nyc, c.nzc = self.nxc, self.nyc, self.nzc
+        for name in ("igridx", "igridy", "igridz"):
+            a = np.ascontiguousarray(getattr(self, name), dtype=np.int32)
+            keep.append(a)
+            setattr(c, name, a.ctypes.data)
+        v = np.ascontiguousarray(self.velocity, dtype=np.float32)
+        keep.append(v)
+        c.velocity = v.ctypes.data
+        c.x0, c.y0, c.z0, c.h = self.x0, self.y0, self.z0, self.h
+        return c
+
+    def dims(self) -> tuple:
+        keep = []
+        c = self.to_c(keep)
+        nx, ny, nz = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().fdgpu_coarse_dims(C.byref(c), C.byref(nx), C.byref(ny), C.byref(nz)), "fdgpu_coarse_dims")
+        return nx.value, ny.value, nz.value
+
+
 @dataclasses.dataclass
 class TableSpec:
     """One station table: par values plus where it sits in a device model."""
@@ -99,6 +144,12 @@ def lib():
         L.fdgpu_table_device_ptr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.fdgpu_run_spec.argtypes = [C.c_char_p]
         L.fdgpu_serve_request_bytes.argtypes = [C.POINTER(CPar), C.c_int, C.POINTER(C.c_ulonglong)]
+        L.fdgpu_coarse_dims.argtypes = [C.POINTER(CCoarse), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]
+        L.fdgpu_models_from_coarse.argtypes = [C.c_void_p, C.POINTER(CCoarse), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), C.c_void_p]
+        L.fdgpu_models_update_from_coarse.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(CCoarse),
+                                                      C.c_void_p]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -180,6 +231,29 @@ class Context:
 
     def release_model(self, model: int):
         _check(lib().fdgpu_model_release(self._h, model), "fdgpu_model_release")
+
+    def models_from_coarse(self, coarse: Coarse, want_fine: bool = False):
+        """(P model, S model[, fine velocities (2, nz, ny, nx)]) of a coarse
+        model, interpolated on the device as c2f does."""
+        keep = []
+        c = coarse.to_c(keep)
+        nx, ny, nz = coarse.dims()
+        fine = np.empty((2, nz, ny, nx), dtype=np.float32) if want_fine else None
+        mp, ms = C.c_int(), C.c_int()
+        _check(lib().fdgpu_models_from_coarse(self._h, C.byref(c), C.byref(mp), C.byref(ms),
+                                              None if fine is None else fine.ctypes.data),
+               "fdgpu_models_from_coarse")
+        return (mp.value, ms.value, fine) if want_fine else (mp.value, ms.value)
+
+    def update_models_from_coarse(self, model_p: int, model_s: int, coarse: Coarse, want_fine: bool = False):
+        keep = []
+        c = coarse.to_c(keep)
+        nx, ny, nz = coarse.dims()
+        fine = np.empty((2, nz, ny, nx), dtype=np.float32) if want_fine else None
+        _check(lib().fdgpu_models_update_from_coarse(self._h, model_p, model_s, C.byref(c),
+                                                     None if fine is None else fine.ctypes.data),
+               "fdgpu_models_update_from_coarse")
+        return fine
 
     # tables
     @staticmethod

@@ -74,6 +74,33 @@ int fdgpu_model_create(fdgpu_ctx *ctx, int nx, int ny, int nz, const float *velo
 int fdgpu_model_update(fdgpu_ctx *ctx, int model, const float *velocity);
 int fdgpu_model_release(fdgpu_ctx *ctx, int model);
 
+/* ------------------------------------------------------ coarse models (c2f)
+ *
+ * The fine velocity grids of an inversion's coarse model, computed on the
+ * device and stored as resident models: the interpolation of c2f
+ * (src/c2f.c:549-575, find_vel 673-727), which produces the P and S velocity
+ * files sphfd reads.  Results are bit-identical to the files c2f writes
+ * (payload after the 232-byte header: P then S, x fastest). */
+typedef struct {
+    int nxc, nyc, nzc;                    /* coarse nodes per axis, >= 2 each (c2f spec nxc nyc nzc) */
+    const int *igridx, *igridy, *igridz;  /* fine intervals per coarse interval: nxc-1, nyc-1, nzc-1
+                                             entries, each >= 1 (spec igridx/igridy/igridz) */
+    double x0, y0, z0, h;                 /* spec origin and fine spacing */
+    const float *velocity;                /* 2*nxc*nyc*nzc coarse velocities, P then S, x fastest
+                                             (the coarse model file's payload) */
+} fdgpu_coarse;
+
+/* Fine dimensions of a coarse model: 1 + the sum of igrid per axis (src/c2f.c:262-277). */
+int fdgpu_coarse_dims(const fdgpu_coarse *coarse, int *nx, int *ny, int *nz);
+/* Creates two models (P and S) from a coarse model, replacing c2f's run and
+ * the file round trip to sphfd (src/c2f.c main, run.sh:26-28).  fine == NULL
+ * or 2*nx*ny*nz floats: receives the fine velocities c2f would write. */
+int fdgpu_models_from_coarse(fdgpu_ctx *ctx, const fdgpu_coarse *coarse, int *model_p, int *model_s, float *fine);
+/* Replaces two existing models (of the coarse model's fine dimensions) with
+ * the interpolation of an updated coarse model (one inversion iteration). */
+int fdgpu_models_update_from_coarse(fdgpu_ctx *ctx, int model_p, int model_s, const fdgpu_coarse *coarse,
+                                    float *fine);
+
 /* One table request.  Table node (0,0,0) sits at model node (ox, oy, oz); the
  * table spans par.nx*par.ny*par.nz nodes inside the model. */
 typedef struct {
@@ -92,8 +119,8 @@ enum { FDGPU_BATCHED = 0, FDGPU_PLAIN = 1 };
  * order.  The tables stay resident; ids are written to table_ids[0..n). */
 int fdgpu_solve(fdgpu_ctx *ctx, const fdgpu_table_desc *tables, int n, int mode, int *table_ids);
 
-/* Device time of the last fdgpu_solve's kernel launches, in milliseconds,
- * measured with HIP events on the launch stream. */
+/* Device time of the last fdgpu_solve's (or coarse-model call's) kernel
+ * launches, in milliseconds, measured with HIP events on the launch stream. */
 double fdgpu_last_kernel_ms(fdgpu_ctx *ctx);
 
 /* Per-face schedule knob (no reference counterpart; results do not depend on

@@ -138,6 +138,11 @@ def lib():
         L.oracle_fdsph3d.restype = C.c_double
         L.oracle_glath.restype = C.c_double
         L.oracle_glath.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.oracle_c2f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                 C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.oracle_c2f_dims.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -205,3 +210,39 @@ def run_reference(par: Par, vel: np.ndarray, workdir: str | None = None,
         if own:
             import shutil
             shutil.rmtree(wd, ignore_errors=True)
+
+
+def c2f(nxc: int, nyc: int, nzc: int, igridx, igridy, igridz, x0: float, y0: float, z0: float, h: float,
+        vel: np.ndarray):
+    """The oracle's c2f interpolation (oracle/c2f_oracle.c): returns
+    ((nx, ny, nz), fine velocities P then S as a flat float32 array)."""
+    gx = np.ascontiguousarray(igridx, dtype=np.int32)
+    gy = np.ascontiguousarray(igridy, dtype=np.int32)
+    gz = np.ascontiguousarray(igridz, dtype=np.int32)
+    v = np.ascontiguousarray(vel, dtype=np.float32)
+    nx, ny, nz = C.c_int(), C.c_int(), C.c_int()
+    L = lib()
+    if L.oracle_c2f_dims(nxc, nyc, nzc, gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, C.byref(nx), C.byref(ny),
+                         C.byref(nz)) != 0:
+        raise ValueError("bad coarse grid")
+    fine = np.empty(2 * nx.value * ny.value * nz.value, dtype=np.float32)
+    if L.oracle_c2f(nxc, nyc, nzc, gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, x0, y0, z0, h, v.ctypes.data,
+                    fine.ctypes.data, C.byref(nx), C.byref(ny), C.byref(nz)) != 0:
+        raise ValueError("bad coarse grid")
+    return (nx.value, ny.value, nz.value), fine
+
+
+def c2f_golden_cases():
+    """The coarse models of tests/golden/c2f_cases.npz with the fine velocities
+    the compiled reference c2f wrote for them."""
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2f_cases.npz"))
+    out = []
+    for s in z["seeds"].tolist():
+        c = {k[len(f"s{s}_"):]: z[k] for k in z.files if k.startswith(f"s{s}_")}
+        for k in ("nxc", "nyc", "nzc"):
+            c[k] = int(c[k])
+        for k in ("x0", "y0", "z0", "h"):
+            c[k] = float(c[k])
+        c["seed"] = s
+        out.append(c)
+    return out

@@ -45,3 +45,20 @@ def test_defaults_and_error_strings_without_gpu():
     assert (p.y0, p.floatsrc, p.srctype, p.ncube, p.reverse, p.headpref, p.parse) == (90.0, 1, 1, 2, 1, 6, 1)
     assert p.maxoff == -1.0 and p.kminus == 1 and p.iplus == 1
     assert fdgpu.lib().fdgpu_strerror(-3).decode().startswith("source")
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_coarse_dims_and_validation_without_gpu():
+    """fdgpu_coarse_dims (host only): 1 + sum(igrid) per axis as c2f computes the
+    fine grid (src/c2f.c:262-277); bad coarse models are refused."""
+    import numpy as np
+    from fdtomoc_amd import fdgpu
+    vel = np.ones(2 * 4 * 3 * 2, dtype=np.float32)
+    c = fdgpu.Coarse(4, 3, 2, np.array([2, 3, 1]), np.array([4, 1]), np.array([5]), 0.0, 0.0, 0.0, 1.0, vel)
+    assert c.dims() == (7, 6, 6)
+    for bad in (dict(nzc=1, igridz=np.array([], dtype=np.int32)), dict(igridx=np.array([2, 0, 1])), dict(h=0.0)):
+        kw = dict(c.__dict__)
+        kw.update(bad)
+        with pytest.raises(fdgpu.FdgpuError) as e:
+            fdgpu.Coarse(**kw).dims()
+        assert e.value.code == -5

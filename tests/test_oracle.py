@@ -262,3 +262,43 @@ def test_linear_gradient_close_to_analytic():
     # a wrong gradient (a dropped term) is far outside that band
     wrong = workloads.gradient_time_flat(g, ob.geometry(par), 5.0, 0.0 + 1e-9)
     assert np.abs(t - wrong).max() > 0.3
+
+
+# ------------------------------------------------------------------ c2f
+
+def test_c2f_matches_reference_fixtures():
+    """oracle_c2f against the fine models the reference c2f (compiled from its
+    own sources) wrote for 16 seeded coarse models, headered and headerless:
+    bit for bit."""
+    cases = ob.c2f_golden_cases()
+    assert len(cases) == 16 and any(c["headered"] for c in cases) and not all(c["headered"] for c in cases)
+    for c in cases:
+        dims, fine = ob.c2f(c["nxc"], c["nyc"], c["nzc"], c["igridx"], c["igridy"], c["igridz"], c["x0"], c["y0"],
+                            c["z0"], c["h"], c["vel"])
+        assert dims == tuple(int(v) for v in c["dims"]), c["seed"]
+        diff = int((fine.view(np.uint32) != c["fine"].view(np.uint32)).sum())
+        assert diff == 0, (c["seed"], diff)
+
+
+def test_c2f_coarse_nodes_and_fine_dims():
+    """Closed forms: the fine grid has 1 + sum(igrid) nodes per axis, and a fine
+    node that falls exactly on a coarse node (integer coordinates, integer
+    spacing) takes that node's slowness unchanged, so its velocity is
+    1/(float)(1/v) rounded in float."""
+    rng = np.random.default_rng(17)
+    nxc, nyc, nzc = 5, 4, 3
+    igx, igy, igz = np.array([2, 3, 1, 4]), np.array([1, 2, 2]), np.array([3, 2])
+    vel = rng.uniform(3.0, 8.0, size=2 * nxc * nyc * nzc).astype(np.float32)
+    (nx, ny, nz), fine = ob.c2f(nxc, nyc, nzc, igx, igy, igz, 10.0, -4.0, 2.0, 1.0, vel)
+    assert (nx, ny, nz) == (1 + igx.sum(), 1 + igy.sum(), 1 + igz.sum())
+    fine = fine.reshape(2, nz, ny, nx)
+    cx = np.concatenate([[0], np.cumsum(igx)])
+    cy = np.concatenate([[0], np.cumsum(igy)])
+    cz = np.concatenate([[0], np.cumsum(igz)])
+    coarse = vel.reshape(2, nzc, nyc, nxc)
+    for ph in range(2):
+        for k, fk in enumerate(cz):
+            for j, fj in enumerate(cy):
+                for i, fi in enumerate(cx):
+                    s = np.float32(1.0 / np.float64(coarse[ph, k, j, i]))
+                    assert fine[ph, fk, fj, fi] == np.float32(1.0) / s, (ph, k, j, i)

@@ -25,6 +25,10 @@ Other workloads (one JSON line each, same keys where they apply):
                   p50/p99 latency and stations/s
   --workload c5   one GPU's share of the 1000-station P+S set on
                   321x321x161: --tables tables per launch, resident
+  --workload c2f  c2f's coarse-to-fine interpolation (the row beside the
+                  path): a 31x31x7 coarse model to the 301x301x61 fine grid
+                  into two resident models per step, the reference c2f timed
+                  beside it
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
     python bench.py --calibrate-copy MIB     # one known-size copy, for HBM counter scaling
@@ -436,6 +440,94 @@ def run_c5(args):
     return out
 
 
+# ------------------------------------------------------------------ c2f
+
+C2F_EXE = os.path.join(REPO, "oracle", "_ref", "c2f_ref")
+
+
+def c2f_reference(co, fine_shape):
+    """The reference c2f compiled from its own sources, one process on one core,
+    on the same coarse model: (wall seconds incl. file I/O, fine velocities)."""
+    wd = tempfile.mkdtemp(prefix="c2f_cpu_")
+    try:
+        with open(os.path.join(wd, "spec"), "w") as f:
+            f.write(workloads.c2f_spec_text(co.nxc, co.nyc, co.nzc, co.igridx, co.igridy, co.igridz, co.x0,
+                                            co.y0, co.z0, co.h))
+        head = tables.make_header(co.nxc, co.nyc, co.nzc, co.x0, co.y0, co.z0, co.h, co.h, co.h,
+                                  text=tables.header_text("CORS", "CART", "BMOD"))
+        with open(os.path.join(wd, "coarse.mod"), "wb") as f:
+            f.write(np.asarray(head, dtype=tables.VHEAD_DTYPE).tobytes())
+            for a in (co.igridx, co.igridy, co.igridz):
+                f.write(np.asarray(a, dtype="<i4").tobytes())
+            f.write(np.asarray(co.velocity, dtype="<f4").tobytes())
+        t0 = time.perf_counter()
+        r = subprocess.run([C2F_EXE], input="spec\n", cwd=wd, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError(f"reference c2f failed: {r.stdout[-500:]}")
+        n = int(np.prod(fine_shape))
+        raw = np.fromfile(os.path.join(wd, "fine.mod"), dtype=np.uint8)
+        return wall, raw[232:232 + 8 * n].view("<f4").reshape((2,) + tuple(fine_shape)).copy()
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+def run_c2f(args):
+    """c2f's coarse-to-fine interpolation (src/c2f.c:549-575, 673-727) on the
+    device: a 31 x 31 x 7 coarse model (P and S) to the 301 x 301 x 61 fine grid,
+    straight into two resident models; a step is one inversion iteration's
+    model update (coarse model up, interpolation, range, models replaced)."""
+    co = fdgpu.Coarse(*workloads.c2f_coarse())
+    nx, ny, nz = co.dims()
+    n = nx * ny * nz
+    ctx = fdgpu.Context(0)
+    mp, ms, fine = ctx.models_from_coarse(co, want_fine=True)
+    for _ in range(args.warmup):
+        ctx.update_models_from_coarse(mp, ms, co)
+    walls, kerns = [], []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        ctx.update_models_from_coarse(mp, ms, co)
+        walls.append(time.perf_counter() - t0)
+        kerns.append(ctx.last_kernel_ms())
+        print(f"# step {s}: {walls[-1] * 1e3:.3f} ms wall, {kerns[-1]:.3f} ms kernel", file=sys.stderr, flush=True)
+    wall = float(np.mean(walls))
+    kern_s = float(np.mean(kerns)) * 1e-3
+    alg = 8 * n   # the P and S slowness of every fine node written once
+    out = {
+        "metric": METRIC + " | c2f fine-model nodes/s",
+        "value": round(n / wall, 1),
+        "unit": "fine nodes/s (P and S), device-resident",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded coarse model, gradient x +-4 % per node, Vs = Vp/1.73 x +-2 %)",
+        "config": {"workload": "c2f: 31x31x7 coarse model (P and S, 10 fine intervals per coarse interval) "
+                               "to the 301x301x61 fine grid, two resident models updated per step",
+                   "fine_nodes": n, "coarse_nodes": co.nxc * co.nyc * co.nzc},
+        "kernel_nodes_per_s": round(n / kern_s, 1),
+        "roofline": {"bound": "hbm", "achieved": round(alg / kern_s / 1e9, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / kern_s / 1e9 / HBM_PEAK_GBS, 6), "traffic": None,
+                     "kernel": "c2f_prep_kernel + c2f_fine_kernel", "kernel_ms": round(kern_s * 1e3, 4),
+                     "algorithmic_bytes": alg},
+    }
+    if not args.no_cpu and os.path.exists(C2F_EXE):
+        cw, cfine = c2f_reference(co, (nz, ny, nx))
+        out["cpu_baseline"] = {"value": round(n / cw, 1), "unit": "fine nodes/s (P and S)", "cores": 1,
+                               "kind": "reference",
+                               "sample": "the whole workload: the reference c2f (oracle/_ref/c2f_ref) on the same "
+                                         "coarse model, one process, wall time incl. reading the coarse file and "
+                                         "writing its three fine-model files"}
+        out["gpu_over_cpu"] = round(cw / wall, 1)
+        out["nodes_differing_vs_reference"] = int((cfine.view(np.uint32) != fine.view(np.uint32)).sum())
+    else:
+        out["cpu_baseline"] = None
+    ctx.release_model(mp)
+    ctx.release_model(ms)
+    ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ calibration
 
 def calibrate_copy(mib: int) -> None:
@@ -457,7 +549,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "c2f"])
     ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -476,7 +568,7 @@ def main():
         print("bench.py: this build measures one GPU; the station tables shard across GPUs with no "
               "communication on the hot path, so multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
-    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload]
+    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

@@ -33,7 +33,7 @@ int check(const fdgpu_coarse *c, int &nx, int &ny, int &nz) {
     for (int i = 0; i < c->nxc - 1; i++) { if (c->igridx[i] < 1) return FDGPU_EINPUT; a += c->igridx[i]; }
     for (int i = 0; i < c->nyc - 1; i++) { if (c->igridy[i] < 1) return FDGPU_EINPUT; b += c->igridy[i]; }
     for (int i = 0; i < c->nzc - 1; i++) { if (c->igridz[i] < 1) return FDGPU_EINPUT; d += c->igridz[i]; }
-    if (a > (1 << 24) || b > (1 << 24) || d > (1 << 24) || a * b * d > (1LL << 40)) return FDGPU_EINPUT;
+    if (a > (1 << 24) || b > (1 << 24) || d > (1 << 24) || a * b * d >= (1LL << 31)) return FDGPU_EINPUT;
     nx = (int)a;
     ny = (int)b;
     nz = (int)d;
@@ -68,9 +68,16 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
     const size_t b_rg = 256;
     const size_t b_fine = fine ? align256(sizeof(float) * 2 * (size_t)n) : 0;
     const size_t bytes = b_co + b_g + b_ax + b_rg + b_fine;
-    char *blk = nullptr;
-    int rc = dev_alloc(c, (void **)&blk, bytes);
-    if (rc) return rc;
+    if (bytes > c->coarse_bytes) {
+        FD_HIP(hipStreamSynchronize(c->stream));
+        dev_free(c, c->d_coarse, c->coarse_bytes);
+        c->d_coarse = nullptr;
+        c->coarse_bytes = 0;
+        const int rc = dev_alloc(c, (void **)&c->d_coarse, bytes);
+        if (rc) return rc;
+        c->coarse_bytes = bytes;
+    }
+    char *blk = c->d_coarse;
     float *d_co = (float *)blk;
     float *d_g = (float *)(blk + b_co);
     C2fAxis *d_ax = (C2fAxis *)(blk + b_co + b_g);
@@ -97,7 +104,6 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
     ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
     float ms_ = 0.f;
     if (ok && hipEventElapsedTime(&ms_, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms_;
-    dev_free(c, blk, bytes);
     if (!ok) return FDGPU_EHIP;
     // velocity ranges bound the reverse-pass skip (fdprep::skip_safe); a
     // model with a non-positive or non-finite velocity never skips

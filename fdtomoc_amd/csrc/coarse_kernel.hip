@@ -5,8 +5,8 @@
 //
 //   c2f_prep_kernel  coarse velocities -> slowness (a double quotient rounded
 //                    to float, c2f.c:549-552); per fine coordinate of each
-//                    axis the coarse cell and the float coordinate
-//   c2f_fine_kernel  one fine node per lane, P and S: trilinear slowness in
+//                    axis the coarse cell and the fraction across it
+//   c2f_fine_kernel  fine nodes across the lanes, P and S: trilinear slowness in
 //                    the reference's float operation order, velocity 1/v,
 //                    the model's slowness 1/velocity (sphfd.c:689-690), and
 //                    the velocity range of each phase
@@ -44,7 +44,9 @@ __global__ void c2f_prep_kernel(const C2fGrid grid, float *coarse, long long nco
         const float v = (float)(o + i * grid.h);
         C2fAxis e;
         e.cell = coarse_cell(g, nc, v);
-        e.v = v;
+        const float gi = g[e.cell];
+        const float hg = g[e.cell + 1] - gi;
+        e.f = (v - gi) / hg;
         axis[q] = e;
     }
 }
@@ -63,8 +65,9 @@ __device__ void fold_range(unsigned lo, unsigned hi, unsigned bad, unsigned *ran
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 1; q < nw; q++) { lo = min(lo, slo[q]); hi = max(hi, shi[q]); bad |= sbad[q]; }
-        atomicMin(&range[0], lo);
-        atomicMax(&range[1], hi);
+        // the words only move one way: skip the atomics that cannot change them
+        if (lo < __hip_atomic_load(&range[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&range[0], lo);
+        if (hi > __hip_atomic_load(&range[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&range[1], hi);
         if (bad) atomicOr(&range[2], 1u);
     }
     __syncthreads();
@@ -73,51 +76,44 @@ __device__ void fold_range(unsigned lo, unsigned hi, unsigned bad, unsigned *ran
 __global__ void c2f_fine_kernel(const C2fGrid grid, const float *__restrict__ sl, const C2fAxis *__restrict__ axis,
                                 float *__restrict__ vel_out, float *__restrict__ slow_p, float *__restrict__ slow_s,
                                 unsigned *range) {
-    const long long nrow = (long long)grid.ny * grid.nz, n = nrow * grid.nx;
-    const long long nxyzc = (long long)grid.nxc * grid.nyc * grid.nzc;
+    // fine nodes in file order (x fastest) across the lanes: coalesced writes;
+    // the host keeps nx*ny*nz below 2^31
+    const unsigned nx = (unsigned)grid.nx, ny = (unsigned)grid.ny;
+    const unsigned n = nx * ny * (unsigned)grid.nz;
+    const unsigned nxyzc = (unsigned)(grid.nxc * grid.nyc * grid.nzc);
     unsigned lo[2] = {0xffffffffu, 0xffffffffu}, hi[2] = {0u, 0u}, bad[2] = {0u, 0u};
-    // one fine row (j, k) per block row, x across the lanes: coalesced writes
-    for (long long r = blockIdx.y; r < nrow; r += gridDim.y) {
-        const int j = (int)(r % grid.ny), k = (int)(r / grid.ny);
-        const C2fAxis ey = axis[grid.nx + j], ez = axis[grid.nx + grid.ny + k];
-        const int cj = ey.cell, ck = ez.cell;
-        const float yj = grid.gy[cj], hy = grid.gy[cj + 1] - yj;
-        const float zk = grid.gz[ck], hz = grid.gz[ck + 1] - zk;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const unsigned r = q / nx, i = q - r * nx;
+        const unsigned k = r / ny, j = r - k * ny;
+        const C2fAxis ex = axis[i], ey = axis[nx + j], ez = axis[nx + ny + k];
+        const int ci = ex.cell, cj = ey.cell, ck = ez.cell;
         const int nk = grid.nxc * grid.nyc * ck;
         const int nj = grid.nxc * cj;
         const int nk2 = grid.nxc * grid.nyc * (ck + 1);
         const int nj2 = grid.nxc * (cj + 1);
-        const float fy = (ey.v - yj) / hy;
-        const float fz = (ez.v - zk) / hz;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < grid.nx; i += gridDim.x * blockDim.x) {
-            const long long q = r * grid.nx + i;
-            const C2fAxis ex = axis[i];
-            const int ci = ex.cell;
-            const float xi = grid.gx[ci], hx = grid.gx[ci + 1] - xi;
-            const float fx = (ex.v - xi) / hx;
+        const float fx = ex.f, fy = ey.f, fz = ez.f;
 #pragma unroll
-            for (int ph = 0; ph < 2; ph++) {
-                const float *s = sl + ph * nxyzc + ci;
-                float tv = 0.f;
-                tv = (1.f - fx) * (1.f - fy) * (1.f - fz) * s[nk + nj];
-                tv = tv + fx * (1.f - fy) * (1.f - fz) * s[nk + nj + 1];
-                tv = tv + (1.f - fx) * fy * (1.f - fz) * s[nk + nj2];
-                tv = tv + (1.f - fx) * (1.f - fy) * fz * s[nk2 + nj];
-                tv = tv + fx * fy * (1.f - fz) * s[nk + nj2 + 1];
-                tv = tv + fx * (1.f - fy) * fz * s[nk2 + nj + 1];
-                tv = tv + (1.f - fx) * fy * fz * s[nk2 + nj2];
-                tv = tv + fx * fy * fz * s[nk2 + nj2 + 1];
-                const float vel = 1.f / tv;                     // c2f.c:570
-                if (vel_out) vel_out[ph * n + q] = vel;
-                (ph == 0 ? slow_p : slow_s)[q] = 1.0f / vel;    // sphfd.c:689-690
-                unsigned u;
-                __builtin_memcpy(&u, &vel, 4);
-                if (vel > 0.f && vel < __builtin_inff()) {
-                    lo[ph] = min(lo[ph], u);
-                    hi[ph] = max(hi[ph], u);
-                } else {
-                    bad[ph] = 1u;
-                }
+        for (int ph = 0; ph < 2; ph++) {
+            const float *s = sl + ph * nxyzc + ci;
+            float tv = 0.f;
+            tv = (1.f - fx) * (1.f - fy) * (1.f - fz) * s[nk + nj];
+            tv = tv + fx * (1.f - fy) * (1.f - fz) * s[nk + nj + 1];
+            tv = tv + (1.f - fx) * fy * (1.f - fz) * s[nk + nj2];
+            tv = tv + (1.f - fx) * (1.f - fy) * fz * s[nk2 + nj];
+            tv = tv + fx * fy * (1.f - fz) * s[nk + nj2 + 1];
+            tv = tv + fx * (1.f - fy) * fz * s[nk2 + nj + 1];
+            tv = tv + (1.f - fx) * fy * fz * s[nk2 + nj2];
+            tv = tv + fx * fy * fz * s[nk2 + nj2 + 1];
+            const float vel = 1.f / tv;                     // c2f.c:570
+            if (vel_out) vel_out[(size_t)ph * n + q] = vel;
+            (ph == 0 ? slow_p : slow_s)[q] = 1.0f / vel;    // sphfd.c:689-690
+            unsigned u;
+            __builtin_memcpy(&u, &vel, 4);
+            if (vel > 0.f && vel < __builtin_inff()) {
+                lo[ph] = min(lo[ph], u);
+                hi[ph] = max(hi[ph], u);
+            } else {
+                bad[ph] = 1u;
             }
         }
     }
@@ -134,10 +130,11 @@ extern "C" hipError_t fd_launch_c2f(const C2fGrid *grid, float *d_coarse, C2fAxi
     const long long m = ncoarse > nax ? ncoarse : nax;
     int blocks = (int)((m + 255) / 256);
     hipLaunchKernelGGL(c2f_prep_kernel, dim3(blocks), dim3(256), 0, stream, *grid, d_coarse, ncoarse, d_axis);
-    const long long nrow = (long long)grid->ny * grid->nz;
-    const int bx = (grid->nx + 255) / 256;
-    const int by = (int)(nrow < 65535 ? nrow : 65535);
-    hipLaunchKernelGGL(c2f_fine_kernel, dim3(bx < 16 ? bx : 16, by), dim3(256), 0, stream, *grid,
+    // at most 2048 blocks striding over the nodes: each block folds its range once
+    const long long n = (long long)grid->nx * grid->ny * grid->nz;
+    if (n >= (1LL << 31)) return hipErrorInvalidValue;
+    const long long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(c2f_fine_kernel, dim3((unsigned)(nb < 2048 ? nb : 2048)), dim3(256), 0, stream, *grid,
                        (const float *)d_coarse, (const C2fAxis *)d_axis, d_vel_out, d_slow_p, d_slow_s, d_range);
     return hipGetLastError();
 }

@@ -13,10 +13,11 @@ struct C2fGrid {
     double x0, y0, z0, h;
 };
 
-// One fine coordinate of an axis: its coarse cell and its float value.
+// One fine coordinate of an axis: its coarse cell and its fraction across
+// the cell, (v - g[cell]) / (g[cell + 1] - g[cell]) in float (c2f.c:698-704).
 struct C2fAxis {
     int cell;
-    float v;
+    float f;
 };
 
 // Converts d_coarse (2*nxc*nyc*nzc velocities, P then S) to slowness in

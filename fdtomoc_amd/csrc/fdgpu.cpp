@@ -319,6 +319,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     for (auto &kv : c->models) dev_free(c, kv.second.d_slow, kv.second.bytes);
     clear_graphs(c);
     clear_geos(c);
+    dev_free(c, c->d_coarse, c->coarse_bytes);
     dev_free(c, c->d_jobs, sizeof(fdsweep::TableJob) * (size_t)c->jobs_cap);
     dev_free(c, c->d_scratch, c->scratch_bytes);
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);

@@ -161,6 +161,9 @@ struct fdgpu_ctx {
     // geometry tables of the grids solved so far, keyed by the bits of
     // (ny, nz, h, dq, df, z0r, sin(qc) table); shared by every table of a grid
     std::map<std::vector<unsigned long long>, fdgpu_detail::GeoBuf> geos;
+    // scratch of the coarse-to-fine interpolation, grown on demand
+    char *d_coarse = nullptr;
+    size_t coarse_bytes = 0;
     double last_ms = 0.0;
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     fdgpu_detail::Serve serve;

@@ -176,3 +176,33 @@ def c5(stations: int = 1000, seed: int = 5):
     vs = (vp / np.float32(1.73)).astype(np.float32)
     sts = synth.random_surface_stations(g, rng, stations)
     return g, vp, vs, sts
+
+
+# ------------------------------------------------------------------ c2f
+
+def c2f_coarse(seed: int = 6, nxc: int = 31, nyc: int = 31, nzc: int = 7, step: int = 10, h: float = 1.0):
+    """A coarse inversion model for c2f's interpolation (src/c2f.c): nxc x nyc x
+    nzc nodes every `step` fine nodes (31 x 31 x 7 at step 10 gives the
+    301 x 301 x 61 fine grid, the largest the reference's arrays hold,
+    include/common/parameter.h), Vp 5.0 -> 8.0 km/s over 0-64 km times a
+    seeded +-4 % perturbation per node, Vs = Vp / 1.73 times +-2 %.  Returns
+    (nxc, nyc, nzc, igridx, igridy, igridz, x0, y0, z0, h, velocities P then S)."""
+    rng = np.random.default_rng(seed)
+    z = np.arange(nzc, dtype=np.float64) * step * h
+    vp = (5.0 + 3.0 * np.clip(z / 64.0, 0.0, None))[:, None, None] * np.ones((nzc, nyc, nxc))
+    vp = vp * (1.0 + 0.04 * rng.uniform(-1.0, 1.0, size=vp.shape))
+    vs = vp / 1.73 * (1.0 + 0.02 * rng.uniform(-1.0, 1.0, size=vp.shape))
+    vel = np.concatenate([vp.ravel(), vs.ravel()]).astype(np.float32)
+    ig = lambda n: np.full(n - 1, step, dtype=np.int32)  # noqa: E731
+    return nxc, nyc, nzc, ig(nxc), ig(nyc), ig(nzc), 0.0, 0.0, 0.0, h, vel
+
+
+def c2f_spec_text(nxc, nyc, nzc, igridx, igridy, igridz, x0, y0, z0, h, oldvfil="coarse.mod",
+                  finevel="fine.mod") -> str:
+    """A c2f spec file (keys of src/c2f.c:81-83, 170-257); c2f reads nxc
+    values from each igrid line, the last one unused."""
+    def row(name, arr):
+        return name + " " + " ".join(str(int(v)) for v in list(arr) + [0]) + "\n"
+    return (f"nxc {nxc}\nnyc {nyc}\nnzc {nzc}\nh {h!r}\noldvfil {oldvfil}\ntgrdfil grid.txt\n"
+            f"finevel {finevel}\nx0 {x0!r}\ny0 {y0!r}\nz0 {z0!r}\n"
+            + row("igridx", igridx) + row("igridy", igridy) + row("igridz", igridz))

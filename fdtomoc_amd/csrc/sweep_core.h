@@ -80,6 +80,16 @@ FD_INLINE double xdiv(double n, double d, double y) {
 #endif
 }
 
+// xdiv for a numerator and divisor that are finite by construction (sums of
+// precomputed geometric reciprocals over products of two layer radii): the
+// special-case step is not needed.  A zero numerator here is +0 (a sum of
+// opposite values rounds to +0), whose quotient the formula gives as +0.
+FD_INLINE double xdiv_geo(double n, double d, double y) {
+    const double q0 = n * y;
+    const double e = fma(d, q0, -n);
+    return fma(-e, y, q0);
+}
+
 // ---------------------------------------------------------------- stencils
 //
 // The stencils of sphfd.c with every sub-expression that depends on the grid
@@ -759,7 +769,7 @@ FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j
         layer_pair<0, o.dz>(G, k, R, Y);
         const double z = row_at(G, j)[kRowZ + o.dy + 1];
         const double w = (o.dy == 0 ? G.idqsq : -G.idqsq) + (o.dx == 0 ? z : -z);
-        b += t[I] * ((o.dz == 0 ? G.ihsq : -G.ihsq) + xdiv(w, R, Y));
+        b += t[I] * ((o.dz == 0 ? G.ihsq : -G.ihsq) + xdiv_geo(w, R, Y));
     });
     b = 2.0 * b;
     // c: t_i^2 (1/h^2 + (1/dq^2 + 1/(df sqi)^2) / ri^2) ...
@@ -768,7 +778,7 @@ FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j
         constexpr P3 o = T.pt[I];
         const double *L = lay_at(G, k + o.dz);
         const double w = G.idqsq + row_at(G, j + o.dy)[kRowIS2];
-        c += t[I] * t[I] * (G.ihsq + xdiv(w, L[kLayR00], L[kLayY00]));
+        c += t[I] * t[I] * (G.ihsq + xdiv_geo(w, L[kLayR00], L[kLayY00]));
     });
     // ... + 2 t_i sum_j t_j ((gi gj)/h^2 + ((ni nj)/dq^2 + (mi mj)/((df^2 sqi) sqj)) / (ri rj))
     static_for<0, 6>([&](auto I) {
@@ -780,13 +790,13 @@ FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j
             layer_pair<u.dz, v.dz>(G, k, R, Y);
             const double x = row_pair<u.dy, v.dy>(G, j);
             const double w = (u.dy == v.dy ? G.idqsq : -G.idqsq) + (u.dx == v.dx ? x : -x);
-            c2 += t[J] * ((u.dz == v.dz ? G.ihsq : -G.ihsq) + xdiv(w, R, Y));
+            c2 += t[J] * ((u.dz == v.dz ? G.ihsq : -G.ihsq) + xdiv_geo(w, R, Y));
         });
         c += 2.0 * c2 * t[I];
     });
     // a: the node's own diagonal term
     const double *L7 = lay_at(G, k);
-    const double a = G.ihsq + xdiv(G.idqsq + row_at(G, j)[kRowIS2], L7[kLayR00], L7[kLayY00]);
+    const double a = G.ihsq + xdiv_geo(G.idqsq + row_at(G, j)[kRowIS2], L7[kLayR00], L7[kLayY00]);
     c = c - 16.0 * slo * slo;
     const double rad = b * b - 4.0 * a * c;
     if (!(rad >= 0.)) return kAcausal;

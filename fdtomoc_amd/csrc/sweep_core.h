@@ -516,11 +516,11 @@ struct FaceSpec<SOUTH> {
 // with X a reciprocal that depends on rows only and R = r_a * r_b a product
 // of two layer radii.  X, R and RN(1/R) come from per-table row and layer
 // tables, the division by R from xdiv, so each coefficient costs an add and
-// one fast division and equals the reference's bit for bit.  Values that
-// depend on both row and layer (the reciprocal grid distances of fdsph2d,
-// fdsphne and fdsphnf and the `a` terms of the quadratics) come from a cell
-// table.  Every entry is computed with the reference's operations in the
-// reference's order.
+// one fast division and equals the reference's bit for bit.  The few values
+// that depend on both row and layer (the reciprocal of the distance
+// (r sin q) df and the `a` terms of the quadratics) are divided per node.
+// Every entry is computed with the reference's operations in the reference's
+// order.
 //
 // Row entry j (j = -1..ny), with sq(j) = sin(qc(j)), dfsq = df*df:
 //   kRowIS2        1 / ((df sq(j)) (df sq(j)))               (D terms)
@@ -531,16 +531,9 @@ struct FaceSpec<SOUTH> {
 // Layer entry k (k = -1..nz), r(k) = z0r - h k:
 //   kLayR00, kLayY00   r(k) r(k) and its reciprocal
 //   kLayR01, kLayY01   r(k) r(k+1) and its reciprocal
-//   kLayYQ             RN(1 / (r(k) dq))
-// Cell entry (j, k):
-//   kCellRS + (dz+1)*3 + (dy+1)   RN(1 / S(j+dy, k+dz)), S(j, k) = (r(k) sq(j)) df
-//   kCellA + p, kCellY + p        a = 1/d^2 + 1/e^2 of fdsph2d/fdsphne for the
-//                                 node's distance pairs p = {S,H}, {Q,H}, {S,Q}
-//                                 (H = h, Q = r(k) dq) and RN(1 / (2 a))
-//   kCellY3                       RN(1 / (2 a)) of fdsph3d
+//   kLayYQ, kLayIQ2    RN(1 / Q) and RN(1 / (Q Q)), Q = r(k) dq
 constexpr int kRowIS2 = 0, kRowZ = 1, kRowX0 = 4, kRowXp = 5, kRowXq = 6, kRowStride = 7;
-constexpr int kLayR00 = 0, kLayY00 = 1, kLayR01 = 2, kLayY01 = 3, kLayYQ = 4, kLayStride = 5;
-constexpr int kCellRS = 0, kCellA = 9, kCellY = 12, kCellY3 = 15, kCellStride = 16;
+constexpr int kLayR00 = 0, kLayY00 = 1, kLayR01 = 2, kLayY01 = 3, kLayYQ = 4, kLayIQ2 = 5, kLayStride = 6;
 
 struct GeoParams {
     int ny, nz;
@@ -548,21 +541,19 @@ struct GeoParams {
     const double *sinq;   // sin(qc(j)) for j = -1..ny, at [j + 1]
 };
 
-// Buffer layout: rows | layers | cells.
+// Buffer layout: rows | layers.
 FD_INLINE long long geo_rows_doubles(int ny) { return (long long)(ny + 2) * kRowStride; }
 FD_INLINE long long geo_layers_doubles(int nz) { return (long long)(nz + 2) * kLayStride; }
-FD_INLINE long long geo_doubles(int ny, int nz) {
-    return geo_rows_doubles(ny) + geo_layers_doubles(nz) + (long long)(ny + 2) * (nz + 2) * kCellStride;
-}
-// Work items of a fill: one per row, layer and cell entry.
-FD_INLINE long long geo_items(int ny, int nz) { return (long long)(ny + 2) + (nz + 2) + (long long)(ny + 2) * (nz + 2); }
+FD_INLINE long long geo_doubles(int ny, int nz) { return geo_rows_doubles(ny) + geo_layers_doubles(nz); }
+// Work items of a fill: one per row and layer entry.
+FD_INLINE long long geo_items(int ny, int nz) { return (long long)(ny + 2) + (nz + 2); }
 
 // Fills work item `it` of the geometry buffer `out`.
 FD_INLINE void geo_fill_item(const GeoParams &G, long long it, double *out) {
     auto rcz = [&](int kk) { return G.z0r - G.h * (kk); };
     auto sq = [&](int jj) { return G.sinq[jj + 1]; };
-    const double hsq = G.h * G.h, dqsq = G.dq * G.dq, dfsq = G.df * G.df;
-    const long long nr = G.ny + 2, nl = G.nz + 2;
+    const double dfsq = G.df * G.df;
+    const long long nr = G.ny + 2;
     if (it < nr) {
         const int j = (int)it - 1;
         double *E = out + it * kRowStride;
@@ -581,41 +572,16 @@ FD_INLINE void geo_fill_item(const GeoParams &G, long long it, double *out) {
         return;
     }
     it -= nr;
-    if (it < nl) {
-        const int k = (int)it - 1;
-        double *E = out + nr * kRowStride + it * kLayStride;
-        for (int q = 0; q < kLayStride; q++) E[q] = 0.0;
-        E[kLayR00] = rcz(k) * rcz(k);
-        E[kLayY00] = 1.0 / E[kLayR00];
-        E[kLayR01] = rcz(k) * rcz(k + 1);
-        E[kLayY01] = 1.0 / E[kLayR01];
-        E[kLayYQ] = 1.0 / (rcz(k) * G.dq);
-        return;
-    }
-    it -= nl;
-    const int j = (int)(it % nr) - 1, k = (int)(it / nr) - 1;
-    double *E = out + nr * kRowStride + nl * kLayStride + it * kCellStride;
-    for (int q = 0; q < kCellStride; q++) E[q] = 0.0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            const int jj = j + dy, kk = k + dz;
-            if (jj < -1 || jj > G.ny || kk < -1 || kk > G.nz) continue;
-            E[kCellRS + (dz + 1) * 3 + (dy + 1)] = 1.0 / (rcz(kk) * sq(jj) * G.df);
-        }
-    if (j < 0 || j >= G.ny || k < 0 || k >= G.nz) return;
-    const double S = rcz(k) * sq(j) * G.df, Q = rcz(k) * G.dq, H = G.h;
-    const double aSH = 1.0 / (S * S) + 1.0 / (H * H);
-    const double aQH = 1.0 / (Q * Q) + 1.0 / (H * H);
-    const double aSQ = 1.0 / (S * S) + 1.0 / (Q * Q);
-    E[kCellA + 0] = aSH;
-    E[kCellA + 1] = aQH;
-    E[kCellA + 2] = aSQ;
-    E[kCellY + 0] = 1.0 / (2 * aSH);
-    E[kCellY + 1] = 1.0 / (2 * aQH);
-    E[kCellY + 2] = 1.0 / (2 * aSQ);
-    const double dfsinq7 = G.df * sq(j), r7 = rcz(k);
-    const double a3 = 1.0 / hsq + (1.0 / dqsq + 1. / (dfsinq7 * dfsinq7)) / (r7 * r7);
-    E[kCellY3] = 1.0 / (2.0 * a3);
+    const int k = (int)it - 1;
+    double *E = out + nr * kRowStride + it * kLayStride;
+    for (int q = 0; q < kLayStride; q++) E[q] = 0.0;
+    E[kLayR00] = rcz(k) * rcz(k);
+    E[kLayY00] = 1.0 / E[kLayR00];
+    E[kLayR01] = rcz(k) * rcz(k + 1);
+    E[kLayY01] = 1.0 / E[kLayR01];
+    const double Q = rcz(k) * G.dq;
+    E[kLayYQ] = 1.0 / Q;
+    E[kLayIQ2] = 1.0 / (Q * Q);
 }
 
 // index of the distance pair {f, g} of the node (forms DH, DRDQ, DRSDF)
@@ -634,16 +600,12 @@ struct Geo {
     const double *sinq;       // sin(qc(j)) for j = -1..ny, stored at [j + 1]
     const double *rowt;       // geometry row table (entry j at [(j + 1) * kRowStride])
     const double *layt;       // geometry layer table (entry k at [(k + 1) * kLayStride])
-    const double *cellt;      // geometry cell table
     int nx, ny, nz;
     int box[3][2];            // current box sides x1 x2 / y1 y2 / z1 z2
 };
 
 FD_INLINE const double *row_at(const Geo &G, int j) { return G.rowt + (j + 1) * kRowStride; }
 FD_INLINE const double *lay_at(const Geo &G, int k) { return G.layt + (k + 1) * kLayStride; }
-FD_INLINE const double *cell_at(const Geo &G, int j, int k) {
-    return G.cellt + ((long long)(k + 1) * (G.ny + 2) + (j + 1)) * kCellStride;
-}
 // r(k+a) r(k+b) and its reciprocal (|a - b| <= 1); the product is commutative.
 template <int A, int B>
 FD_INLINE void layer_pair(const Geo &G, int k, double &R, double &Y) {
@@ -740,12 +702,28 @@ FD_INLINE double sinq_of(const Nbhd &N, int dy) { return N.sq[dy + 1]; }
 FD_INLINE double dist_of(const Geo &G, const Nbhd &N, DS d, int k) {
     return d.form == DH ? G.h : d.form == DRDQ ? rc(G, k + d.dz) * G.dq : rc(G, k + d.dz) * sinq_of(N, d.dy) * G.df;
 }
-// A distance and its reciprocal; ce is the node's cell entry.
-FD_INLINE Dist dist_y(const Geo &G, const Nbhd &N, const double *ce, DS d, int k) {
+// A distance and its correctly rounded reciprocal: h and r dq from the
+// tables, (r sin q) df divided here (its reciprocal depends on row and layer).
+FD_INLINE Dist dist_y(const Geo &G, const Nbhd &N, DS d, int k) {
     Dist r;
     r.d = dist_of(G, N, d, k);
-    r.y = d.form == DH ? G.rh : d.form == DRDQ ? lay_at(G, k + d.dz)[kLayYQ] : ce[kCellRS + (d.dz + 1) * 3 + (d.dy + 1)];
+    r.y = d.form == DH ? G.rh : d.form == DRDQ ? lay_at(G, k + d.dz)[kLayYQ] : 1.0 / r.d;
     return r;
+}
+
+// a = 1/d^2 + 1/e^2 of fdsph2d/fdsphne for the node's distance pair P
+// ({S,H}, {Q,H}, {S,Q}: S = (r sin q) df, Q = r dq, H = h) and RN(1/(2 a)).
+template <int P>
+FD_INLINE void pair_a(const Geo &G, const Nbhd &N, int k, double &a, double &ya) {
+    const double iH = G.ihsq, iQ = lay_at(G, k)[kLayIQ2];
+    if constexpr (P == 1) {
+        a = iQ + iH;
+    } else {
+        const double S = dist_of(G, N, DS{DRSDF, 0, 0}, k);
+        const double iS = 1.0 / (S * S);
+        a = P == 0 ? iS + iH : iS + iQ;
+    }
+    ya = 1.0 / (2 * a);
 }
 FD_INLINE double factor_of(const Geo &G, const Nbhd &N, DS f, int k) {
     return f.form == XH ? G.h : f.form == XR ? rc(G, k + f.dz) : f.form == XDQ ? G.dq : f.form == XDF ? G.df
@@ -753,9 +731,9 @@ FD_INLINE double factor_of(const Geo &G, const Nbhd &N, DS f, int k) {
 }
 
 // fdsph3d of trial TI of face F at node (j, k) (sphfd.c:5935-6010), every
-// geometric sub-expression from the geometry tables; ce is the cell entry.
+// geometric sub-expression from the geometry tables.
 template <int F, int TI>
-FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j, int k, const double *ce) {
+FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j, int k) {
     constexpr Trial T = FaceSpec<F>::trials[TI];
     double slo = 0.;
 #pragma unroll
@@ -801,7 +779,7 @@ FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j
     const double rad = b * b - 4.0 * a * c;
     if (!(rad >= 0.)) return kAcausal;
     const double sr = sqrt(rad);
-    const double a2 = 2.0 * a, ya = ce[kCellY3];
+    const double a2 = 2.0 * a, ya = 1.0 / (2.0 * a);
     const double x1 = xdiv(-b + sr, a2, ya);
     const double x2 = xdiv(-b - sr, a2, ya);
     bool bad1 = false, bad2 = false;
@@ -841,7 +819,6 @@ template <int F>
 FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float told, float tin0, float tin4,
                      NodeOut &out) {
     const float slow = N.s[nb_index<F>(0, 0, 0)];
-    const double *ce = cell_at(G, j, k);
     double t0 = tin0, t4 = tin4;   // the carried registers
     bool own04 = false, self = false, used_in = false;
     double guess = told;
@@ -865,7 +842,7 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
                 if constexpr (P < 7) t[P] = N.t[q];
                 s[P] = N.s[q];
             });
-            const double tr = st_3d<F, TI>(t, s, G, j, k, ce);
+            const double tr = st_3d<F, TI>(t, s, G, j, k);
             t0 = t[0];
             t4 = t[4];
             own04 = true;
@@ -881,9 +858,10 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
             s[5] = slow;
             static_assert(T.d[3].dz == 0 && T.d[3].dy == 0 && T.d[4].dz == 0 && T.d[4].dy == 0, "node distances");
             constexpr int pa = geo_pair(T.d[3].form, T.d[4].form);
-            const double tr = st_ne(t, s, dist_y(G, N, ce, T.d[0], k), dist_y(G, N, ce, T.d[1], k),
-                                    dist_y(G, N, ce, T.d[2], k), dist_y(G, N, ce, T.d[3], k),
-                                    dist_y(G, N, ce, T.d[4], k), ce[kCellA + pa], ce[kCellY + pa]);
+            double a, ya;
+            pair_a<pa>(G, N, k, a, ya);
+            const double tr = st_ne(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_y(G, N, T.d[2], k),
+                                    dist_y(G, N, T.d[3], k), dist_y(G, N, T.d[4], k), a, ya);
             t0 = t[0];
             t4 = t[4];
             own04 = true;
@@ -902,10 +880,12 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
             if (!own04) used_in = true;
             static_assert(T.d[0].dz == 0 && T.d[0].dy == 0 && T.d[1].dz == 0 && T.d[1].dy == 0, "node distances");
             constexpr int pa = geo_pair(T.d[0].form, T.d[1].form);
-            const Dist D01 = dist_y(G, N, ce, T.d[0], k);
+            double a, ya;
+            pair_a<pa>(G, N, k, a, ya);
+            const Dist D01 = dist_y(G, N, T.d[0], k);
             const double d01 = D01.d;
-            const double tr = st_2d(t, s, D01, dist_y(G, N, ce, T.d[1], k), dist_y(G, N, ce, T.d[2], k),
-                                    dist_y(G, N, ce, T.d[3], k), ce[kCellA + pa], ce[kCellY + pa]);
+            const double tr = st_2d(t, s, D01, dist_y(G, N, T.d[1], k), dist_y(G, N, T.d[2], k),
+                                    dist_y(G, N, T.d[3], k), a, ya);
             if (tr < guess) {
                 if constexpr (T.fhead) fhead = (guess - tr) / (d01 * (double)slow);
                 guess = tr;
@@ -918,7 +898,7 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
                 s[P] = N.s[q];
             });
             s[5] = slow;
-            const double tr = st_nf(t, s, dist_y(G, N, ce, T.d[0], k), dist_y(G, N, ce, T.d[1], k), dist_of(G, N, T.d[2], k));
+            const double tr = st_nf(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_of(G, N, T.d[2], k));
             t0 = t[0];
             t4 = t[4];
             own04 = true;

@@ -761,7 +761,6 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
     G.idqsq = 1. / (J.dq * J.dq);
     G.rowt = geo_lds ? sh.geol : gbuf;
     G.layt = G.rowt + geo_rows_doubles(J.ny);
-    G.cellt = gbuf + nrl;
     G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;
     int x1 = J.box[0][0], x2 = J.box[0][1], y1 = J.box[1][0], y2 = J.box[1][1], z1 = J.box[2][0], z2 = J.box[2][1];
     int dx1 = J.dflag[0], dx2 = J.dflag[1], dy1 = J.dflag[2], dy2 = J.dflag[3], dz1 = J.dflag[4], dz2 = J.dflag[5];

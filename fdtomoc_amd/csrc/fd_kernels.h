@@ -7,14 +7,26 @@
 #include "sweep_solver.h"
 
 // Two builds of the sweep kernel, both at two wavefronts per SIMD:
-//   wide    512 lanes, one workgroup per CU (LDS arena of kArenaWords words):
-//           the fastest per table, used while there are no more tables than CUs
-//   narrow  256 lanes, two workgroups per CU (LDS arena of kNarrowArenaWords):
-//           two tables share a CU, used for larger batches
+//   wide    512 lanes, one workgroup per CU (LDS arena of kArenaWords words,
+//           geometry tables up to kGeoLdsDoubles): the fastest per table, used
+//           while there are no more tables than CUs
+//   narrow  256 lanes, two workgroups per CU (LDS arena of kNarrowArenaWords,
+//           geometry tables up to kNarrowGeoLds): two tables share a CU, used
+//           for larger batches
 // The server kernel is built wide.
 enum { FD_WIDE = 0, FD_NARROW = 1 };
 constexpr int kWideThreads = 512, kNarrowThreads = 256;
-constexpr int kNarrowArenaWords = 13600;
+constexpr int kNarrowArenaWords = 13400;
+constexpr int kNarrowGeoLds = 3400;   // geometry row/layer doubles in LDS (wide: fdsweep::kGeoLdsDoubles)
+
+// Geometry row/layer doubles a table of this grid keeps in LDS, and what
+// each build holds.
+inline long long fd_geo_lds_need(int ny, int nz) {
+    return fdsweep::geo_rows_doubles(ny) + fdsweep::geo_layers_doubles(nz);
+}
+inline long long fd_geo_lds_capacity(int variant) {
+    return variant == FD_NARROW ? kNarrowGeoLds : fdsweep::kGeoLdsDoubles;
+}
 
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream);

@@ -377,7 +377,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     // prepare every table on the host first: no device work on bad input
     std::vector<Table> tabs((size_t)n);
     std::vector<std::vector<float>> presets((size_t)n);
-    long long maxF = 1;
+    long long maxF = 1, geo_need = 0;
     for (int t = 0; t < n; t++) {
         const fdgpu_table_desc &d = desc[t];
         auto mit = c->models.find(d.model);
@@ -386,7 +386,10 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         int rc = prepare_table(d, m.nx, m.ny, m.nz, tabs[(size_t)t], presets[(size_t)t]);
         if (rc) return rc;
         maxF = std::max(maxF, fdprep::max_face(d.par.nx, d.par.ny, d.par.nz));
+        geo_need = std::max(geo_need, fd_geo_lds_need(d.par.ny, d.par.nz));
     }
+    // every table's geometry row/layer tables must fit the LDS of the wide build
+    if (geo_need > fd_geo_lds_capacity(FD_WIDE)) return FDGPU_EINPUT;
     // device buffers per table
     std::vector<fdsweep::TableJob> jobs((size_t)n);
     std::map<std::vector<unsigned long long>, int> geo_keep;
@@ -487,6 +490,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     if (mode == FDGPU_BATCHED) {
         if (c->lanes == kNarrowThreads) variant = FD_NARROW;
         else if (c->lanes == 0 && n > c->num_cu * c->wg_per_cu[FD_WIDE]) variant = FD_NARROW;
+        if (geo_need > fd_geo_lds_capacity(variant)) variant = FD_WIDE;   // same results, larger LDS
     }
     int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu[variant]) : 1;
     if (nslots < 1) nslots = 1;

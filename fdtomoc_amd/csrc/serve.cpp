@@ -186,6 +186,8 @@ int fdgpu_serve_start(fdgpu_ctx *c, const fdgpu_serve_config *cfg) {
     if (!c || !cfg || cfg->max_nx < 1 || cfg->max_ny < 1 || cfg->max_nz < 1 || cfg->queue_capacity < 0 ||
         cfg->workgroups < 0)
         return FDGPU_EINPUT;
+    // the server is the wide build: its LDS holds the geometry tables of grids up to this size
+    if (fd_geo_lds_need(cfg->max_ny, cfg->max_nz) > fd_geo_lds_capacity(FD_WIDE)) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
     Serve &S = c->serve;
     if (S.on) return FDGPU_EBUSY;

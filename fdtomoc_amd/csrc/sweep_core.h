@@ -594,27 +594,34 @@ constexpr int geo_pair(int f, int g) {
 // --------------------------------------------------------- node evaluation
 
 // Per-table geometry shared by all nodes (sphfd.c:151-156, 437-455).
+// Pointer to the geometry row/layer tables, which live in LDS on the GPU.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(3))) double *GeoPtr;
+#else
+typedef const double *GeoPtr;
+#endif
+
 struct Geo {
     double h, dq, df, y0, z0r;
     double rh, ihsq, idqsq;   // RN(1/h), 1/(h h), 1/(dq dq)
     const double *sinq;       // sin(qc(j)) for j = -1..ny, stored at [j + 1]
-    const double *rowt;       // geometry row table (entry j at [(j + 1) * kRowStride])
-    const double *layt;       // geometry layer table (entry k at [(k + 1) * kLayStride])
+    GeoPtr rowt;              // geometry row table in LDS (entry j at [(j + 1) * kRowStride])
+    GeoPtr layt;              // geometry layer table in LDS (entry k at [(k + 1) * kLayStride])
     int nx, ny, nz;
     int box[3][2];            // current box sides x1 x2 / y1 y2 / z1 z2
 };
 
-FD_INLINE const double *row_at(const Geo &G, int j) { return G.rowt + (j + 1) * kRowStride; }
-FD_INLINE const double *lay_at(const Geo &G, int k) { return G.layt + (k + 1) * kLayStride; }
+FD_INLINE GeoPtr row_at(const Geo &G, int j) { return G.rowt + (j + 1) * kRowStride; }
+FD_INLINE GeoPtr lay_at(const Geo &G, int k) { return G.layt + (k + 1) * kLayStride; }
 // r(k+a) r(k+b) and its reciprocal (|a - b| <= 1); the product is commutative.
 template <int A, int B>
 FD_INLINE void layer_pair(const Geo &G, int k, double &R, double &Y) {
     if constexpr (A == B) {
-        const double *L = lay_at(G, k + A);
+        const GeoPtr L = lay_at(G, k + A);
         R = L[kLayR00];
         Y = L[kLayY00];
     } else {
-        const double *L = lay_at(G, k + (A < B ? A : B));
+        const GeoPtr L = lay_at(G, k + (A < B ? A : B));
         R = L[kLayR01];
         Y = L[kLayY01];
     }
@@ -754,7 +761,7 @@ FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j
     double c = 0.;
     static_for<0, 7>([&](auto I) {
         constexpr P3 o = T.pt[I];
-        const double *L = lay_at(G, k + o.dz);
+        const GeoPtr L = lay_at(G, k + o.dz);
         const double w = G.idqsq + row_at(G, j + o.dy)[kRowIS2];
         c += t[I] * t[I] * (G.ihsq + xdiv_geo(w, L[kLayR00], L[kLayY00]));
     });
@@ -773,7 +780,7 @@ FD_INLINE double st_3d(const double t[7], const double s[8], const Geo &G, int j
         c += 2.0 * c2 * t[I];
     });
     // a: the node's own diagonal term
-    const double *L7 = lay_at(G, k);
+    const GeoPtr L7 = lay_at(G, k);
     const double a = G.ihsq + xdiv_geo(G.idqsq + row_at(G, j)[kRowIS2], L7[kLayR00], L7[kLayY00]);
     c = c - 16.0 * slo * slo;
     const double rad = b * b - 4.0 * a * c;

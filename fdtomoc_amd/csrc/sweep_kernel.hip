@@ -32,10 +32,10 @@ struct DevExec {
     __device__ __forceinline__ long long clock() const { return (long long)wall_clock64(); }
 };
 
-template <int THREADS, int ARENA>
+template <int THREADS, int ARENA, int GEOL>
 __global__ void __launch_bounds__(THREADS, 2)
     sphfd_sweep_kernel(const fdsweep::TableJob *jobs, int njobs, const fdsweep::Scratch *slots, int *queue) {
-    __shared__ fdsweep::SharedT<ARENA> sh;
+    __shared__ fdsweep::SharedT<ARENA, GEOL> sh;
     DevExec x;
     const fdsweep::Scratch W = slots[blockIdx.x];
     while (true) {
@@ -140,10 +140,10 @@ __global__ void slowness_kernel(const float *__restrict__ vel, float *__restrict
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream) {
     if (variant == FD_NARROW)
-        hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords>), dim3(nslots), dim3(kNarrowThreads),
+        hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds>), dim3(nslots), dim3(kNarrowThreads),
                            0, stream, d_jobs, njobs, d_slots, d_queue);
     else
-        hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords>), dim3(nslots), dim3(kWideThreads), 0,
+        hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>), dim3(nslots), dim3(kWideThreads), 0,
                            stream, d_jobs, njobs, d_slots, d_queue);
     return hipGetLastError();
 }
@@ -164,8 +164,8 @@ extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::Ser
 }
 
 extern "C" const void *fd_sweep_kernel_ptr(int variant) {
-    return variant == FD_NARROW ? (const void *)sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords>
-                                : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords>;
+    return variant == FD_NARROW ? (const void *)sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds>
+                                : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>;
 }
 
 extern "C" hipError_t fd_launch_reset(int *d_queue, hipStream_t stream) {

@@ -107,12 +107,13 @@ struct Scratch {
 
 // LDS scratch of a workgroup: radix histograms during a sort, then the LEVEL
 // schedule's byte-packed pending counters and successor masks (faces up to
-// about 94K nodes; larger faces use global scratch).  The geometry row and
-// layer tables of the table being solved sit in LDS beside it when they fit
-// (7 (ny + 2) + 5 (nz + 2) <= kGeoLdsDoubles, e.g. 321 x 161 grids); else
-// they are read from HBM.
-constexpr int kArenaWords = 29600;
-constexpr int kGeoLdsDoubles = 3200;
+// about 83K nodes with the wide build's arena; larger faces use global
+// scratch).  Beside it, the geometry row and layer tables of the table being
+// solved: geo_rows_doubles(ny) + geo_layers_doubles(nz) doubles, at most
+// kGeoLds (the host sends larger grids to a build that holds them, or refuses
+// them).
+constexpr int kArenaWords = 26000;
+constexpr int kGeoLdsDoubles = 6800;
 
 template <int AW, int GL = kGeoLdsDoubles>
 struct SharedT {
@@ -740,12 +741,20 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
         for (long long e = tid; e < ne; e += T) geo_fill_item(gp, e, J.geo_fill);
         x.sync();
     }
-    // the row and layer tables into LDS when they fit
+    // the row and layer tables into LDS
     const double *gbuf = J.geo_fill ? J.geo_fill : J.geo;
     const long long nrl = geo_rows_doubles(J.ny) + geo_layers_doubles(J.nz);
-    const bool geo_lds = nrl <= SH::kGeoLds;
-    if (geo_lds)
-        for (long long q = tid; q < nrl; q += T) sh.geol[q] = gbuf[q];
+    if (nrl > SH::kGeoLds) {   // never sent by the host (it picks a build that holds them)
+        if (tid == 0) {
+            for (int s = 0; s < kStatCount; s++) sh.st[s] = 0;
+            sh.st[kStatError] = 64;
+        }
+        x.sync();
+        for (int q = tid; q < kStatCount; q += T) J.stats[q] = sh.st[q];
+        x.sync();
+        return;
+    }
+    for (long long q = tid; q < nrl; q += T) sh.geol[q] = gbuf[q];
     if (tid == 0) {
         for (int s = 0; s < kStatCount; s++) sh.st[s] = 0;
         sh.cin0 = 0.0f;
@@ -759,7 +768,7 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
     G.rh = 1.0 / J.h;
     G.ihsq = 1. / (J.h * J.h);
     G.idqsq = 1. / (J.dq * J.dq);
-    G.rowt = geo_lds ? sh.geol : gbuf;
+    G.rowt = (GeoPtr)sh.geol;
     G.layt = G.rowt + geo_rows_doubles(J.ny);
     G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;
     int x1 = J.box[0][0], x2 = J.box[0][1], y1 = J.box[1][0], y2 = J.box[1][1], z1 = J.box[2][0], z2 = J.box[2][1];

@@ -213,3 +213,18 @@ def test_captured_launch_replays(ctx):
         ctx.release(other)
     for m in models:
         ctx.release_model(m)
+
+
+def test_grid_beyond_geometry_lds_is_refused(ctx):
+    """A grid whose geometry row/layer tables do not fit the wide build's LDS
+    (7 (ny + 2) + 6 (nz + 2) > 6800 doubles, e.g. 1000 rows) is refused
+    before any device work, with FDGPU_EINPUT."""
+    from fdtomoc_amd import fdgpu
+    nx, ny, nz = 3, 1000, 6
+    v = np.full((nz, ny, nx), 6.0, dtype=np.float32)
+    m = ctx.model(v)
+    par = dict(nx=nx, ny=ny, nz=nz, h=1.0, x0=120.0, y0=24.0, z0=0.0, fxs=120.0, fys=23.99, fzs=0.0)
+    with pytest.raises(fdgpu.FdgpuError) as e:
+        ctx.solve([fdgpu.TableSpec(par, m)], fdgpu.BATCHED)
+    assert e.value.code == -5
+    ctx.release_model(m)

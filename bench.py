@@ -56,10 +56,10 @@ from fdtomoc_amd import autotune, fdgpu, synth, tables, workloads  # noqa: E402
 METRIC = "stations/s & nodes/s batched vs plain at <=1e-4 s err; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VISIT = 12          # slowness read + time read + time write per node visit
-# HBM bytes (FETCH_SIZE x 1.99993 + WRITE_SIZE x 1.0, copy-calibrated) that
+# HBM bytes (FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0, copy-calibrated) that
 # rocprofv3 counted for one launch of the default 96-table workload
 # (profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md)
-TRAFFIC_BYTES_C2 = 342.71e9
+TRAFFIC_BYTES_C2 = 323.92e9
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 
 

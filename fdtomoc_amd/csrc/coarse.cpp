@@ -23,9 +23,9 @@ using namespace fdgpu_detail;
         if (_e != hipSuccess) return hip_fail(_e);    \
     } while (0)
 
-namespace {
+namespace fdgpu_detail {
 
-int check(const fdgpu_coarse *c, int &nx, int &ny, int &nz) {
+int coarse_check(const fdgpu_coarse *c, int &nx, int &ny, int &nz) {
     if (!c || !c->igridx || !c->igridy || !c->igridz || !c->velocity) return FDGPU_EINPUT;
     if (c->nxc < 2 || c->nyc < 2 || c->nzc < 2) return FDGPU_EINPUT;
     if (!(c->h > 0.0)) return FDGPU_EINPUT;
@@ -40,6 +40,22 @@ int check(const fdgpu_coarse *c, int &nx, int &ny, int &nz) {
     return FDGPU_OK;
 }
 
+void coarse_coords(const fdgpu_coarse *co, std::vector<float> &g) {
+    // c2f.c:258-277: node coordinates in float, accumulated from the doubles
+    g.assign((size_t)(co->nxc + co->nyc + co->nzc), 0.f);
+    float *gx = g.data(), *gy = gx + co->nxc, *gz = gy + co->nyc;
+    gx[0] = (float)co->x0;
+    gy[0] = (float)co->y0;
+    gz[0] = (float)co->z0;
+    for (int i = 1; i < co->nxc; i++) gx[i] = (float)(gx[i - 1] + co->h * co->igridx[i - 1]);
+    for (int i = 1; i < co->nyc; i++) gy[i] = (float)(gy[i - 1] + co->h * co->igridy[i - 1]);
+    for (int i = 1; i < co->nzc; i++) gz[i] = (float)(gz[i - 1] + co->h * co->igridz[i - 1]);
+}
+
+}  // namespace fdgpu_detail
+
+namespace {
+
 float bits_float(unsigned u) {
     float f;
     memcpy(&f, &u, 4);
@@ -49,15 +65,8 @@ float bits_float(unsigned u) {
 // Runs the kernels into the two models' slowness grids; sets their velocity
 // ranges and copies the fine velocities to `fine` when it is not null.
 int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Model &mp, Model &ms, float *fine) {
-    // c2f.c:258-277: node coordinates in float, accumulated from the doubles
-    std::vector<float> g((size_t)(co->nxc + co->nyc + co->nzc));
-    float *gx = g.data(), *gy = gx + co->nxc, *gz = gy + co->nyc;
-    gx[0] = (float)co->x0;
-    gy[0] = (float)co->y0;
-    gz[0] = (float)co->z0;
-    for (int i = 1; i < co->nxc; i++) gx[i] = (float)(gx[i - 1] + co->h * co->igridx[i - 1]);
-    for (int i = 1; i < co->nyc; i++) gy[i] = (float)(gy[i - 1] + co->h * co->igridy[i - 1]);
-    for (int i = 1; i < co->nzc; i++) gz[i] = (float)(gz[i - 1] + co->h * co->igridz[i - 1]);
+    std::vector<float> g;
+    coarse_coords(co, g);
     const long long ncoarse = 2LL * co->nxc * co->nyc * co->nzc;
     const long long nax = (long long)nx + ny + nz;
     const long long n = (long long)nx * ny * nz;
@@ -125,13 +134,13 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
 
 int fdgpu_coarse_dims(const fdgpu_coarse *coarse, int *nx, int *ny, int *nz) {
     if (!nx || !ny || !nz) return FDGPU_EINPUT;
-    return check(coarse, *nx, *ny, *nz);
+    return coarse_check(coarse, *nx, *ny, *nz);
 }
 
 int fdgpu_models_from_coarse(fdgpu_ctx *c, const fdgpu_coarse *coarse, int *model_p, int *model_s, float *fine) {
     if (!c || !model_p || !model_s) return FDGPU_EINPUT;
     int nx, ny, nz;
-    int rc = check(coarse, nx, ny, nz);
+    int rc = coarse_check(coarse, nx, ny, nz);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
     if (c->serve.on) return FDGPU_EBUSY;   // no allocation while the request loop runs
@@ -165,7 +174,7 @@ int fdgpu_models_update_from_coarse(fdgpu_ctx *c, int model_p, int model_s, cons
                                     float *fine) {
     if (!c || model_p == model_s) return FDGPU_EINPUT;
     int nx, ny, nz;
-    int rc = check(coarse, nx, ny, nz);
+    int rc = coarse_check(coarse, nx, ny, nz);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
     if (c->serve.on) return FDGPU_EBUSY;

@@ -15,40 +15,16 @@
 // separately, as in the reference's x86-64 build.
 #include <hip/hip_runtime.h>
 
-#include "coarse_kernels.h"
+#include "coarse_core.h"
 
 namespace {
-
-// The coarse cell holding v on axis g (c2f.c:675-683): the first node
-// strictly beyond v, clamped to the last interval, minus one.
-__device__ int coarse_cell(const float *g, int n, float v) {
-    int i;
-    for (i = 1; i < n; i++)
-        if (g[i] > v) break;
-    if (i == n) i--;
-    return i - 1;
-}
 
 __global__ void c2f_prep_kernel(const C2fGrid grid, float *coarse, long long ncoarse, C2fAxis *axis) {
     const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = t0; q < ncoarse; q += stride) coarse[q] = (float)(1. / (double)coarse[q]);
+    for (long long q = t0; q < ncoarse; q += stride) coarse[q] = c2fdev::slowness(coarse[q]);
     const long long nax = (long long)grid.nx + grid.ny + grid.nz;
-    for (long long q = t0; q < nax; q += stride) {
-        // axis 0: x (c2f.c:566), 1: y (c2f.c:563), 2: z (c2f.c:560)
-        const int a = q < grid.nx ? 0 : q < grid.nx + grid.ny ? 1 : 2;
-        const int i = (int)(a == 0 ? q : a == 1 ? q - grid.nx : q - grid.nx - grid.ny);
-        const double o = a == 0 ? grid.x0 : a == 1 ? grid.y0 : grid.z0;
-        const float *g = a == 0 ? grid.gx : a == 1 ? grid.gy : grid.gz;
-        const int nc = a == 0 ? grid.nxc : a == 1 ? grid.nyc : grid.nzc;
-        const float v = (float)(o + i * grid.h);
-        C2fAxis e;
-        e.cell = coarse_cell(g, nc, v);
-        const float gi = g[e.cell];
-        const float hg = g[e.cell + 1] - gi;
-        e.f = (v - gi) / hg;
-        axis[q] = e;
-    }
+    for (long long q = t0; q < nax; q += stride) axis[q] = c2fdev::axis_entry(grid, q);
 }
 
 // Block-wide min/max of positive finite velocities (as ordered bit patterns)
@@ -86,24 +62,9 @@ __global__ void c2f_fine_kernel(const C2fGrid grid, const float *__restrict__ sl
         const unsigned r = q / nx, i = q - r * nx;
         const unsigned k = r / ny, j = r - k * ny;
         const C2fAxis ex = axis[i], ey = axis[nx + j], ez = axis[nx + ny + k];
-        const int ci = ex.cell, cj = ey.cell, ck = ez.cell;
-        const int nk = grid.nxc * grid.nyc * ck;
-        const int nj = grid.nxc * cj;
-        const int nk2 = grid.nxc * grid.nyc * (ck + 1);
-        const int nj2 = grid.nxc * (cj + 1);
-        const float fx = ex.f, fy = ey.f, fz = ez.f;
 #pragma unroll
         for (int ph = 0; ph < 2; ph++) {
-            const float *s = sl + ph * nxyzc + ci;
-            float tv = 0.f;
-            tv = (1.f - fx) * (1.f - fy) * (1.f - fz) * s[nk + nj];
-            tv = tv + fx * (1.f - fy) * (1.f - fz) * s[nk + nj + 1];
-            tv = tv + (1.f - fx) * fy * (1.f - fz) * s[nk + nj2];
-            tv = tv + (1.f - fx) * (1.f - fy) * fz * s[nk2 + nj];
-            tv = tv + fx * fy * (1.f - fz) * s[nk + nj2 + 1];
-            tv = tv + fx * (1.f - fy) * fz * s[nk2 + nj + 1];
-            tv = tv + (1.f - fx) * fy * fz * s[nk2 + nj2];
-            tv = tv + fx * fy * fz * s[nk2 + nj2 + 1];
+            const float tv = c2fdev::trilinear(grid, sl + ph * nxyzc, ex, ey, ez);
             const float vel = 1.f / tv;                     // c2f.c:570
             if (vel_out) vel_out[(size_t)ph * n + q] = vel;
             (ph == 0 ? slow_p : slow_s)[q] = 1.0f / vel;    // sphfd.c:689-690

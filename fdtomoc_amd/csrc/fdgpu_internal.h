@@ -184,6 +184,10 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots);
 fdprep::ParValues to_values(const fdgpu_par &p);
 // Frees every cached geometry table (stream idle).
 void clear_geos(fdgpu_ctx *c);
+// A coarse model's validity and fine dimensions (src/c2f.c:262-277), and its
+// node coordinates gx | gy | gz in float (c2f.c:258-277).
+int coarse_check(const fdgpu_coarse *c, int &nx, int &ny, int &nz);
+void coarse_coords(const fdgpu_coarse *c, std::vector<float> &g);
 // Host preparation of one table request (validation, geometry, source); `preset`
 // receives the srctype 2/3 initial grid.
 int prepare_table(const fdgpu_table_desc &d, int model_nx, int model_ny, int model_nz, Table &T,

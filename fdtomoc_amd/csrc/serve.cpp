@@ -46,12 +46,29 @@ namespace {
 
 constexpr double kIdleSeconds = 0.25;
 
-// Per-request device block: time | sinq | stats | stamps | last_full | geometry table | slowness
+// Per-request device block: time | sinq | stats | stamps | last_full |
+// geometry tables | slowness (own velocities or a coarse model) | coarse
+// slowness | coarse node coordinates | axis entries (coarse model)
 struct DevLayout {
-    size_t time, sinq, stats, stamp, lastf, geo, slow, total;
+    size_t time, sinq, stats, stamp, lastf, geo, slow, cslow, cg, caxis, total;
 };
 
-DevLayout dev_layout(const fdgpu_par &p, bool own_vel) {
+// Sizes of a coarse request's extra parts (zero for other requests).
+struct CoarseSizes {
+    long long ncoarse = 0;   // coarse nodes of one phase
+    int ncg = 0;             // coarse node coordinates
+    long long nax = 0;       // axis entries
+};
+
+CoarseSizes coarse_sizes(const fdgpu_coarse &co, const fdgpu_par &p) {
+    CoarseSizes cs;
+    cs.ncoarse = (long long)co.nxc * co.nyc * co.nzc;
+    cs.ncg = co.nxc + co.nyc + co.nzc;
+    cs.nax = (long long)p.nx + p.ny + p.nz;
+    return cs;
+}
+
+DevLayout dev_layout(const fdgpu_par &p, bool own_vel, const CoarseSizes &cs = CoarseSizes()) {
     const long long n = (long long)p.nx * p.ny * p.nz;
     const int maxdim = std::max(p.nx, std::max(p.ny, p.nz));
     DevLayout L;
@@ -63,16 +80,21 @@ DevLayout dev_layout(const fdgpu_par &p, bool own_vel) {
     L.lastf = o; o += align256(sizeof(int) * 6 * (size_t)maxdim);
     L.geo = o; o += align256(sizeof(double) * (size_t)fdsweep::geo_doubles(p.ny, p.nz));
     L.slow = o; if (own_vel) o += align256(sizeof(float) * (size_t)n);
+    L.cslow = o; o += cs.ncoarse ? align256(sizeof(float) * (size_t)cs.ncoarse) : 0;
+    L.cg = o; o += cs.ncg ? align256(sizeof(float) * (size_t)cs.ncg) : 0;
+    L.caxis = o; o += cs.nax ? align256(sizeof(C2fAxis) * (size_t)cs.nax) : 0;
     L.total = o;
     return L;
 }
 
-// Per-request staging block: sinq | init_idx | init_dist | init_val | velocity | preset
+// Per-request staging block: sinq | init_idx | init_dist | init_val | velocity
+// | preset | coarse velocities | coarse node coordinates
 struct StageLayout {
-    size_t sinq, idx, dist, val, vel, preset, total;
+    size_t sinq, idx, dist, val, vel, preset, cvel, cg, total;
 };
 
-StageLayout stage_layout(const fdprep::Prepared &g, long long n, bool own_vel, bool preset) {
+StageLayout stage_layout(const fdprep::Prepared &g, long long n, bool own_vel, bool preset,
+                         const CoarseSizes &cs = CoarseSizes()) {
     StageLayout L;
     size_t o = 0;
     L.sinq = o; o += align256(sizeof(double) * g.sinq.size());
@@ -81,6 +103,8 @@ StageLayout stage_layout(const fdprep::Prepared &g, long long n, bool own_vel, b
     L.val = o; o += align256(sizeof(float) * g.init_val.size());
     L.vel = o; if (own_vel) o += align256(sizeof(float) * (size_t)n);
     L.preset = o; if (preset) o += align256(sizeof(float) * (size_t)n);
+    L.cvel = o; o += cs.ncoarse ? align256(sizeof(float) * (size_t)cs.ncoarse) : 0;
+    L.cg = o; o += cs.ncg ? align256(sizeof(float) * (size_t)cs.ncg) : 0;
     L.total = o;
     return L;
 }
@@ -246,7 +270,12 @@ int fdgpu_serve_start(fdgpu_ctx *c, const fdgpu_serve_config *cfg) {
     return FDGPU_OK;
 }
 
-int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *velocity, int *request) {
+namespace {
+
+// One request: a window of a resident model, or the request's own velocities,
+// or one phase of a coarse model interpolated on the device.
+int submit_impl(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *velocity, const fdgpu_coarse *coarse,
+                int phase, int *request) {
     if (!c || !d || !request) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
     Serve &S = c->serve;
@@ -255,7 +284,14 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
     const fdgpu_par &p = d->par;
     const Model *m = nullptr;
     int mnx = p.nx, mny = p.ny, mnz = p.nz;
-    if (velocity) {
+    CoarseSizes cs;
+    if (coarse) {
+        int cx, cy, cz;
+        if (coarse_check(coarse, cx, cy, cz) || cx != p.nx || cy != p.ny || cz != p.nz) return FDGPU_EINPUT;
+        if (phase != 0 && phase != 1) return FDGPU_EINPUT;
+        cs = coarse_sizes(*coarse, p);
+    }
+    if (velocity || coarse) {
         if (d->ox || d->oy || d->oz) return FDGPU_EINPUT;
     } else {
         auto it = c->models.find(d->model);
@@ -272,9 +308,9 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
         (void)ensure_running(c);
         if ((long long)S.pending.size() >= S.cap) return FDGPU_EBUSY;
     }
-    const bool own = velocity != nullptr;
-    const DevLayout DL = dev_layout(p, own);
-    const StageLayout SL = stage_layout(T.prep, T.n, own, p.srctype != 1);
+    const bool own = velocity != nullptr || coarse != nullptr;   // the request brings its own slowness
+    const DevLayout DL = dev_layout(p, own, cs);
+    const StageLayout SL = stage_layout(T.prep, T.n, velocity != nullptr, p.srctype != 1, cs);
     const size_t a_off = S.arena_alloc.take(DL.total);
     if (a_off == SIZE_MAX) return FDGPU_EBUSY;
     const size_t s_off = S.stage_alloc.take(SL.total);
@@ -288,8 +324,13 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
     memcpy(st + SL.idx, T.prep.init_idx.data(), sizeof(int) * T.prep.init_idx.size());
     memcpy(st + SL.dist, T.prep.init_dist.data(), sizeof(double) * T.prep.init_dist.size());
     memcpy(st + SL.val, T.prep.init_val.data(), sizeof(float) * T.prep.init_val.size());
-    double vmin = 0.0, vmax = 0.0;
-    if (own) {
+    double vmin = 0.0, vmax = 0.0;   // a coarse request has no range here: no skip (results identical)
+    if (coarse) {
+        std::vector<float> g;
+        coarse_coords(coarse, g);
+        memcpy(st + SL.cvel, coarse->velocity + phase * cs.ncoarse, sizeof(float) * (size_t)cs.ncoarse);
+        memcpy(st + SL.cg, g.data(), sizeof(float) * g.size());
+    } else if (own) {
         memcpy(st + SL.vel, velocity, sizeof(float) * (size_t)T.n);
         vmin = 1e300; vmax = -1e300;
         for (long long q = 0; q < T.n; q++) {
@@ -341,9 +382,21 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
     J.skip_ok = fdprep::skip_safe(to_values(p), T.prep, vmin, vmax) ? 1 : 0;
     J.change_budget = c->change_budget;
     J.geo_fill = (double *)(blk + DL.geo);   // filled by the workgroup that takes the request
-    job.vel_src = own ? (const float *)(st + SL.vel) : nullptr;
+    job.vel_src = velocity ? (const float *)(st + SL.vel) : nullptr;
     job.slow_dst = own ? (float *)(blk + DL.slow) : nullptr;
-    job.nvel = own ? T.n : 0;
+    job.nvel = velocity ? T.n : 0;
+    if (coarse) {
+        job.cvel_src = (const float *)(st + SL.cvel);
+        job.cg_src = (const float *)(st + SL.cg);
+        job.cslow_dst = (float *)(blk + DL.cslow);
+        job.caxis = (C2fAxis *)(blk + DL.caxis);
+        C2fGrid &g = job.cgrid;
+        g.nxc = coarse->nxc; g.nyc = coarse->nyc; g.nzc = coarse->nzc;
+        const float *gd = (const float *)(blk + DL.cg);
+        g.gx = gd; g.gy = gd + coarse->nxc; g.gz = gd + coarse->nxc + coarse->nyc;
+        g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+        g.x0 = coarse->x0; g.y0 = coarse->y0; g.z0 = coarse->z0; g.h = coarse->h;
+    }
     job.sinq_src = (const double *)(st + SL.sinq);
     job.sinq_dst = (double *)(blk + DL.sinq);
     job.nsinq = (int)T.prep.sinq.size();
@@ -365,6 +418,18 @@ int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *vel
     S.pending[seq] = id;
     *request = id;
     return ensure_running(c);
+}
+
+}  // namespace
+
+int fdgpu_serve_submit(fdgpu_ctx *c, const fdgpu_table_desc *d, const float *velocity, int *request) {
+    return submit_impl(c, d, velocity, nullptr, 0, request);
+}
+
+int fdgpu_serve_submit_coarse(fdgpu_ctx *c, const fdgpu_table_desc *d, const fdgpu_coarse *coarse, int phase,
+                              int *request) {
+    if (!coarse) return FDGPU_EINPUT;
+    return submit_impl(c, d, nullptr, coarse, phase, request);
 }
 
 int fdgpu_serve_poll(fdgpu_ctx *c, int request, int *done) {
@@ -437,9 +502,16 @@ int fdgpu_serve_write(fdgpu_ctx *c, int request, const char *path, const void *v
     return fdio::write_table(path, par, prep, vel_header, host.data());
 }
 
-int fdgpu_serve_request_bytes(const fdgpu_par *par, int own_velocity, unsigned long long *bytes) {
+int fdgpu_serve_request_bytes(const fdgpu_par *par, int own_velocity, const fdgpu_coarse *coarse,
+                              unsigned long long *bytes) {
     if (!par || !bytes || par->nx <= 0 || par->ny <= 0 || par->nz <= 0) return FDGPU_EINPUT;
-    *bytes = (unsigned long long)dev_layout(*par, own_velocity != 0).total;
+    CoarseSizes cs;
+    if (coarse) {
+        int cx, cy, cz;
+        if (coarse_check(coarse, cx, cy, cz) || cx != par->nx || cy != par->ny || cz != par->nz) return FDGPU_EINPUT;
+        cs = coarse_sizes(*coarse, *par);
+    }
+    *bytes = (unsigned long long)dev_layout(*par, own_velocity != 0 || coarse, cs).total;
     return FDGPU_OK;
 }
 

@@ -11,6 +11,7 @@
 // before the server kernel was launched.
 #pragma once
 
+#include "coarse_kernels.h"
 #include "sweep_solver.h"
 
 namespace fdsweep {
@@ -31,6 +32,14 @@ struct ServeJob {
     int nsinq;
     int pad;
     const float *preset_src;   // host: srctype 2/3 initial grid (nullable)
+    // A request that carries one phase of a coarse model instead of
+    // velocities: the workgroup interpolates its slowness into slow_dst
+    // (c2f's interpolation, src/c2f.c:549-575) before solving.
+    const float *cvel_src;     // host: nxc*nyc*nzc coarse velocities (nullable)
+    const float *cg_src;       // host: coarse node coordinates gx | gy | gz
+    float *cslow_dst;          // device: the coarse slowness
+    C2fAxis *caxis;            // device: nx + ny + nz axis entries
+    C2fGrid cgrid;             // gx/gy/gz point at the device copy of cg_src
 };
 
 // Device counters of the server (one allocation).

@@ -14,6 +14,7 @@
 // operation by operation like the reference's x86-64 build.
 #include <hip/hip_runtime.h>
 
+#include "coarse_core.h"
 #include "fd_kernels.h"
 #include "serve_shared.h"
 #include "sweep_solver.h"
@@ -104,6 +105,29 @@ __global__ void __launch_bounds__(kWideThreads, 2)
         if (sj.preset_src) {
             const long long n = (long long)sj.job.nx * sj.job.ny * sj.job.nz;
             for (long long q = tid; q < n; q += T) sj.job.time[q] = sj.preset_src[q];
+        }
+        if (sj.cvel_src) {
+            // the request's slowness from its coarse model: coarse slowness and
+            // node coordinates, per-axis cells, then every fine node (as
+            // c2f_prep_kernel and c2f_fine_kernel compute them)
+            const C2fGrid &cg = sj.cgrid;
+            const int ncg = cg.nxc + cg.nyc + cg.nzc;
+            const long long nxyzc = (long long)cg.nxc * cg.nyc * cg.nzc;
+            float *gdst = (float *)cg.gx;
+            for (int q = tid; q < ncg; q += T) gdst[q] = sj.cg_src[q];
+            for (long long q = tid; q < nxyzc; q += T) sj.cslow_dst[q] = c2fdev::slowness(sj.cvel_src[q]);
+            __syncthreads();
+            const long long nax = (long long)cg.nx + cg.ny + cg.nz;
+            for (long long q = tid; q < nax; q += T) sj.caxis[q] = c2fdev::axis_entry(cg, q);
+            __syncthreads();
+            const unsigned nx = (unsigned)cg.nx, ny = (unsigned)cg.ny, n = nx * ny * (unsigned)cg.nz;
+            for (unsigned q = tid; q < n; q += T) {
+                const unsigned r = q / nx, i = q - r * nx;
+                const unsigned k = r / ny, j = r - k * ny;
+                const float tv = c2fdev::trilinear(cg, sj.cslow_dst, sj.caxis[i], sj.caxis[nx + j], sj.caxis[nx + ny + k]);
+                const float vel = 1.f / tv;             // c2f.c:570
+                sj.slow_dst[q] = 1.0f / vel;           // sphfd.c:689-690
+            }
         }
         __syncthreads();
         fdsweep::solve_table(x, sh, sj.job, W);

@@ -143,13 +143,15 @@ def lib():
         L.fdgpu_table_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_table_device_ptr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.fdgpu_run_spec.argtypes = [C.c_char_p]
-        L.fdgpu_serve_request_bytes.argtypes = [C.POINTER(CPar), C.c_int, C.POINTER(C.c_ulonglong)]
+        L.fdgpu_serve_request_bytes.argtypes = [C.POINTER(CPar), C.c_int, C.c_void_p, C.POINTER(C.c_ulonglong)]
         L.fdgpu_coarse_dims.argtypes = [C.POINTER(CCoarse), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]
         L.fdgpu_models_from_coarse.argtypes = [C.c_void_p, C.POINTER(CCoarse), C.POINTER(C.c_int),
                                                C.POINTER(C.c_int), C.c_void_p]
         L.fdgpu_models_update_from_coarse.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(CCoarse),
                                                       C.c_void_p]
+        L.fdgpu_serve_submit_coarse.argtypes = [C.c_void_p, C.POINTER(CTable), C.POINTER(CCoarse), C.c_int,
+                                                C.POINTER(C.c_int)]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -185,10 +187,13 @@ def make_par(**kw) -> CPar:
     return p
 
 
-def serve_request_bytes(own_velocity: bool = False, **par) -> int:
-    """HBM arena bytes one serving request of these sizes takes."""
+def serve_request_bytes(own_velocity: bool = False, coarse: "Coarse | None" = None, **par) -> int:
+    """HBM arena bytes one serving request of these sizes takes (carrying its
+    own velocities, or the coarse model `coarse`)."""
     b = C.c_ulonglong()
-    _check(lib().fdgpu_serve_request_bytes(C.byref(make_par(**par)), int(own_velocity), C.byref(b)),
+    keep = []
+    cc = C.byref(coarse.to_c(keep)) if coarse is not None else None
+    _check(lib().fdgpu_serve_request_bytes(C.byref(make_par(**par)), int(own_velocity), cc, C.byref(b)),
            "fdgpu_serve_request_bytes")
     return int(b.value)
 
@@ -337,6 +342,18 @@ class Context:
         rid = C.c_int()
         _check(lib().fdgpu_serve_submit(self._h, C.byref(d), None if v is None else v.ctypes.data, C.byref(rid)),
                "fdgpu_serve_submit")
+        return rid.value
+
+    def serve_submit_coarse(self, spec: TableSpec, coarse: Coarse, phase: int) -> int:
+        """Admits one request whose velocities are phase `phase` (0 P, 1 S) of a
+        coarse model, interpolated on the device by the workgroup that takes it."""
+        d = CTable()
+        keep = []
+        self._fill(d, spec, keep)
+        c = coarse.to_c(keep)
+        rid = C.c_int()
+        _check(lib().fdgpu_serve_submit_coarse(self._h, C.byref(d), C.byref(c), int(phase), C.byref(rid)),
+               "fdgpu_serve_submit_coarse")
         return rid.value
 
     def serve_poll(self, req: int) -> bool:

@@ -182,6 +182,16 @@ int fdgpu_serve_start(fdgpu_ctx *ctx, const fdgpu_serve_config *cfg);
  * par.nx*ny*nz velocities (e.g. the model after an inversion update; origin
  * must be 0); it is copied before the call returns. */
 int fdgpu_serve_submit(fdgpu_ctx *ctx, const fdgpu_table_desc *d, const float *velocity, int *request);
+/* Admits one table whose velocities are phase `phase` (0 = P, 1 = S) of a
+ * coarse model (fdgpu_coarse, as an inversion iteration emits it): the
+ * workgroup that takes the request interpolates its slowness as c2f does
+ * (src/c2f.c:549-575, 673-727) and then solves it; the coarse model's fine
+ * grid must be the table's (origin 0).  No reference counterpart: it replaces
+ * c2f's run, its files and sphfd's re-read of them for one table.  Such a
+ * request skips no reverse-pass evaluation (no velocity range is known before
+ * the interpolation); results are identical either way. */
+int fdgpu_serve_submit_coarse(fdgpu_ctx *ctx, const fdgpu_table_desc *d, const fdgpu_coarse *coarse, int phase,
+                              int *request);
 int fdgpu_serve_poll(fdgpu_ctx *ctx, int request, int *done);
 /* Waits until the request is finished (timeout_s < 0: no limit; FDGPU_ETIMEOUT). */
 int fdgpu_serve_wait(fdgpu_ctx *ctx, int request, double timeout_s);
@@ -189,9 +199,12 @@ int fdgpu_serve_read(fdgpu_ctx *ctx, int request, float *host, unsigned long lon
 int fdgpu_serve_stats(fdgpu_ctx *ctx, int request, int *stats, int nstats);
 /* Writes a finished request's table as fdgpu_table_write does. */
 int fdgpu_serve_write(fdgpu_ctx *ctx, int request, const char *path, const void *vel_header);
-/* HBM arena bytes one request with these sizes takes (own_velocity: it
- * carries its own velocities), for sizing cfg->arena_bytes. */
-int fdgpu_serve_request_bytes(const fdgpu_par *par, int own_velocity, unsigned long long *bytes);
+/* HBM arena bytes one request with these sizes takes, for sizing
+ * cfg->arena_bytes: own_velocity, it carries its own velocities; coarse (or
+ * NULL), it carries this coarse model (fdgpu_serve_submit_coarse; implies
+ * own_velocity). */
+int fdgpu_serve_request_bytes(const fdgpu_par *par, int own_velocity, const fdgpu_coarse *coarse,
+                              unsigned long long *bytes);
 /* Returns a finished request's arena block. */
 int fdgpu_serve_release(fdgpu_ctx *ctx, int request);
 /* info[0..5]: serving on, requests outstanding, arena bytes used, arena bytes,

@@ -62,3 +62,24 @@ def test_coarse_dims_and_validation_without_gpu():
         with pytest.raises(fdgpu.FdgpuError) as e:
             fdgpu.Coarse(**kw).dims()
         assert e.value.code == -5
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_serve_request_bytes_without_gpu():
+    """fdgpu_serve_request_bytes (host only): a request that carries its own
+    velocities takes at least one more grid of floats; one that carries a
+    coarse model takes that, plus its coarse slownesses, coordinates and axis
+    entries; a coarse model whose fine grid is not the table's is refused."""
+    import numpy as np
+    from fdtomoc_amd import fdgpu
+    vel = np.ones(2 * 4 * 3 * 2, dtype=np.float32)
+    c = fdgpu.Coarse(4, 3, 2, np.array([2, 3, 1]), np.array([4, 1]), np.array([5]), 0.0, 0.0, 0.0, 1.0, vel)
+    nx, ny, nz = c.dims()
+    base = fdgpu.serve_request_bytes(nx=nx, ny=ny, nz=nz)
+    own = fdgpu.serve_request_bytes(True, nx=nx, ny=ny, nz=nz)
+    co = fdgpu.serve_request_bytes(coarse=c, nx=nx, ny=ny, nz=nz)
+    assert own >= base + 4 * nx * ny * nz
+    assert co >= own + 4 * 4 * 3 * 2 + 4 * (4 + 3 + 2) + 8 * (nx + ny + nz)
+    with pytest.raises(fdgpu.FdgpuError) as e:
+        fdgpu.serve_request_bytes(coarse=c, nx=nx + 1, ny=ny, nz=nz)
+    assert e.value.code == -5

@@ -166,3 +166,50 @@ def test_idle_exit_relaunch_and_stop_drains(ctx):
     assert _bits(ctx.read(tid, s1.shape), ob.solve(p1, ob.slowness(s1))[0]) == 0
     ctx.release(tid)
     ctx.release_model(m)
+
+
+def test_coarse_model_requests(ctx):
+    """Requests that carry one phase of a coarse model (an inversion update)
+    arrive while a fine-velocity request runs: the workgroup that takes each
+    interpolates the slowness on the device; every result equals the oracle
+    chain (c2f, then sphfd) and the same velocities submitted as fine
+    velocities, bit for bit.  A coarse model whose fine grid is not the
+    table's, or a phase other than 0/1, is refused."""
+    from fdtomoc_amd import fdgpu, synth
+    rng = np.random.default_rng(61)
+    nxc, nyc, nzc = 6, 5, 4
+    vp = rng.uniform(4.5, 7.5, size=nxc * nyc * nzc).astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    co = fdgpu.Coarse(nxc, nyc, nzc, np.full(nxc - 1, 6), np.array([7, 6, 6, 5]), np.array([5, 5, 4]),
+                      0.0, 0.0, 0.0, 1.0, np.concatenate([vp, vs]))
+    nx, ny, nz = co.dims()
+    (_, _, _), fine = ob.c2f(nxc, nyc, nzc, co.igridx, co.igridy, co.igridz, 0.0, 0.0, 0.0, 1.0, co.velocity)
+    fine = fine.reshape(2, nz, ny, nx)
+    g = synth.centred_grid(nx, ny, nz, 1.0)
+    st = synth.surface_node_station(g, rng, margin=2)
+    par = ob.Par(nx=nx, ny=ny, nz=nz, h=1.0, x0=g.x0, y0=g.y0, z0=g.z0, fxs=st.lon, fys=st.lat, fzs=st.depth,
+                 reverse=50)
+    spec = fdgpu.TableSpec(par_dict(par), -1)
+    ctx.serve_start((nx, ny, nz))
+    try:
+        rf = ctx.serve_submit(spec, fine[1])            # S as fine velocities
+        rp = ctx.serve_submit_coarse(spec, co, 0)       # P and S as the coarse model
+        rs = ctx.serve_submit_coarse(spec, co, 1)
+        small = fdgpu.Coarse(nxc, nyc, nzc, np.full(nxc - 1, 5), co.igridy, co.igridz, 0.0, 0.0, 0.0, 1.0,
+                             co.velocity)
+        for bad, ph in ((small, 0), (co, 2)):
+            with pytest.raises(fdgpu.FdgpuError) as e:
+                ctx.serve_submit_coarse(spec, bad, ph)
+            assert e.value.code == -5
+        for r in (rf, rp, rs):
+            ctx.serve_wait(r, 120.0)
+        t = {r: ctx.serve_read(r, (nz, ny, nx)) for r in (rf, rp, rs)}
+        for r in (rf, rp, rs):
+            ctx.serve_release(r)
+    finally:
+        ctx.serve_stop()
+    t_p = ob.solve(par, ob.slowness(fine[0]))[0]
+    t_s = ob.solve(par, ob.slowness(fine[1]))[0]
+    assert _bits(t[rp], t_p) == 0
+    assert _bits(t[rs], t_s) == 0
+    assert _bits(t[rf], t[rs]) == 0

@@ -29,7 +29,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = [("sweep_kernel.hip", True), ("coarse_kernel.hip", True), ("fdgpu.cpp", False), ("sphfd_io.cpp", False),
-           ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False), ("coarse.cpp", False)]
+           ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False), ("coarse.cpp", False),
+           ("loc_kernel.hip", True), ("loc.cpp", False)]
 
 
 def _run(cmd, verbose):

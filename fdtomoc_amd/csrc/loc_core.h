@@ -1,0 +1,166 @@
+// loc_core.h -- device functions of sphfdloc's grid-search location
+// (src/sphfdloc.c:752-1052, find_time 1305-1345): the trilinear table time,
+// a trial point's weighted residual statistics, and the two refinement
+// boxes.  Double table times and weights, float residuals and sums, in the
+// reference's operation order; compiled without FP contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "loc_kernels.h"
+
+namespace locdev {
+
+constexpr unsigned long long kNone = ~0ull;
+
+// find_time's cell (sphfdloc.c:1306-1335): the clamped cell holding the
+// point, as the offset of its low corner, and the eight corner weights in the
+// reference's term order, each the product of its three factors left to
+// right.  The weights depend only on the point, so a trial computes them
+// once for all of its picks.
+struct Cell {
+    long long base;
+    double w[8];
+};
+
+__device__ __forceinline__ Cell cell_of(const LocGrid &g, double x, double y, double z) {
+    int i = (int)((x - g.x0) / g.df);
+    int j = (int)((y - g.y0) / g.dq);
+    int k = (int)((z - g.z0) / g.h);
+    i = max(0, min(i, g.nx - 2));
+    j = max(0, min(j, g.ny - 2));
+    k = max(0, min(k, g.nz - 2));
+    const double fx = (x - (g.x0 + g.df * i)) / g.df;
+    const double fy = (y - (g.y0 + g.dq * j)) / g.dq;
+    const double fz = (z - (g.z0 + g.h * k)) / g.h;
+    const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
+    Cell c;
+    c.base = (long long)g.nx * g.ny * k + (long long)g.nx * j + i;
+    c.w[0] = gx * gy * gz;
+    c.w[1] = fx * gy * gz;
+    c.w[2] = gx * fy * gz;
+    c.w[3] = gx * gy * fz;
+    c.w[4] = fx * fy * gz;
+    c.w[5] = fx * gy * fz;
+    c.w[6] = gx * fy * fz;
+    c.w[7] = fx * fy * fz;
+    return c;
+}
+
+// The table time at the cell's point: the eight terms summed in order
+// (sphfdloc.c:1337-1344).
+__device__ __forceinline__ double interp(const float *__restrict__ t, const Cell &c, long long sy, long long sz) {
+    const float *p = t + c.base;
+    double tp = c.w[0] * (double)p[0];
+    tp = tp + c.w[1] * (double)p[1];
+    tp = tp + c.w[2] * (double)p[sy];
+    tp = tp + c.w[3] * (double)p[sz];
+    tp = tp + c.w[4] * (double)p[sy + 1];
+    tp = tp + c.w[5] * (double)p[sz + 1];
+    tp = tp + c.w[6] * (double)p[sz + sy];
+    tp = tp + c.w[7] * (double)p[sz + sy + 1];
+    return tp;
+}
+
+// A pick's residual at a trial point from its table time (sphfdloc.c:768-772).
+__device__ __forceinline__ float residual(const LocGrid &g, const LocPick &p, double t) {
+    if (p.scale) t *= (double)g.vpvs;
+    return (float)((double)p.obs - t);
+}
+
+struct Sums {
+    float avres, rsq, facs;
+};
+
+// A usable pick's terms (sphfdloc.c:773-781).
+__device__ __forceinline__ void add(Sums &s, float r, float w) {
+    const float r1 = r * w;
+    const float r2 = r1 * r;
+    s.avres += r1;
+    s.rsq += r2;
+    s.facs += w;
+}
+
+// The trial's deviation (sphfdloc.c:786-790).
+__device__ __forceinline__ float deviation(const Sums &s, int ngood) {
+    const float avwt = s.facs / (float)ngood;
+    const float var = (s.rsq - s.avres * s.avres / s.facs) / ((float)(ngood - 4) * avwt);
+    return sqrtf(fabsf(var));
+}
+
+// Ordering key of a trial that beats `bound` (the strict comparison of
+// sphfdloc.c:793): deviations are +0 or positive, so their bit patterns
+// order as their values, and among equal deviations the lower scan index
+// (the one the reference's loops reach first) wins.
+__device__ __forceinline__ unsigned long long key_of(float dev, float bound, unsigned idx) {
+    return dev < bound ? ((unsigned long long)__float_as_uint(dev) << 32) | idx : kNone;
+}
+
+// A refinement box: n[a] points per axis from o[a] in steps d[a].
+struct Box {
+    double o[3], d[3];
+    int n[3];
+};
+
+// The first refinement's box around the node (sphfdloc.c:844-884).
+__device__ __forceinline__ Box box_first(const LocGrid &g, const int node[3]) {
+    const int n1 = g.ndiv + 1, n2 = n1 + g.ndiv;
+    const double org[3] = {g.x0, g.y0, g.z0}, sp[3] = {g.df, g.dq, g.h};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    Box b;
+    for (int a = 0; a < 3; a++) {
+        const int lo = a == 2 ? g.kmin : 0;
+        int s = node[a] - 1;
+        b.n[a] = n2;
+        if (s < lo) {
+            s = node[a];
+            b.n[a] = n1;
+        }
+        if (node[a] == nn[a] - 1) b.n[a] = n1 - 1;
+        b.o[a] = org[a] + sp[a] * s;
+        b.d[a] = sp[a] / g.ndiv;
+    }
+    return b;
+}
+
+// The second refinement's box around the hypocentre (sphfdloc.c:958-1001);
+// the far faces are the grid's far corner rounded to float (502-504).
+__device__ __forceinline__ Box box_second(const LocGrid &g, const double e[3]) {
+    const int n1 = g.ndiv2 + 1, n2 = n1 + g.ndiv2;
+    const double sp[3] = {g.df, g.dq, g.h};
+    const float far[3] = {(float)(g.x0 + (g.nx - 1) * g.df), (float)(g.y0 + (g.ny - 1) * g.dq),
+                          (float)(g.z0 + (g.nz - 1) * g.h)};
+    const double low[3] = {g.x0, g.y0, g.z0 + g.kmin * g.h};
+    Box b;
+    for (int a = 0; a < 3; a++) {
+        const double d1 = sp[a] / g.ndiv;
+        b.o[a] = e[a] - d1;
+        b.n[a] = n2;
+        if (b.o[a] <= low[a]) {
+            b.o[a] = e[a];
+            b.n[a] = n1;
+        }
+        if (e[a] + d1 > (double)far[a]) b.n[a] = n1 - 1;
+        b.d[a] = d1 / g.ndiv2;
+    }
+    return b;
+}
+
+__device__ __forceinline__ unsigned box_points(const Box &b) {
+    if (b.n[0] <= 0 || b.n[1] <= 0 || b.n[2] <= 0) return 0u;
+    return (unsigned)b.n[0] * (unsigned)b.n[1] * (unsigned)b.n[2];
+}
+
+// Point q of a box in the reference's loop order (z outer, x inner); each
+// coordinate is the box origin plus step times count (sphfdloc.c:887-892).
+__device__ __forceinline__ void box_point(const Box &b, unsigned q, double p[3]) {
+    const unsigned i = q % (unsigned)b.n[0];
+    const unsigned r = q / (unsigned)b.n[0];
+    const unsigned j = r % (unsigned)b.n[1];
+    const unsigned k = r / (unsigned)b.n[1];
+    p[0] = b.o[0] + b.d[0] * (int)i;
+    p[1] = b.o[1] + b.d[1] * (int)j;
+    p[2] = b.o[2] + b.d[2] * (int)k;
+}
+
+}  // namespace locdev

@@ -1,0 +1,193 @@
+// loc_kernel.hip -- gfx950 kernels of sphfdloc's grid-search location
+// (src/sphfdloc.c:752-1052, find_time 1305-1345) over resident tables.
+//
+// One search pass of a batch of events is seven launches on one stream:
+//   loc_init_kernel     per event: clear the running minimum
+//   loc_nodes_kernel    nodes of layers kmin.. across the lanes, every event of
+//                       the batch in turn: a pick's table value is one
+//                       coalesced load, and the events that share stations
+//                       re-read the same node tile from cache
+//   loc_commit_kernel   per event: decode the winning trial, evaluate it again
+//                       (residuals and computed times of every pick, the mean
+//                       residual), move the hypocentre
+//   loc_refine_kernel   refinement points across the lanes, one grid row of
+//                       blocks per event: trilinear times from the tables
+//   (commit, refine, commit again for the second refinement)
+// Every block folds its best trial into the event's key with one 64-bit
+// atomic minimum; the key orders by deviation, then by scan index, so the
+// result is the reference's sequential first minimum whatever the order in
+// which blocks run.
+//
+// Compiled with -ffp-contract=off: every multiply and add rounds separately,
+// as in the reference's x86-64 build.
+#include <hip/hip_runtime.h>
+
+#include "loc_core.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// Block-wide minimum of the lanes' keys, folded into *dst.
+__device__ __forceinline__ void fold_min(unsigned long long key, unsigned long long *dst) {
+    __shared__ unsigned long long part[kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)key, off);
+        const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(key >> 32), off);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        key = o < key ? o : key;
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; w++) key = part[w] < key ? part[w] : key;
+        // the key only moves down: skip the atomic when it cannot lower it
+        if (key != locdev::kNone &&
+            key < __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(dst, key);
+    }
+    __syncthreads();
+}
+
+__global__ void loc_init_kernel(const int *__restrict__ active, int nactive, LocState *state) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nactive) return;
+    LocState &s = state[active[a]];
+    s.key = locdev::kNone;
+    s.stdmin = __builtin_inff();   // (float)DBL_MAX
+    s.avrmin = 0.0f;
+    for (int q = 0; q < 3; q++) {
+        s.node[q] = 0;
+        s.e[q] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void loc_nodes_kernel(const LocGrid g, const LocPick *__restrict__ picks,
+                                                             const LocEvent *__restrict__ events,
+                                                             const int *__restrict__ active, int nactive,
+                                                             LocState *state) {
+    const long long nxy = (long long)g.nx * g.ny;
+    const long long n0 = nxy * g.kmin;
+    const long long n = n0 + (long long)blockIdx.x * kThreads + threadIdx.x;
+    const bool in = n < nxy * g.nz;
+    for (int a = 0; a < nactive; a++) {
+        const int ev = active[a];
+        const LocEvent E = events[ev];
+        unsigned long long key = locdev::kNone;
+        if (in) {
+            locdev::Sums s = {0.0f, 0.0f, 0.0f};
+            for (int p = E.first; p < E.first + E.npick; p++) {
+                const LocPick P = picks[p];
+                if (!P.good) continue;
+                locdev::add(s, locdev::residual(g, P, (double)P.table[n]), P.pwt);
+            }
+            key = locdev::key_of(locdev::deviation(s, E.ngood), __builtin_inff(), (unsigned)(n - n0));
+        }
+        fold_min(key, &state[ev].key);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void loc_refine_kernel(const LocGrid g, int second,
+                                                              const LocPick *__restrict__ picks,
+                                                              const LocEvent *__restrict__ events,
+                                                              const int *__restrict__ active, LocState *state) {
+    const int ev = active[blockIdx.y];
+    const LocEvent E = events[ev];
+    const LocState &st = state[ev];
+    const locdev::Box b = second ? locdev::box_second(g, st.e) : locdev::box_first(g, st.node);
+    const unsigned npts = locdev::box_points(b);
+    if ((unsigned)blockIdx.x * kThreads >= npts) return;   // the whole block: beyond this event's box
+    const unsigned q = blockIdx.x * kThreads + threadIdx.x;
+    unsigned long long key = locdev::kNone;
+    if (q < npts) {
+        double pt[3];
+        locdev::box_point(b, q, pt);
+        const locdev::Cell c = locdev::cell_of(g, pt[0], pt[1], pt[2]);
+        const long long sy = g.nx, sz = (long long)g.nx * g.ny;
+        locdev::Sums s = {0.0f, 0.0f, 0.0f};
+        for (int p = E.first; p < E.first + E.npick; p++) {
+            const LocPick P = picks[p];
+            if (!P.good) continue;
+            locdev::add(s, locdev::residual(g, P, locdev::interp(P.table, c, sy, sz)), P.pwt);
+        }
+        key = locdev::key_of(locdev::deviation(s, E.ngood), st.stdmin, q);
+    }
+    fold_min(key, &state[ev].key);
+}
+
+// Evaluates the winning trial again for every pick (sphfdloc.c:793-803):
+// the same functions in the same order give the same bits as the search.
+__global__ void loc_commit_kernel(const LocGrid g, int stage, const LocPick *__restrict__ picks,
+                                  const LocEvent *__restrict__ events, const int *__restrict__ active, int nactive,
+                                  LocState *state, float *__restrict__ res, float *__restrict__ tps) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nactive) return;
+    const int ev = active[a];
+    const LocEvent E = events[ev];
+    LocState st = state[ev];
+    const unsigned long long key = st.key;
+    const long long sy = g.nx, sz = (long long)g.nx * g.ny;
+    long long node_n = 0;
+    locdev::Cell c;
+    if (stage == 0) {
+        int nd[3] = {0, 0, 0};
+        if (key != locdev::kNone) {
+            node_n = sz * g.kmin + (long long)(unsigned)key;
+            nd[0] = (int)(node_n % g.nx);
+            nd[1] = (int)((node_n / g.nx) % g.ny);
+            nd[2] = (int)(node_n / sz);
+        }
+        const double org[3] = {g.x0, g.y0, g.z0}, sp[3] = {g.df, g.dq, g.h};
+        for (int q = 0; q < 3; q++) {
+            st.node[q] = nd[q];
+            st.e[q] = org[q] + nd[q] * sp[q];
+        }
+    } else if (key != locdev::kNone) {
+        const locdev::Box b = stage == 2 ? locdev::box_second(g, st.e) : locdev::box_first(g, st.node);
+        double pt[3];
+        locdev::box_point(b, (unsigned)key, pt);
+        for (int q = 0; q < 3; q++) st.e[q] = pt[q];
+        c = locdev::cell_of(g, pt[0], pt[1], pt[2]);
+    }
+    if (key != locdev::kNone) {
+        locdev::Sums s = {0.0f, 0.0f, 0.0f};
+        for (int p = E.first; p < E.first + E.npick; p++) {
+            const LocPick P = picks[p];
+            const double t = stage == 0 ? (double)P.table[node_n] : locdev::interp(P.table, c, sy, sz);
+            const float r = locdev::residual(g, P, t);
+            if (P.good) locdev::add(s, r, P.pwt);
+            res[p] = r;
+            tps[p] = P.obs - r;
+        }
+        st.stdmin = locdev::deviation(s, E.ngood);
+        st.avrmin = s.avres / s.facs;
+    }
+    st.key = locdev::kNone;
+    state[ev] = st;
+}
+
+}  // namespace
+
+extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_picks, const LocEvent *d_events,
+                                         const int *d_active, int nactive, LocState *d_state, float *d_res,
+                                         float *d_tps, hipStream_t stream) {
+    if (nactive <= 0) return hipSuccess;
+    const LocGrid g = *grid;
+    const long long nodes = (long long)g.nx * g.ny * (g.nz - g.kmin);
+    const int small = (nactive + 63) / 64;
+    hipLaunchKernelGGL(loc_init_kernel, dim3(small), dim3(64), 0, stream, d_active, nactive, d_state);
+    if (nodes > 0)
+        hipLaunchKernelGGL(loc_nodes_kernel, dim3((unsigned)((nodes + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           stream, g, d_picks, d_events, d_active, nactive, d_state);
+    hipLaunchKernelGGL(loc_commit_kernel, dim3(small), dim3(64), 0, stream, g, 0, d_picks, d_events, d_active,
+                       nactive, d_state, d_res, d_tps);
+    for (int stage = 1; stage <= 2; stage++) {
+        const long long side = 2LL * (stage == 1 ? g.ndiv : g.ndiv2) + 1;
+        const long long pts = side * side * side;
+        hipLaunchKernelGGL(loc_refine_kernel, dim3((unsigned)((pts + kThreads - 1) / kThreads), (unsigned)nactive),
+                           dim3(kThreads), 0, stream, g, stage == 2 ? 1 : 0, d_picks, d_events, d_active, d_state);
+        hipLaunchKernelGGL(loc_commit_kernel, dim3(small), dim3(64), 0, stream, g, stage, d_picks, d_events,
+                           d_active, nactive, d_state, d_res, d_tps);
+    }
+    return hipGetLastError();
+}

@@ -59,6 +59,32 @@ class CCoarse(C.Structure):
                 ("velocity", C.c_void_p)]
 
 
+class CLocParams(C.Structure):
+    """fdgpu_loc_params: sphfdloc's search grid and parameters."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("x0", C.c_double), ("y0", C.c_double),
+                ("z0", C.c_double), ("df", C.c_double), ("dq", C.c_double), ("h", C.c_double), ("kmin", C.c_int),
+                ("ndiv", C.c_int), ("ndiv2", C.c_int), ("ivs", C.c_int), ("vpvs", C.c_float), ("nthres", C.c_int),
+                ("resthres", C.c_float), ("resthrep", C.c_float), ("stdmax", C.c_float)]
+
+
+class CLocEvent(C.Structure):
+    """fdgpu_loc_event: one event's picks and, on return, its location."""
+    _fields_ = [("npick", C.c_int), ("table", C.c_void_p), ("obstime", C.c_void_p), ("pwt", C.c_void_p),
+                ("phase", C.c_void_p), ("isgood", C.c_void_p), ("res", C.c_void_p), ("tcalc", C.c_void_p),
+                ("status", C.c_int), ("ngood", C.c_int), ("passes", C.c_int), ("node", C.c_int * 3),
+                ("stdmin", C.c_float), ("avrmin", C.c_float), ("exm", C.c_double), ("eym", C.c_double),
+                ("ezm", C.c_double)]
+
+
+def loc_params(**kw) -> CLocParams:
+    """fdgpu_loc_params with sphfdloc's defaults (src/sphfdloc.c:111-128), then kw."""
+    p = CLocParams()
+    lib().fdgpu_loc_params_defaults(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 @dataclasses.dataclass
 class Coarse:
     """A coarse model: node counts, fine intervals per coarse interval, the
@@ -152,6 +178,9 @@ def lib():
                                                       C.c_void_p]
         L.fdgpu_serve_submit_coarse.argtypes = [C.c_void_p, C.POINTER(CTable), C.POINTER(CCoarse), C.c_int,
                                                 C.POINTER(C.c_int)]
+        L.fdgpu_loc_params_defaults.argtypes = [C.POINTER(CLocParams)]
+        L.fdgpu_loc_params_defaults.restype = None
+        L.fdgpu_locate.argtypes = [C.c_void_p, C.POINTER(CLocParams), C.POINTER(CLocEvent), C.c_int]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -236,6 +265,36 @@ class Context:
 
     def release_model(self, model: int):
         _check(lib().fdgpu_model_release(self._h, model), "fdgpu_model_release")
+
+    def locate(self, params: CLocParams, events: list) -> list:
+        """Locates events over resident tables (sphfdloc's grid search).  Each
+        event is a dict with `table` (ids), `obstime`, `pwt`, `phase` (a str of
+        'P'/'S') and optionally `isgood`; returns one dict per event with the
+        status, counts, node, deviation, mean residual, hypocentre, and the
+        final use flags, residuals and computed times per pick."""
+        keep, arr = [], (CLocEvent * len(events))()
+        outs = []
+        for e, ev in zip(arr, events):
+            n = len(ev["table"])
+            tab = np.ascontiguousarray(ev["table"], dtype=np.int32)
+            obs = np.ascontiguousarray(ev["obstime"], dtype=np.float32)
+            pwt = np.ascontiguousarray(ev["pwt"], dtype=np.float32)
+            ph = np.frombuffer(ev["phase"].encode(), dtype=np.uint8).copy()
+            good = np.ascontiguousarray(ev.get("isgood", np.ones(n)), dtype=np.int32).copy()
+            res, tc = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+            if not (len(obs) == len(pwt) == len(ph) == len(good) == n):
+                raise ValueError("pick arrays of different lengths")
+            keep += [tab, obs, pwt, ph, good, res, tc]
+            e.npick = n
+            e.table, e.obstime, e.pwt = tab.ctypes.data, obs.ctypes.data, pwt.ctypes.data
+            e.phase, e.isgood, e.res, e.tcalc = ph.ctypes.data, good.ctypes.data, res.ctypes.data, tc.ctypes.data
+            outs.append((good, res, tc))
+        _check(lib().fdgpu_locate(self._h, C.byref(params), arr, len(events)), "fdgpu_locate")
+        out = []
+        for e, (good, res, tc) in zip(arr, outs):
+            out.append(dict(status=e.status, ngood=e.ngood, passes=e.passes, node=tuple(e.node), stdmin=e.stdmin,
+                            avrmin=e.avrmin, exm=e.exm, eym=e.eym, ezm=e.ezm, isgood=good, res=res, tps=tc))
+        return out
 
     def models_from_coarse(self, coarse: Coarse, want_fine: bool = False):
         """(P model, S model[, fine velocities (2, nz, ny, nx)]) of a coarse

@@ -101,6 +101,55 @@ int fdgpu_models_from_coarse(fdgpu_ctx *ctx, const fdgpu_coarse *coarse, int *mo
 int fdgpu_models_update_from_coarse(fdgpu_ctx *ctx, int model_p, int model_s, const fdgpu_coarse *coarse,
                                     float *fine);
 
+/* Grid-search location over resident tables (sphfdloc, src/sphfdloc.c).
+ * The grid is in sphfdloc's units after its set-up (sphfdloc.c:382-393):
+ * x0 longitude and y0 geocentric colatitude of the origin in radians, z0
+ * depth in km, df/dq the longitude/colatitude spacings in radians, h in km;
+ * the remaining fields are the program's parameters (sphfdloc.c:111-128,
+ * spec keys of the same names). */
+typedef struct {
+    int nx, ny, nz;
+    double x0, y0, z0, df, dq, h;
+    int kmin;                   /* first layer of the node search (2) */
+    int ndiv, ndiv2;            /* divisions of the two refinements (20, 20; at most 512) */
+    int ivs;                    /* 0: an S pick's time is vpvs times its table (1) */
+    float vpvs;                 /* 1.78 */
+    int nthres;                 /* fewest usable picks (8) */
+    float resthres, resthrep;   /* outlier limits, seconds and percent (0.5, 5.0) */
+    float stdmax;               /* largest accepted deviation (15.0) */
+} fdgpu_loc_params;
+
+/* One event: its picks (the lines of its event file, sphfdloc.c:612-676) and,
+ * on return, its location.  isgood is read and updated by the outlier passes;
+ * res and tcalc receive each pick's residual and computed time at the
+ * hypocentre (resmin and tps of sphfdloc.c:799-801; zero where no pass found
+ * a minimum). */
+typedef struct {
+    int npick;                  /* 1..800 (maxobs) */
+    const int *table;           /* pick p's resident table (fdgpu_solve), of the grid's dimensions */
+    const float *obstime;       /* arrival time minus the event's reference time, seconds */
+    const float *pwt;           /* weight 1/rwt^2 (sphfdloc.c:650) */
+    const char *phase;          /* 'P' or 'S' */
+    int *isgood;                /* 1 usable, 0 flagged ('*'); updated */
+    float *res;                 /* out: npick residuals */
+    float *tcalc;               /* out: npick computed times */
+    int status;                 /* out: 1 located; 2 enough picks, deviation above stdmax; 0 too few picks */
+    int ngood;                  /* out: usable picks after the outlier passes */
+    int passes;                 /* out: searches run */
+    int node[3];                /* out: node of the last node search */
+    float stdmin, avrmin;       /* out: deviation and mean weighted residual at the hypocentre */
+    double exm, eym, ezm;       /* out: hypocentre (longitude, colatitude in radians; depth in km) */
+} fdgpu_loc_event;
+
+/* sphfdloc's parameter defaults (sphfdloc.c:111-128); grid fields zero. */
+void fdgpu_loc_params_defaults(fdgpu_loc_params *p);
+/* Locates n events (replaces sphfdloc's per-event loop, sphfdloc.c:752-1130,
+ * and its table reads): each search pass runs on the device for every event
+ * still being located; the outlier and retry decisions run on the host
+ * between passes.  FDGPU_EINPUT for bad parameters, an unknown table or one
+ * whose dimensions are not the grid's, or a phase other than P and S. */
+int fdgpu_locate(fdgpu_ctx *ctx, const fdgpu_loc_params *p, fdgpu_loc_event *events, int n);
+
 /* One table request.  Table node (0,0,0) sits at model node (ox, oy, oz); the
  * table spans par.nx*par.ny*par.nz nodes inside the model. */
 typedef struct {

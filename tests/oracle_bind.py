@@ -143,6 +143,10 @@ def lib():
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.oracle_c2f_dims.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.oracle_loc_find_time.restype = C.c_double
+        L.oracle_loc_find_time.argtypes = [C.POINTER(LocGrid), C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.oracle_locate.argtypes = [C.POINTER(LocGrid), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_char_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(LocOut)]
         _lib = L
     return _lib
 
@@ -246,3 +250,54 @@ def c2f_golden_cases():
         c["seed"] = s
         out.append(c)
     return out
+
+
+class LocGrid(C.Structure):
+    """oracle_loc_grid (oracle/loc_oracle.h); field order as declared there."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("x0", C.c_double), ("y0", C.c_double),
+                ("z0", C.c_double), ("df", C.c_double), ("dq", C.c_double), ("h", C.c_double), ("kmin", C.c_int),
+                ("ndiv", C.c_int), ("ndiv2", C.c_int), ("ivs", C.c_int), ("vpvs", C.c_float), ("nthres", C.c_int),
+                ("resthres", C.c_float), ("resthrep", C.c_float), ("stdmax", C.c_float)]
+
+
+class LocOut(C.Structure):
+    _fields_ = [("status", C.c_int), ("ngood", C.c_int), ("passes", C.c_int), ("node", C.c_int * 3),
+                ("stdmin", C.c_float), ("avrmin", C.c_float), ("exm", C.c_double), ("eym", C.c_double),
+                ("ezm", C.c_double)]
+
+
+# sphfdloc's defaults (src/sphfdloc.c:111-128)
+LOC_DEFAULTS = dict(kmin=2, ndiv=20, ndiv2=20, ivs=1, vpvs=1.78, nthres=8, resthres=0.5, resthrep=5.0, stdmax=15.0)
+
+
+def loc_grid(**kw) -> LocGrid:
+    d = dict(LOC_DEFAULTS)
+    d.update(kw)
+    return LocGrid(**d)
+
+
+def loc_find_time(g: LocGrid, table: np.ndarray, x: float, y: float, z: float) -> float:
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    return lib().oracle_loc_find_time(C.byref(g), t.ctypes.data, x, y, z)
+
+
+def locate(g: LocGrid, tables, obstime, pwt, phase: str, isgood) -> dict:
+    """The oracle's location of one event (oracle/loc_oracle.c).  tables[p] is
+    pick p's travel-time grid; returns the result fields plus the final use
+    flags, residuals and computed times per pick."""
+    ts = [np.ascontiguousarray(t, dtype=np.float32) for t in tables]
+    ptrs = (C.c_void_p * len(ts))(*[t.ctypes.data for t in ts])
+    n = len(ts)
+    obs = np.ascontiguousarray(obstime, dtype=np.float32)
+    w = np.ascontiguousarray(pwt, dtype=np.float32)
+    good = np.ascontiguousarray(isgood, dtype=np.int32).copy()
+    res = np.zeros(n, dtype=np.float32)
+    tps = np.zeros(n, dtype=np.float32)
+    out = LocOut()
+    if lib().oracle_locate(C.byref(g), ptrs, n, obs.ctypes.data, w.ctypes.data, phase.encode(), good.ctypes.data,
+                           res.ctypes.data, tps.ctypes.data, C.byref(out)) != 0:
+        raise ValueError("bad location input")
+    r = {name: getattr(out, name) for name, _ in LocOut._fields_}
+    r["node"] = tuple(out.node)
+    r.update(isgood=good, res=res, tps=tps)
+    return r

@@ -1,0 +1,176 @@
+"""GPU parity of the grid-search location (sphfdloc's search, src/sphfdloc.c:
+752-1130 with find_time 1305-1345, on the device): events located over
+tables solved on the device equal the oracle's location over the same
+tables, field for field and bit for bit: status, usable-pick count, passes,
+node, deviation, mean residual, hypocentre, use flags, residuals and
+computed times."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from cases import par_dict
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 31, 29, 16
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from fdtomoc_amd import fdgpu
+    c = fdgpu.Context(0)
+    yield c
+    c.close()
+
+
+def _bits(a, b):
+    return int((np.asarray(a, dtype=np.float32).view(np.uint32) !=
+                np.asarray(b, dtype=np.float32).view(np.uint32)).sum())
+
+
+@pytest.fixture(scope="module")
+def tables(ctx):
+    """16 station tables of one P model and one S model (32 tables)."""
+    from fdtomoc_amd import fdgpu, synth
+    g = synth.centred_grid(NX, NY, NZ, 1.0)
+    rng = np.random.default_rng(71)
+    vp = synth.gradient_velocity(g, 4.8, 7.6, 16.0) * synth.layered_perturbation(g, rng, amp=0.03)
+    vp = vp.astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs, pars = [], []
+    for s in range(16):
+        st = synth.surface_node_station(g, np.random.default_rng(100 + s), margin=1)
+        par = ob.Par(nx=NX, ny=NY, nz=NZ, h=g.h, x0=g.x0, y0=g.y0, z0=g.z0, fxs=st.lon, fys=st.lat, fzs=st.depth,
+                     reverse=50)
+        pars.append(par)
+        specs += [fdgpu.TableSpec(par_dict(par), mp), fdgpu.TableSpec(par_dict(par), ms)]
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    host = [ctx.read(i, (NZ, NY, NX)) for i in ids]
+    yield g, ids, host
+    for i in ids:
+        ctx.release(i)
+    ctx.release_model(mp)
+    ctx.release_model(ms)
+
+
+def _params(g, **kw):
+    """The search grid as sphfdloc sets it up (sphfdloc.c:382-393)."""
+    degrad = float(np.float32(0.017453292))
+    hpi = float(np.float32(1.570796))
+    from fdtomoc_amd import synth
+    colat = hpi - synth.glath(g.y0 * degrad, g.z0)[0]
+    d = dict(nx=NX, ny=NY, nz=NZ, x0=g.x0 * degrad, y0=colat, z0=g.z0, h=g.h, dq=g.h / 6371.0,
+             df=abs(g.h / (6371.0 * math.sin(colat))))
+    d.update(kw)
+    return d
+
+
+def _events(g, ids, host, p, seed, count=8):
+    """Events at sub-grid sources: each pick a station table's trilinear time
+    (the oracle's find_time) plus an origin offset and noise; some picks
+    flagged unusable, some late (outliers), P and S."""
+    rng = np.random.default_rng(seed)
+    grid = ob.loc_grid(**p)
+    evs = []
+    for e in range(count):
+        src = (rng.uniform(1, NX - 2), rng.uniform(1, NY - 2), rng.uniform(p["kmin"] + 0.5, NZ - 2))
+        x, y, z = p["x0"] + src[0] * p["df"], p["y0"] + src[1] * p["dq"], p["z0"] + src[2] * p["h"]
+        n = int(rng.integers(9, 22))
+        sta = rng.choice(16, size=n, replace=True)
+        ph = "".join(rng.choice(["P", "S"], size=n, p=[0.6, 0.4]))
+        tab, ref = [], []
+        for s, c in zip(sta, ph):
+            q = 2 * int(s) + (1 if (c == "S" and p["ivs"] == 1) else 0)
+            tab.append(ids[q])
+            ref.append(host[q])
+        t0 = rng.uniform(0.5, 3.0)
+        obs = []
+        for t, c in zip(ref, ph):
+            v = ob.loc_find_time(grid, t, x, y, z)
+            if c == "S" and p["ivs"] == 0:
+                v *= p.get("vpvs", 1.78)
+            obs.append(v + t0 + rng.normal(0, 0.03))
+        obs = np.array(obs, dtype=np.float32)
+        if e % 3 == 1:
+            obs[int(rng.integers(n))] += np.float32(rng.uniform(2.0, 6.0))
+        good = np.ones(n, dtype=np.int32)
+        if e % 4 == 2:
+            good[int(rng.integers(n))] = 0
+        rwt = rng.choice([0.05, 0.1, 0.2, 0.4], size=n).astype(np.float32)
+        pwt = (np.float32(1.0) / (rwt * rwt)).astype(np.float32)
+        evs.append(dict(table=tab, tabs=ref, obstime=obs, pwt=pwt, phase=ph, isgood=good))
+    return evs
+
+
+def _check(ctx, p, evs):
+    from fdtomoc_amd import fdgpu
+    got = ctx.locate(fdgpu.loc_params(**p), evs)
+    grid = ob.loc_grid(**p)
+    for k, (ev, r) in enumerate(zip(evs, got)):
+        o = ob.locate(grid, ev["tabs"], ev["obstime"], ev["pwt"], ev["phase"], ev["isgood"])
+        print(f"event {k}: status {r['status']} passes {r['passes']} node {r['node']} std {r['stdmin']}", flush=True)
+        for f in ("status", "ngood", "passes", "node", "exm", "eym", "ezm"):
+            assert r[f] == o[f], (k, f, r[f], o[f])
+        assert _bits([r["stdmin"], r["avrmin"]], [o["stdmin"], o["avrmin"]]) == 0, k
+        assert np.array_equal(r["isgood"], o["isgood"]), k
+        assert _bits(r["res"], o["res"]) == 0 and _bits(r["tps"], o["tps"]) == 0, k
+    return got
+
+
+def test_locations_match_oracle(ctx, tables):
+    """Default parameters, S picks on S tables (ivs = 1): every field equal to
+    the oracle; the hypocentres land near the sources."""
+    g, ids, host = tables
+    p = _params(g, **ob.LOC_DEFAULTS)
+    got = _check(ctx, p, _events(g, ids, host, p, seed=1))
+    assert sum(r["status"] == 1 for r in got) >= 6
+    assert any(r["passes"] > 1 for r in got)
+
+
+def test_locations_scaled_s_and_small_divisions(ctx, tables):
+    """ivs = 0 (S times are vpvs times the P tables), other divisions, kmin and
+    thresholds: still every field equal to the oracle."""
+    g, ids, host = tables
+    p = _params(g, **dict(ob.LOC_DEFAULTS, ivs=0, ndiv=7, ndiv2=5, kmin=0, nthres=6, resthres=0.3))
+    _check(ctx, p, _events(g, ids, host, p, seed=2, count=10))
+
+
+def test_edge_events(ctx, tables):
+    """Sources at the grid's faces and corners (one-sided and shortened
+    refinement boxes), four picks (a zero variance divisor: no trial beats
+    the start, the hypocentre stays at the origin), and too few picks."""
+    g, ids, host = tables
+    p = _params(g, **ob.LOC_DEFAULTS)
+    grid = ob.loc_grid(**p)
+    evs = []
+    for src in ((0, 0, 2), (NX - 1, NY - 1, NZ - 1), (0, NY - 1, 2), (NX - 1, 0, NZ - 1), (15, 14, 2)):
+        x, y, z = p["x0"] + src[0] * p["df"], p["y0"] + src[1] * p["dq"], p["z0"] + src[2] * p["h"]
+        sel = list(range(0, 32, 2))
+        obs = np.array([ob.loc_find_time(grid, host[q], x, y, z) + 1.0 for q in sel], dtype=np.float32)
+        evs.append(dict(table=[ids[q] for q in sel], tabs=[host[q] for q in sel], obstime=obs,
+                        pwt=np.ones(len(sel), np.float32), phase="P" * len(sel), isgood=np.ones(len(sel), np.int32)))
+    for n in (4, 6):
+        sel = list(range(0, 2 * n, 2))
+        obs = np.array([host[q][5, 9, 11] + 0.5 for q in sel], dtype=np.float32)
+        evs.append(dict(table=[ids[q] for q in sel], tabs=[host[q] for q in sel], obstime=obs,
+                        pwt=np.ones(n, np.float32), phase="P" * n, isgood=np.ones(n, np.int32)))
+    got = _check(ctx, p, evs)
+    assert got[5]["stdmin"] == math.inf and got[5]["node"] == (0, 0, 0) and got[5]["status"] == 0
+    assert got[6]["status"] == 0
+
+
+def test_locate_errors(ctx, tables):
+    """Unknown tables, tables of other dimensions, bad phases and bad
+    parameters are refused with FDGPU_EINPUT."""
+    from fdtomoc_amd import fdgpu
+    g, ids, host = tables
+    p = _params(g, **ob.LOC_DEFAULTS)
+    ev = dict(table=ids[:10:2], obstime=np.ones(5, np.float32), pwt=np.ones(5, np.float32), phase="PPPPP")
+    for bad_ev, bad_p in ((dict(ev, table=[999999] * 5), p), (dict(ev, phase="PPXPP"), p),
+                          (ev, dict(p, nx=NX + 1)), (ev, dict(p, ndiv=0)), (ev, dict(p, kmin=-1))):
+        with pytest.raises(fdgpu.FdgpuError) as e:
+            ctx.locate(fdgpu.loc_params(**bad_p), [bad_ev])
+        assert e.value.code == -5

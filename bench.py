@@ -29,6 +29,10 @@ Other workloads (one JSON line each, same keys where they apply):
                   path): a 31x31x7 coarse model to the 301x301x61 fine grid
                   into two resident models per step, the reference c2f timed
                   beside it
+  --workload loc  sphfdloc's grid-search location (the row after c2f): 256
+                  events of 24-40 picks over the 96 resident tables of the
+                  default workload, all events per step, the oracle's location
+                  timed on one core on a sample of the events
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
     python bench.py --calibrate-copy MIB     # one known-size copy, for HBM counter scaling
@@ -528,6 +532,104 @@ def run_c2f(args):
     return out
 
 
+# ------------------------------------------------------------------ location
+
+def loc_oracle_rate(params, events, host, sample):
+    """cpu_baseline leg: the oracle's location (oracle/loc_oracle.c, a scalar
+    restatement of sphfdloc's per-event loop) on one core over the first
+    `sample` events; also returns its results for the comparison."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_bind as ob
+    grid = ob.loc_grid(**params)
+    outs, t0 = [], time.perf_counter()
+    for ev in events[:sample]:
+        outs.append(ob.locate(grid, [host[int(q)] for q in ev["qs"]], ev["obstime"], ev["pwt"], ev["phase"],
+                              ev["isgood"]))
+    return sample / (time.perf_counter() - t0), outs
+
+
+def run_loc(args):
+    """sphfdloc's grid search (src/sphfdloc.c:752-1130, find_time 1305-1345) over
+    resident tables: the 96 tables of the default workload (161 x 161 x 81,
+    48 stations P and S) solved once, untimed; a step locates 256 synthetic
+    events (24-40 picks each, outliers and flagged picks included): every
+    search pass on the device, the outlier decisions on the host between
+    passes."""
+    g, vp, vs, stations = workloads.c2()
+    ctx = fdgpu.Context(0)
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs = []
+    for st in stations:
+        for m in (mp, ms):
+            specs.append(fdgpu.TableSpec(workloads.table_par(g, st), m))
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    host = [ctx.read(i, (g.nz, g.ny, g.nx)) for i in ids]
+    events = workloads.loc_events(g, lambda q, k, j, i: float(host[q][k, j, i]), len(ids), count=args.events)
+    for ev in events:
+        ev["table"] = [ids[int(q)] for q in ev["qs"]]
+    params = workloads.loc_params(g)
+    cp = fdgpu.loc_params(**params)
+    for _ in range(args.warmup):
+        ctx.locate(cp, events)
+    walls, node_ms, node_bytes, pass_ms, passes = [], [], [], [], []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        got = ctx.locate(cp, events)
+        walls.append(time.perf_counter() - t0)
+        t = ctx.locate_timing()
+        node_ms.append(t["node_ms"])
+        node_bytes.append(t["node_bytes"])
+        pass_ms.append(t["pass_ms"])
+        passes.append(t["passes"])
+        print(f"# step {s}: {walls[-1] * 1e3:.2f} ms wall, {t['passes']} passes, {t['pass_ms']:.2f} ms device, "
+              f"{t['node_ms']:.2f} ms node search", file=sys.stderr, flush=True)
+    wall = float(np.mean(walls))
+    nev = len(events)
+    located = sum(r["status"] == 1 for r in got)
+    err = [max(abs(r["node"][0] - ev["src"][0]), abs(r["node"][1] - ev["src"][1]), abs(r["node"][2] - ev["src"][2]))
+           for r, ev in zip(got, events) if r["status"] == 1]
+    ach = float(np.sum(node_bytes)) / (float(np.sum(node_ms)) * 1e-3) / 1e9
+    out = {
+        "metric": METRIC + " | sphfdloc events/s",
+        "value": round(nev / wall, 2),
+        "unit": "events located/s (tables resident)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64",
+        "data": "synthetic (events at random nodes of the default workload's tables, 0.05 s noise, 5 % outliers)",
+        "config": {"workload": f"loc: {nev} events x 24-40 picks over 96 resident tables (48 stations P+S) of "
+                               "161x161x81, sphfdloc defaults (kmin 2, ndiv 20, ndiv2 20, nthres 8)",
+                   "events": nev, "picks": int(sum(len(e["qs"]) for e in events)),
+                   "nodes_searched": g.nx * g.ny * (g.nz - 2)},
+        "passes_per_step": float(np.mean(passes)),
+        "device_ms_per_step": round(float(np.mean(pass_ms)), 3),
+        "located": located, "max_node_offset_from_source": max(err) if err else None,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "loc_nodes_kernel",
+                     "kernel_ms": round(float(np.sum(node_ms)) / float(np.sum(passes)), 4),
+                     "algorithmic_bytes": float(np.sum(node_bytes)) / float(np.sum(passes)),
+                     "what": "per launch: 4 bytes per table a usable pick of the pass names, per searched node"},
+    }
+    out["cpu_baseline"] = None
+    if not args.no_cpu:
+        n = max(1, min(args.loc_cpu_events, nev))
+        rate, outs = loc_oracle_rate(params, events, host, n)
+        out["cpu_baseline"] = {"value": round(rate, 4), "unit": "events located/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {n} of the {nev} events: the oracle's location "
+                                         "(oracle/loc_oracle.c, scalar C) over the same tables in host memory"}
+        out["gpu_over_cpu"] = round(out["value"] / rate, 1)
+        same = all(r["node"] == o["node"] and r["exm"] == o["exm"] and r["eym"] == o["eym"] and r["ezm"] == o["ezm"]
+                   and np.float32(r["stdmin"]).view(np.uint32) == np.float32(o["stdmin"]).view(np.uint32)
+                   and r["passes"] == o["passes"] for r, o in zip(got, outs))
+        out["sample_identical_to_oracle"] = bool(same)
+    for i in ids:
+        ctx.release(i)
+    ctx.release_model(mp)
+    ctx.release_model(ms)
+    ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ calibration
 
 def calibrate_copy(mib: int) -> None:
@@ -549,7 +651,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "c2f"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "c2f", "loc"])
     ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -559,6 +661,8 @@ def main():
     ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=3.0, help="c4: mean arrivals per second")
     ap.add_argument("--tables", type=int, default=256, help="c5: tables per launch")
+    ap.add_argument("--events", type=int, default=256, help="loc: events per step")
+    ap.add_argument("--loc-cpu-events", type=int, default=8, help="loc: events in the CPU baseline sample")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -568,7 +672,7 @@ def main():
         print("bench.py: this build measures one GPU; the station tables shard across GPUs with no "
               "communication on the hot path, so multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
-    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f}[args.workload]
+    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f, "loc": run_loc}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

@@ -299,6 +299,7 @@ int fdgpu_open(int device, unsigned long long budget, fdgpu_ctx **out) {
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
         hipMalloc(&c->d_queue, sizeof(int)) != hipSuccess) {
         fdgpu_close(c);
         return FDGPU_EHIP;
@@ -326,6 +327,8 @@ int fdgpu_close(fdgpu_ctx *c) {
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_a) (void)hipEventDestroy(c->ev_a);
+    if (c->ev_b) (void)hipEventDestroy(c->ev_b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FDGPU_OK;

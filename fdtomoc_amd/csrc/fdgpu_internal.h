@@ -141,6 +141,7 @@ struct fdgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;   // around one kernel of a sequence (location's node search)
     unsigned long long budget = 0, used = 0;
     int num_cu = 0;
     int wg_per_cu[2] = {1, 1};   // resident workgroups per CU of the wide / narrow sweep kernel
@@ -165,6 +166,8 @@ struct fdgpu_ctx {
     char *d_coarse = nullptr;
     size_t coarse_bytes = 0;
     double last_ms = 0.0;
+    // the last fdgpu_locate: passes, pass ms, node-search ms, node-search table bytes
+    double loc_timing[4] = {0.0, 0.0, 0.0, 0.0};
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     fdgpu_detail::Serve serve;
     std::mutex mu;

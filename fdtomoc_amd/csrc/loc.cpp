@@ -12,6 +12,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <set>
 #include <vector>
 
 #include "fdgpu_internal.h"
@@ -97,22 +99,40 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
     }
     const size_t npk = picks.size();
 
-    // one device block: picks | events | active list | states | residuals | computed times
+    // one device block: picks | events | active list | states | residuals |
+    // computed times | usable picks | their events | their scale flags
     const size_t b_pk = align256(sizeof(LocPick) * npk);
     const size_t b_ev = align256(sizeof(LocEvent) * (size_t)nev);
     const size_t b_ac = align256(sizeof(int) * (size_t)nev);
     const size_t b_st = align256(sizeof(LocState) * (size_t)nev);
     const size_t b_rs = align256(sizeof(float) * npk);
-    const size_t bytes = b_pk + b_ev + b_ac + b_st + 2 * b_rs;
+    const size_t b_sp = align256(sizeof(LocSPick) * npk);
+    const size_t b_se = align256(sizeof(LocSEvent) * (size_t)nev);
+    const size_t b_sc = align256(npk);
+    const size_t bytes = b_pk + b_ev + b_ac + b_st + 2 * b_rs + b_sp + b_se + b_sc;
     char *blk = nullptr;
     int rc = dev_alloc(c, (void **)&blk, bytes);
     if (rc) return rc;
-    LocPick *d_pk = (LocPick *)blk;
-    LocEvent *d_ev = (LocEvent *)(blk + b_pk);
-    int *d_ac = (int *)(blk + b_pk + b_ev);
-    LocState *d_st = (LocState *)(blk + b_pk + b_ev + b_ac);
-    float *d_res = (float *)(blk + b_pk + b_ev + b_ac + b_st);
-    float *d_tps = (float *)(blk + b_pk + b_ev + b_ac + b_st + b_rs);
+    size_t off = 0;
+    auto carve = [&](size_t b) {
+        char *p = blk + off;
+        off += b;
+        return p;
+    };
+    LocPick *d_pk = (LocPick *)carve(b_pk);
+    LocEvent *d_ev = (LocEvent *)carve(b_ev);
+    int *d_ac = (int *)carve(b_ac);
+    LocState *d_st = (LocState *)carve(b_st);
+    float *d_res = (float *)carve(b_rs);
+    float *d_tps = (float *)carve(b_rs);
+    LocSPick *d_sp = (LocSPick *)carve(b_sp);
+    LocSEvent *d_se = (LocSEvent *)carve(b_se);
+    unsigned char *d_sc = (unsigned char *)carve(b_sc);
+    std::vector<LocSPick> spk;
+    std::vector<LocSEvent> sev((size_t)nev);
+    std::vector<unsigned char> ssc;
+    spk.reserve(npk);
+    ssc.reserve(npk);
 
     LocGrid g;
     memset(&g, 0, sizeof g);
@@ -125,20 +145,54 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
     std::vector<float> res(npk), tps(npk);
     std::vector<int> active;
     for (int e = 0; e < nev; e++) active.push_back(e);
-    double kernel_ms = 0.0;
+    double kernel_ms = 0.0, node_ms = 0.0, node_bytes = 0.0;
+    int npass = 0;
     bool ok = hipMemsetAsync(d_res, 0, sizeof(float) * npk, c->stream) == hipSuccess &&
               hipMemsetAsync(d_tps, 0, sizeof(float) * npk, c->stream) == hipSuccess;
     while (ok && !active.empty()) {
-        for (int e : active) events[(size_t)e].ngood = run[(size_t)e].ngood;
+        // the pass's usable picks per event, and the variance terms no trial
+        // changes, in the reference's float order (sphfdloc.c:781, 786-788)
+        spk.clear();
+        ssc.clear();
+        for (int e : active) {
+            LocEvent &E = events[(size_t)e];
+            E.ngood = run[(size_t)e].ngood;
+            LocSEvent &S = sev[(size_t)e];
+            S.first = (int)spk.size();
+            S.scaled = 0;
+            float facs = 0.0f;
+            for (int p = E.first; p < E.first + E.npick; p++) {
+                const LocPick &k = picks[(size_t)p];
+                if (!k.good) continue;
+                spk.push_back(LocSPick{k.table, k.obs, k.pwt});
+                ssc.push_back((unsigned char)k.scale);
+                S.scaled |= k.scale;
+                facs += k.pwt;
+            }
+            S.count = (int)spk.size() - S.first;
+            S.facs = facs;
+            const float avwt = facs / (float)E.ngood;
+            S.den = (float)(E.ngood - 4) * avwt;
+            S.pad = 0;
+        }
+        if (spk.empty()) {   // keep the device pointers valid for an all-flagged pass
+            spk.push_back(LocSPick{picks[0].table, 0.0f, 0.0f});
+            ssc.push_back(0);
+        }
         ok = hipMemcpyAsync(d_pk, picks.data(), sizeof(LocPick) * npk, hipMemcpyHostToDevice, c->stream) ==
                  hipSuccess &&
+             hipMemcpyAsync(d_sp, spk.data(), sizeof(LocSPick) * spk.size(), hipMemcpyHostToDevice, c->stream) ==
+                 hipSuccess &&
+             hipMemcpyAsync(d_se, sev.data(), sizeof(LocSEvent) * (size_t)nev, hipMemcpyHostToDevice, c->stream) ==
+                 hipSuccess &&
+             hipMemcpyAsync(d_sc, ssc.data(), ssc.size(), hipMemcpyHostToDevice, c->stream) == hipSuccess &&
              hipMemcpyAsync(d_ev, events.data(), sizeof(LocEvent) * (size_t)nev, hipMemcpyHostToDevice,
                             c->stream) == hipSuccess &&
              hipMemcpyAsync(d_ac, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice, c->stream) ==
                  hipSuccess &&
              hipEventRecord(c->ev0, c->stream) == hipSuccess &&
-             fd_launch_loc_pass(&g, d_pk, d_ev, d_ac, (int)active.size(), d_st, d_res, d_tps, c->stream) ==
-                 hipSuccess &&
+             fd_launch_loc_pass(&g, d_pk, d_ev, d_sp, d_se, d_sc, d_ac, (int)active.size(), d_st, d_res, d_tps,
+                                c->ev_a, c->ev_b, c->stream) == hipSuccess &&
              hipEventRecord(c->ev1, c->stream) == hipSuccess &&
              hipMemcpyAsync(st.data(), d_st, sizeof(LocState) * (size_t)nev, hipMemcpyDeviceToHost, c->stream) ==
                  hipSuccess &&
@@ -150,6 +204,15 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
         if (!ok) break;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) kernel_ms += ms;
+        if (hipEventElapsedTime(&ms, c->ev_a, c->ev_b) == hipSuccess) node_ms += ms;
+        // the node search reads each table a usable pick of the pass names once
+        // per searched node, however many events share it
+        std::set<const float *> used;
+        for (int e : active)
+            for (int p = events[(size_t)e].first; p < events[(size_t)e].first + events[(size_t)e].npick; p++)
+                if (picks[(size_t)p].good) used.insert(picks[(size_t)p].table);
+        node_bytes += 4.0 * (double)used.size() * (double)P.nx * P.ny * std::max(0, P.nz - P.kmin);
+        npass++;
         std::vector<int> next;
         for (int e : active) {
             EventRun &R = run[(size_t)e];
@@ -192,6 +255,10 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
     dev_free(c, blk, bytes);
     if (!ok) return FDGPU_EHIP;
     c->last_ms = kernel_ms;
+    c->loc_timing[0] = npass;
+    c->loc_timing[1] = kernel_ms;
+    c->loc_timing[2] = node_ms;
+    c->loc_timing[3] = node_bytes;
     for (int e = 0; e < nev; e++) {
         fdgpu_loc_event &E = ev[e];
         const LocEvent &L = events[(size_t)e];
@@ -212,5 +279,12 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
         E.eym = S.e[1];
         E.ezm = S.e[2];
     }
+    return FDGPU_OK;
+}
+
+int fdgpu_locate_timing(fdgpu_ctx *c, double *t, int n) {
+    if (!c || !t || n < 0) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int q = 0; q < n && q < 4; q++) t[q] = c->loc_timing[q];
     return FDGPU_OK;
 }

@@ -50,7 +50,7 @@ __device__ __forceinline__ Cell cell_of(const LocGrid &g, double x, double y, do
 // The table time at the cell's point: the eight terms summed in order
 // (sphfdloc.c:1337-1344).
 __device__ __forceinline__ double interp(const float *__restrict__ t, const Cell &c, long long sy, long long sz) {
-    const float *p = t + c.base;
+    const __attribute__((address_space(1))) float *p = (const __attribute__((address_space(1))) float *)t + c.base;
     double tp = c.w[0] * (double)p[0];
     tp = tp + c.w[1] * (double)p[1];
     tp = tp + c.w[2] * (double)p[sy];
@@ -86,6 +86,33 @@ __device__ __forceinline__ float deviation(const Sums &s, int ngood) {
     const float avwt = s.facs / (float)ngood;
     const float var = (s.rsq - s.avres * s.avres / s.facs) / ((float)(ngood - 4) * avwt);
     return sqrtf(fabsf(var));
+}
+
+// The same from the event's fixed terms: facs and the divisor (ngood - 4) *
+// avwt do not depend on the trial point.
+__device__ __forceinline__ float deviation(float avres, float rsq, float facs, float den) {
+    const float var = (rsq - avres * avres / facs) / den;
+    return sqrtf(fabsf(var));
+}
+
+// A usable pick's two sums from its residual.
+__device__ __forceinline__ void add2(float &avres, float &rsq, float r, float w) {
+    const float r1 = r * w;
+    const float r2 = r1 * r;
+    avres += r1;
+    rsq += r2;
+}
+
+// Residual of an unscaled pick: (float)((double)obs - (double)t) of the
+// reference is one float subtraction, since rounding the exact difference of
+// two floats first to double (53 >= 2 * 24 + 2 bits) and then to float gives
+// the single float rounding (Figueroa's double-rounding bound).
+__device__ __forceinline__ float residual_unscaled(float obs, float t) { return obs - t; }
+
+// Table value through the global address space (a pick's table pointer is
+// read from memory, so the compiler would otherwise use a flat load).
+__device__ __forceinline__ float gload(const float *t, long long n) {
+    return *((const __attribute__((address_space(1))) float *)t + n);
 }
 
 // Ordering key of a trial that beats `bound` (the strict comparison of

@@ -4,8 +4,9 @@
 // One search pass of a batch of events is seven launches on one stream:
 //   loc_init_kernel     per event: clear the running minimum
 //   loc_nodes_kernel    nodes of layers kmin.. across the lanes, every event of
-//                       the batch in turn: a pick's table value is one
-//                       coalesced load, and the events that share stations
+//                       the batch in turn over its usable picks only: a pick's
+//                       table value is one coalesced global load (eight in
+//                       flight per wave), and the events that share stations
 //                       re-read the same node tile from cache
 //   loc_commit_kernel   per event: decode the winning trial, evaluate it again
 //                       (residuals and computed times of every pick, the mean
@@ -62,8 +63,39 @@ __global__ void loc_init_kernel(const int *__restrict__ active, int nactive, Loc
     }
 }
 
-__global__ __launch_bounds__(kThreads) void loc_nodes_kernel(const LocGrid g, const LocPick *__restrict__ picks,
-                                                             const LocEvent *__restrict__ events,
+// Sums of the usable picks [first, first + count) at node n, in pick order;
+// the table loads of eight picks are issued before their arithmetic.
+template <bool SCALED>
+__device__ __forceinline__ void node_sums(const LocGrid &g, const LocSPick *__restrict__ sp,
+                                          const unsigned char *__restrict__ sc, int first, int count, long long n,
+                                          float &avres, float &rsq) {
+    constexpr int U = 8;
+    const int end = first + count;
+    int p = first;
+    for (; p + U <= end; p += U) {
+        float t[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) t[u] = locdev::gload(sp[p + u].table, n);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const float obs = sp[p + u].obs;
+            const float r = SCALED && sc[p + u] ? (float)((double)obs - (double)t[u] * (double)g.vpvs)
+                                                : locdev::residual_unscaled(obs, t[u]);
+            locdev::add2(avres, rsq, r, sp[p + u].pwt);
+        }
+    }
+    for (; p < end; p++) {
+        const float t = locdev::gload(sp[p].table, n);
+        const float obs = sp[p].obs;
+        const float r = SCALED && sc[p] ? (float)((double)obs - (double)t * (double)g.vpvs)
+                                        : locdev::residual_unscaled(obs, t);
+        locdev::add2(avres, rsq, r, sp[p].pwt);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void loc_nodes_kernel(const LocGrid g, const LocSPick *__restrict__ sp,
+                                                             const LocSEvent *__restrict__ sev,
+                                                             const unsigned char *__restrict__ sc,
                                                              const int *__restrict__ active, int nactive,
                                                              LocState *state) {
     const long long nxy = (long long)g.nx * g.ny;
@@ -72,27 +104,27 @@ __global__ __launch_bounds__(kThreads) void loc_nodes_kernel(const LocGrid g, co
     const bool in = n < nxy * g.nz;
     for (int a = 0; a < nactive; a++) {
         const int ev = active[a];
-        const LocEvent E = events[ev];
+        const LocSEvent E = sev[ev];
         unsigned long long key = locdev::kNone;
         if (in) {
-            locdev::Sums s = {0.0f, 0.0f, 0.0f};
-            for (int p = E.first; p < E.first + E.npick; p++) {
-                const LocPick P = picks[p];
-                if (!P.good) continue;
-                locdev::add(s, locdev::residual(g, P, (double)P.table[n]), P.pwt);
-            }
-            key = locdev::key_of(locdev::deviation(s, E.ngood), __builtin_inff(), (unsigned)(n - n0));
+            float avres = 0.0f, rsq = 0.0f;
+            if (E.scaled)
+                node_sums<true>(g, sp, sc, E.first, E.count, n, avres, rsq);
+            else
+                node_sums<false>(g, sp, sc, E.first, E.count, n, avres, rsq);
+            key = locdev::key_of(locdev::deviation(avres, rsq, E.facs, E.den), __builtin_inff(), (unsigned)(n - n0));
         }
         fold_min(key, &state[ev].key);
     }
 }
 
 __global__ __launch_bounds__(kThreads) void loc_refine_kernel(const LocGrid g, int second,
-                                                              const LocPick *__restrict__ picks,
-                                                              const LocEvent *__restrict__ events,
+                                                              const LocSPick *__restrict__ sp,
+                                                              const LocSEvent *__restrict__ sev,
+                                                              const unsigned char *__restrict__ sc,
                                                               const int *__restrict__ active, LocState *state) {
     const int ev = active[blockIdx.y];
-    const LocEvent E = events[ev];
+    const LocSEvent E = sev[ev];
     const LocState &st = state[ev];
     const locdev::Box b = second ? locdev::box_second(g, st.e) : locdev::box_first(g, st.node);
     const unsigned npts = locdev::box_points(b);
@@ -104,13 +136,14 @@ __global__ __launch_bounds__(kThreads) void loc_refine_kernel(const LocGrid g, i
         locdev::box_point(b, q, pt);
         const locdev::Cell c = locdev::cell_of(g, pt[0], pt[1], pt[2]);
         const long long sy = g.nx, sz = (long long)g.nx * g.ny;
-        locdev::Sums s = {0.0f, 0.0f, 0.0f};
-        for (int p = E.first; p < E.first + E.npick; p++) {
-            const LocPick P = picks[p];
-            if (!P.good) continue;
-            locdev::add(s, locdev::residual(g, P, locdev::interp(P.table, c, sy, sz)), P.pwt);
+        float avres = 0.0f, rsq = 0.0f;
+#pragma unroll 2
+        for (int p = E.first; p < E.first + E.count; p++) {
+            double t = locdev::interp(sp[p].table, c, sy, sz);
+            if (E.scaled && sc[p]) t *= (double)g.vpvs;
+            locdev::add2(avres, rsq, (float)((double)sp[p].obs - t), sp[p].pwt);
         }
-        key = locdev::key_of(locdev::deviation(s, E.ngood), st.stdmin, q);
+        key = locdev::key_of(locdev::deviation(avres, rsq, E.facs, E.den), st.stdmin, q);
     }
     fold_min(key, &state[ev].key);
 }
@@ -169,23 +202,28 @@ __global__ void loc_commit_kernel(const LocGrid g, int stage, const LocPick *__r
 }  // namespace
 
 extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_picks, const LocEvent *d_events,
-                                         const int *d_active, int nactive, LocState *d_state, float *d_res,
-                                         float *d_tps, hipStream_t stream) {
+                                         const LocSPick *d_spicks, const LocSEvent *d_sevents,
+                                         const unsigned char *d_sscale, const int *d_active, int nactive,
+                                         LocState *d_state, float *d_res, float *d_tps, hipEvent_t node0,
+                                         hipEvent_t node1, hipStream_t stream) {
     if (nactive <= 0) return hipSuccess;
     const LocGrid g = *grid;
     const long long nodes = (long long)g.nx * g.ny * (g.nz - g.kmin);
     const int small = (nactive + 63) / 64;
     hipLaunchKernelGGL(loc_init_kernel, dim3(small), dim3(64), 0, stream, d_active, nactive, d_state);
+    if (node0) (void)hipEventRecord(node0, stream);
     if (nodes > 0)
         hipLaunchKernelGGL(loc_nodes_kernel, dim3((unsigned)((nodes + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           stream, g, d_picks, d_events, d_active, nactive, d_state);
+                           stream, g, d_spicks, d_sevents, d_sscale, d_active, nactive, d_state);
+    if (node1) (void)hipEventRecord(node1, stream);
     hipLaunchKernelGGL(loc_commit_kernel, dim3(small), dim3(64), 0, stream, g, 0, d_picks, d_events, d_active,
                        nactive, d_state, d_res, d_tps);
     for (int stage = 1; stage <= 2; stage++) {
         const long long side = 2LL * (stage == 1 ? g.ndiv : g.ndiv2) + 1;
         const long long pts = side * side * side;
         hipLaunchKernelGGL(loc_refine_kernel, dim3((unsigned)((pts + kThreads - 1) / kThreads), (unsigned)nactive),
-                           dim3(kThreads), 0, stream, g, stage == 2 ? 1 : 0, d_picks, d_events, d_active, d_state);
+                           dim3(kThreads), 0, stream, g, stage == 2 ? 1 : 0, d_spicks, d_sevents, d_sscale, d_active,
+                           d_state);
         hipLaunchKernelGGL(loc_commit_kernel, dim3(small), dim3(64), 0, stream, g, stage, d_picks, d_events,
                            d_active, nactive, d_state, d_res, d_tps);
     }

@@ -32,6 +32,20 @@ struct LocEvent {
     int first, npick, ngood, pad;
 };
 
+// The search kernels' view of one pass: each event's usable picks only,
+// contiguous, and the terms of its variance that no trial changes.
+struct LocSPick {
+    const float *table;
+    float obs, pwt;
+};
+struct LocSEvent {
+    int first, count;   // [first, first + count) of the usable-pick array
+    int scaled;         // some usable pick's time is vpvs times its table
+    float facs;         // sum of the usable picks' weights in pick order (sphfdloc.c:781)
+    float den;          // (ngood - 4) * (facs / ngood) (sphfdloc.c:786-788)
+    int pad;
+};
+
 // Running state of one event's search pass: the best trial of the running
 // stage as (deviation bits << 32 | scan index), the minimum so far, the
 // hypocentre and the node of the node search.
@@ -46,6 +60,11 @@ struct LocState {
 // One search pass for the events listed in d_active: the node search from
 // layer kmin, then the two refinements; d_state, d_res and d_tps receive
 // each event's minimum, hypocentre, and residuals and computed times at it.
+// node0/node1 (or null) are recorded around the node search.
+// d_spicks/d_sevents (indexed by event like d_events) and d_sscale (one flag
+// per usable pick: time = vpvs * table) are the pass's usable picks.
 extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_picks, const LocEvent *d_events,
-                                         const int *d_active, int nactive, LocState *d_state, float *d_res,
-                                         float *d_tps, hipStream_t stream);
+                                         const LocSPick *d_spicks, const LocSEvent *d_sevents,
+                                         const unsigned char *d_sscale, const int *d_active, int nactive,
+                                         LocState *d_state, float *d_res, float *d_tps, hipEvent_t node0,
+                                         hipEvent_t node1, hipStream_t stream);

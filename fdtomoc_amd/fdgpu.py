@@ -181,6 +181,7 @@ def lib():
         L.fdgpu_loc_params_defaults.argtypes = [C.POINTER(CLocParams)]
         L.fdgpu_loc_params_defaults.restype = None
         L.fdgpu_locate.argtypes = [C.c_void_p, C.POINTER(CLocParams), C.POINTER(CLocEvent), C.c_int]
+        L.fdgpu_locate_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -295,6 +296,13 @@ class Context:
             out.append(dict(status=e.status, ngood=e.ngood, passes=e.passes, node=tuple(e.node), stdmin=e.stdmin,
                             avrmin=e.avrmin, exm=e.exm, eym=e.eym, ezm=e.ezm, isgood=good, res=res, tps=tc))
         return out
+
+    def locate_timing(self) -> dict:
+        """Measurements of the last locate: passes, pass and node-search device
+        milliseconds, and the table bytes the node searches read."""
+        t = (C.c_double * 4)()
+        _check(lib().fdgpu_locate_timing(self._h, t, 4), "fdgpu_locate_timing")
+        return dict(passes=int(t[0]), pass_ms=t[1], node_ms=t[2], node_bytes=t[3])
 
     def models_from_coarse(self, coarse: Coarse, want_fine: bool = False):
         """(P model, S model[, fine velocities (2, nz, ny, nx)]) of a coarse

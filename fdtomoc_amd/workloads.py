@@ -206,3 +206,44 @@ def c2f_spec_text(nxc, nyc, nzc, igridx, igridy, igridz, x0, y0, z0, h, oldvfil=
     return (f"nxc {nxc}\nnyc {nyc}\nnzc {nzc}\nh {h!r}\noldvfil {oldvfil}\ntgrdfil grid.txt\n"
             f"finevel {finevel}\nx0 {x0!r}\ny0 {y0!r}\nz0 {z0!r}\n"
             + row("igridx", igridx) + row("igridy", igridy) + row("igridz", igridz))
+
+
+# ------------------------------------------------------------------ location
+
+def loc_params(g: synth.GridSpec, **kw) -> dict:
+    """sphfdloc's search grid for the tables of grid g, set up as the program
+    does (src/sphfdloc.c:382-393: radians, geocentric colatitude, spacings
+    from h when dq and df are 0), with its parameter defaults (111-128)."""
+    degrad = float(np.float32(0.017453292))     # sphfdloc.c:26, a float constant
+    hpi = float(np.float32(1.570796))
+    colat = hpi - synth.glath(g.y0 * degrad, g.z0)[0]
+    d = dict(nx=g.nx, ny=g.ny, nz=g.nz, x0=g.x0 * degrad, y0=colat, z0=g.z0, h=g.h, dq=g.h / 6371.0,
+             df=abs(g.h / (6371.0 * math.sin(colat))), kmin=2, ndiv=20, ndiv2=20, ivs=1, vpvs=1.78, nthres=8,
+             resthres=0.5, resthrep=5.0, stdmax=15.0)
+    d.update(kw)
+    return d
+
+
+def loc_events(g: synth.GridSpec, time_at, ntables: int, count: int = 256, seed: int = 7,
+               picks: tuple = (24, 40)) -> list:
+    """Synthetic events for the tables 0..ntables-1 (P and S of each station in
+    turn): a source node below layer 2, between picks[0] and picks[1] distinct
+    station phases, each arrival the table time at the source plus an origin
+    offset and 0.05 s noise; about one pick in twenty late by 2-6 s (an
+    outlier) and one in thirty flagged unusable.  time_at(q, k, j, i) gives
+    table q's time at a node."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        i, j, k = int(rng.integers(0, g.nx)), int(rng.integers(0, g.ny)), int(rng.integers(2, g.nz - 1))
+        n = int(rng.integers(picks[0], picks[1] + 1))
+        qs = rng.choice(ntables, size=n, replace=False)
+        t0 = rng.uniform(0.0, 5.0)
+        obs = np.array([time_at(int(q), k, j, i) + t0 + rng.normal(0.0, 0.05) for q in qs], dtype=np.float32)
+        late = rng.random(n) < 0.05
+        obs[late] += rng.uniform(2.0, 6.0, int(late.sum())).astype(np.float32)
+        good = (rng.random(n) >= 1.0 / 30).astype(np.int32)
+        rwt = rng.choice(np.array([0.05, 0.1, 0.2], dtype=np.float32), size=n)
+        out.append(dict(qs=qs, obstime=obs, pwt=(np.float32(1.0) / (rwt * rwt)).astype(np.float32),
+                        phase="".join("P" if q % 2 == 0 else "S" for q in qs), isgood=good, src=(i, j, k)))
+    return out

@@ -149,6 +149,11 @@ void fdgpu_loc_params_defaults(fdgpu_loc_params *p);
  * between passes.  FDGPU_EINPUT for bad parameters, an unknown table or one
  * whose dimensions are not the grid's, or a phase other than P and S. */
 int fdgpu_locate(fdgpu_ctx *ctx, const fdgpu_loc_params *p, fdgpu_loc_event *events, int n);
+/* Measurements of the last fdgpu_locate (no reference counterpart): t[0]
+ * search passes, t[1] device time of the passes (ms), t[2] of the node search
+ * alone (ms), t[3] the table bytes the node searches read (4 bytes per table
+ * a usable pick names, per searched node, per pass); n <= 4 values. */
+int fdgpu_locate_timing(fdgpu_ctx *ctx, double *t, int n);
 
 /* One table request.  Table node (0,0,0) sits at model node (ox, oy, oz); the
  * table spans par.nx*par.ny*par.nz nodes inside the model. */

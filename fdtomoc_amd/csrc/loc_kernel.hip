@@ -3,11 +3,12 @@
 //
 // One search pass of a batch of events is seven launches on one stream:
 //   loc_init_kernel     per event: clear the running minimum
-//   loc_nodes_kernel    nodes of layers kmin.. across the lanes, every event of
-//                       the batch in turn over its usable picks only: a pick's
-//                       table value is one coalesced global load (eight in
-//                       flight per wave), and the events that share stations
-//                       re-read the same node tile from cache
+//   loc_nodes_kernel    nodes of layers kmin.. across the lanes, one group of
+//                       the batch's events in turn over their usable picks
+//                       only: a pick's table value is one coalesced global
+//                       load (eight in flight per wave); the blocks of a die
+//                       that run together share node tiles, so the events of
+//                       all groups re-read the same lines from its L2
 //   loc_commit_kernel   per event: decode the winning trial, evaluate it again
 //                       (residuals and computed times of every pick, the mean
 //                       residual), move the hypocentre
@@ -93,16 +94,28 @@ __device__ __forceinline__ void node_sums(const LocGrid &g, const LocSPick *__re
     }
 }
 
+constexpr int kXcds = 8;   // workgroups are dealt to the accelerator dies in turn
+
+// Block b runs on die b % kXcds.  The blocks a die holds at once should share
+// node tiles, so that the tables' lines of those tiles stay in its L2 while
+// every event reads them: die x takes the tiles x, x + kXcds, ..., and
+// ngroups consecutive blocks of a die take the same tile, each with one
+// group of the events.
 __global__ __launch_bounds__(kThreads) void loc_nodes_kernel(const LocGrid g, const LocSPick *__restrict__ sp,
                                                              const LocSEvent *__restrict__ sev,
                                                              const unsigned char *__restrict__ sc,
                                                              const int *__restrict__ active, int nactive,
-                                                             LocState *state) {
+                                                             int ngroups, LocState *state) {
+    const unsigned x = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const long long tile = (long long)(local / ngroups) * kXcds + x;
+    const int grp = (int)(local % ngroups);
     const long long nxy = (long long)g.nx * g.ny;
     const long long n0 = nxy * g.kmin;
-    const long long n = n0 + (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n0 + tile * kThreads >= nxy * g.nz) return;   // the whole block: beyond the last tile
+    const long long n = n0 + tile * kThreads + threadIdx.x;
     const bool in = n < nxy * g.nz;
-    for (int a = 0; a < nactive; a++) {
+    const int a0 = (int)((long long)grp * nactive / ngroups), a1 = (int)((long long)(grp + 1) * nactive / ngroups);
+    for (int a = a0; a < a1; a++) {
         const int ev = active[a];
         const LocSEvent E = sev[ev];
         unsigned long long key = locdev::kNone;
@@ -212,9 +225,15 @@ extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_p
     const int small = (nactive + 63) / 64;
     hipLaunchKernelGGL(loc_init_kernel, dim3(small), dim3(64), 0, stream, d_active, nactive, d_state);
     if (node0) (void)hipEventRecord(node0, stream);
-    if (nodes > 0)
-        hipLaunchKernelGGL(loc_nodes_kernel, dim3((unsigned)((nodes + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           stream, g, d_spicks, d_sevents, d_sscale, d_active, nactive, d_state);
+    if (nodes > 0) {
+        // groups of at least 16 events, at most 8: a die then holds about
+        // 8 x 32 CUs / ngroups tiles at once, their table lines within its L2
+        const int ngroups = nactive >= 128 ? 8 : nactive >= 16 ? nactive / 16 : 1;
+        const long long tiles = (nodes + kThreads - 1) / kThreads;
+        const long long per_die = (tiles + kXcds - 1) / kXcds;
+        hipLaunchKernelGGL(loc_nodes_kernel, dim3((unsigned)(per_die * ngroups * kXcds)), dim3(kThreads), 0, stream, g,
+                           d_spicks, d_sevents, d_sscale, d_active, nactive, ngroups, d_state);
+    }
     if (node1) (void)hipEventRecord(node1, stream);
     hipLaunchKernelGGL(loc_commit_kernel, dim3(small), dim3(64), 0, stream, g, 0, d_picks, d_events, d_active,
                        nactive, d_state, d_res, d_tps);

@@ -111,7 +111,9 @@ def _check(ctx, p, evs):
     grid = ob.loc_grid(**p)
     for k, (ev, r) in enumerate(zip(evs, got)):
         o = ob.locate(grid, ev["tabs"], ev["obstime"], ev["pwt"], ev["phase"], ev["isgood"])
-        print(f"event {k}: status {r['status']} passes {r['passes']} node {r['node']} std {r['stdmin']}", flush=True)
+        if k < 40:
+            print(f"event {k}: status {r['status']} passes {r['passes']} node {r['node']} std {r['stdmin']}",
+                  flush=True)
         for f in ("status", "ngood", "passes", "node", "exm", "eym", "ezm"):
             assert r[f] == o[f], (k, f, r[f], o[f])
         assert _bits([r["stdmin"], r["avrmin"]], [o["stdmin"], o["avrmin"]]) == 0, k
@@ -174,3 +176,41 @@ def test_locate_errors(ctx, tables):
         with pytest.raises(fdgpu.FdgpuError) as e:
             ctx.locate(fdgpu.loc_params(**bad_p), [bad_ev])
         assert e.value.code == -5
+
+
+def test_many_tables_and_event_chunks(ctx):
+    """140 tables and 1100 events in one call, one of them naming 130
+    distinct tables: all equal to the oracle."""
+    from fdtomoc_amd import fdgpu, synth
+    nx, ny, nz = 21, 19, 12
+    g = synth.centred_grid(nx, ny, nz, 1.0)
+    v = synth.gradient_velocity(g, 5.0, 7.0, 12.0).astype(np.float32)
+    m = ctx.model(v)
+    rng = np.random.default_rng(81)
+    specs = []
+    for s in range(140):
+        st = synth.random_surface_stations(g, np.random.default_rng(1000 + s), 1)[0]
+        par = ob.Par(nx=nx, ny=ny, nz=nz, h=g.h, x0=g.x0, y0=g.y0, z0=g.z0, fxs=st.lon, fys=st.lat, fzs=st.depth,
+                     reverse=50)
+        specs.append(fdgpu.TableSpec(par_dict(par), m))
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    host = [ctx.read(i, (nz, ny, nx)) for i in ids]
+    p = dict(ob.LOC_DEFAULTS, nx=nx, ny=ny, nz=nz, x0=2.1, y0=0.9, z0=-1.0, h=1.0, dq=1.0 / 6371.0,
+             df=1.0 / (6371.0 * math.sin(0.9)), ndiv=6, ndiv2=4)
+    grid = ob.loc_grid(**p)
+    evs = []
+    for e in range(1100):
+        n = 130 if e == 5 else int(rng.integers(12, 30))
+        qs = rng.choice(140, size=n, replace=False)
+        src = (rng.uniform(1, nx - 2), rng.uniform(1, ny - 2), rng.uniform(2.5, nz - 2))
+        x, y, z = p["x0"] + src[0] * p["df"], p["y0"] + src[1] * p["dq"], p["z0"] + src[2] * p["h"]
+        obs = np.array([ob.loc_find_time(grid, host[q], x, y, z) + 2.0 + rng.normal(0, 0.02) for q in qs],
+                       dtype=np.float32)
+        evs.append(dict(table=[ids[q] for q in qs], tabs=[host[q] for q in qs], obstime=obs,
+                        pwt=np.ones(n, np.float32), phase="P" * n, isgood=np.ones(n, np.int32)))
+    try:
+        _check(ctx, p, evs)
+    finally:
+        for i in ids:
+            ctx.release(i)
+        ctx.release_model(m)

@@ -65,6 +65,10 @@ BYTES_PER_VISIT = 12          # slowness read + time read + time write per node 
 # (profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md)
 TRAFFIC_BYTES_C2 = 323.92e9
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
+# HBM bytes per node-search launch of the location workload (average over
+# its 5 passes, FETCH_SIZE x 1.99992 + WRITE_SIZE, copy-calibrated)
+# (profiles/loc_nodes_kernel_256events_96tables.md)
+TRAFFIC_BYTES_LOC = 1.0731e9
 
 
 # ------------------------------------------------------------------ helpers
@@ -605,7 +609,8 @@ def run_loc(args):
         "device_ms_per_step": round(float(np.mean(pass_ms)), 3),
         "located": located, "max_node_offset_from_source": max(err) if err else None,
         "roofline": {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "loc_nodes_kernel",
+                     "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_LOC,
+                     "traffic_source": "profiles/loc_nodes_kernel_256events_96tables.md", "kernel": "loc_nodes_kernel",
                      "kernel_ms": round(float(np.sum(node_ms)) / float(np.sum(passes)), 4),
                      "algorithmic_bytes": float(np.sum(node_bytes)) / float(np.sum(passes)),
                      "what": "per launch: 4 bytes per table a usable pick of the pass names, per searched node"},

@@ -1,6 +1,8 @@
 /*
  * fdtomoc_amd.h -- C ABI of the MI355X (gfx950) implementation of FDtomoC's
- * spherical finite-difference eikonal solver `sphfd`.
+ * spherical finite-difference eikonal solver `sphfd`, with the steps either
+ * side of it on the device: c2f's coarse-to-fine model interpolation and
+ * sphfdloc's grid-search location over the resident tables.
  *
  * Two layers:
  *

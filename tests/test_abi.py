@@ -83,3 +83,16 @@ def test_serve_request_bytes_without_gpu():
     with pytest.raises(fdgpu.FdgpuError) as e:
         fdgpu.serve_request_bytes(coarse=c, nx=nx + 1, ny=ny, nz=nz)
     assert e.value.code == -5
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_loc_params_defaults_without_gpu():
+    """fdgpu_loc_params_defaults (host only): sphfdloc's parameter defaults
+    (src/sphfdloc.c:111-128), grid fields zero."""
+    import numpy as np
+    from fdtomoc_amd import fdgpu
+    p = fdgpu.loc_params()
+    assert (p.kmin, p.ndiv, p.ndiv2, p.ivs, p.nthres) == (2, 20, 20, 1, 8)
+    assert p.vpvs == np.float32(1.78) and p.resthres == np.float32(0.5)
+    assert p.resthrep == np.float32(5.0) and p.stdmax == np.float32(15.0)
+    assert (p.nx, p.ny, p.nz, p.x0, p.h) == (0, 0, 0, 0.0, 0.0)

@@ -610,7 +610,7 @@ int fdgpu_table_write(fdgpu_ctx *c, int table, const char *path, const void *vel
     {
         std::lock_guard<std::mutex> lk(c->mu);
         auto it = c->tables.find(table);
-        if (it == c->tables.end()) return FDGPU_EINPUT;
+        if (it == c->tables.end() || it->second.from_file) return FDGPU_EINPUT;
         host.resize((size_t)it->second.n);
         FD_HIP(hipSetDevice(c->device));
         FD_HIP(hipMemcpyAsync(host.data(), it->second.d_time, sizeof(float) * host.size(), hipMemcpyDeviceToHost,
@@ -618,6 +618,14 @@ int fdgpu_table_write(fdgpu_ctx *c, int table, const char *path, const void *vel
         FD_HIP(hipStreamSynchronize(c->stream));
         par = it->second.par;
         prep = it->second.prep;
+    }
+    // NULL: the reference's header for a headerless velocity file
+    // (sphfd.c:615-656, with the velocity swab), as the drop-in derives it
+    fdio::VHead head;
+    if (!vel_header) {
+        memset(&head, 0, sizeof head);
+        head = fdio::table_header(to_values(par), prep, head, false);
+        vel_header = &head;
     }
     return fdio::write_table(path, par, prep, vel_header, host.data());
 }

@@ -36,6 +36,7 @@ struct Table {
     int *d_stats = nullptr;
     std::vector<int> stats;
     fdprep::Prepared prep;
+    bool from_file = false;    // loaded by fdgpu_table_load: no solve inputs behind it
 };
 
 // First-fit allocator of byte ranges inside one block (HBM arena or pinned

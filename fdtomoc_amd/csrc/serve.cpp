@@ -499,6 +499,14 @@ int fdgpu_serve_write(fdgpu_ctx *c, int request, const char *path, const void *v
         par = R->par;
         prep = R->prep;
     }
+    // NULL: the reference's header for a headerless velocity file
+    // (sphfd.c:615-656, with the velocity swab), as the drop-in derives it
+    fdio::VHead head;
+    if (!vel_header) {
+        memset(&head, 0, sizeof head);
+        head = fdio::table_header(to_values(par), prep, head, false);
+        vel_header = &head;
+    }
     return fdio::write_table(path, par, prep, vel_header, host.data());
 }
 

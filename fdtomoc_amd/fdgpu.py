@@ -182,6 +182,7 @@ def lib():
         L.fdgpu_loc_params_defaults.restype = None
         L.fdgpu_locate.argtypes = [C.c_void_p, C.POINTER(CLocParams), C.POINTER(CLocEvent), C.c_int]
         L.fdgpu_locate_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.fdgpu_table_load.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -296,6 +297,14 @@ class Context:
             out.append(dict(status=e.status, ngood=e.ngood, passes=e.passes, node=tuple(e.node), stdmin=e.stdmin,
                             avrmin=e.avrmin, exm=e.exm, eym=e.eym, ezm=e.ezm, isgood=good, res=res, tps=tc))
         return out
+
+    def load_table(self, path: str, shape: tuple) -> int:
+        """A table file (headered or headerless, as FDtomoC's consumers read
+        it) loaded into a new resident table; shape is (nz, ny, nx)."""
+        nz, ny, nx = shape
+        tid = C.c_int()
+        _check(lib().fdgpu_table_load(self._h, os.fsencode(path), nx, ny, nz, C.byref(tid)), "fdgpu_table_load")
+        return tid.value
 
     def locate_timing(self) -> dict:
         """Measurements of the last locate: passes, pass and node-search device

@@ -195,10 +195,19 @@ int fdgpu_set_workgroup(fdgpu_ctx *ctx, int lanes);
 /* Copies a resident table (nx*ny*nz floats) to host memory. */
 int fdgpu_table_read(fdgpu_ctx *ctx, int table, float *host, unsigned long long count);
 /* Writes a resident table as the reference does (src/sphfd.c:5702-5803):
- * `parse` decimation, 232-byte header derived from `vel_header` (232 bytes of
- * the velocity file, or NULL for the reference's headerless fallback),
- * optional byte swap (par.swab 2|3). */
+ * `parse` decimation, 232-byte header derived from `vel_header` (the velocity
+ * file's 232 bytes as the reference holds them, i.e. after its swab 1|3 of
+ * the numeric fields, sphfd.c:668-682; or NULL for the header the reference
+ * builds for a headerless velocity file, sphfd.c:615-656), optional byte swap
+ * (par.swab 2|3). */
 int fdgpu_table_write(fdgpu_ctx *ctx, int table, const char *path, const void *vel_header);
+/* Loads a travel-time table file into a new resident table, read as
+ * FDtomoC's consumers read one (sphrayderv.c:1184-1217, sphfdloc's
+ * read_timefiles): a file whose header's first token is "HEAD" has its
+ * nx*ny*nz native floats after the 232-byte header, any other file from byte
+ * 0.  For fdgpu_locate and fdgpu_table_read; fdgpu_table_write refuses such a
+ * table (no solve inputs behind it).  FDGPU_EIO: missing or short file. */
+int fdgpu_table_load(fdgpu_ctx *ctx, const char *path, int nx, int ny, int nz, int *table);
 /* Sweep counters of a table: passes, restarts, LEVEL faces, CHANGE faces,
  * LEVEL rounds, CHANGE rounds, fallbacks, node evaluations, deferrals, error
  * bits, last restart wall, then the first pass's head-wave counts (7). */

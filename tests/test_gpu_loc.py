@@ -214,3 +214,49 @@ def test_many_tables_and_event_chunks(ctx):
         for i in ids:
             ctx.release(i)
         ctx.release_model(m)
+
+
+def test_loaded_table_files(ctx, tables, tmp_path):
+    """Table files loaded as FDtomoC's consumers read them (sphrayderv.c:
+    1184-1217): files this library writes in the reference layout (header
+    first), headerless files, and a header whose first token follows white
+    space; the loaded tables equal the solved ones and locate the same
+    events bit for bit.  Short files give FDGPU_EIO; writing a loaded table
+    is refused."""
+    from fdtomoc_amd import fdgpu
+    g, ids, host = tables
+    shape = (NZ, NY, NX)
+    loaded = []
+    for k, i in enumerate(ids):
+        path = str(tmp_path / f"t{k}.times")
+        if k % 3 == 0:
+            ctx.write(i, path, None)                      # reference layout with its header
+        elif k % 3 == 1:
+            host[k].astype("<f4").tofile(path)             # headerless
+        else:
+            head = bytearray(232)
+            head[:8] = b"  HEADFI"
+            with open(path, "wb") as f:
+                f.write(bytes(head))
+                f.write(host[k].astype("<f4").tobytes())
+        loaded.append(ctx.load_table(path, shape))
+    for k, t in enumerate(loaded):
+        assert _bits(ctx.read(t, shape), host[k]) == 0, k
+    p = _params(g, **ob.LOC_DEFAULTS)
+    evs = _events(g, ids, host, p, seed=5, count=6)
+    a = ctx.locate(fdgpu.loc_params(**p), evs)
+    index = {i: q for q, i in enumerate(ids)}
+    b = ctx.locate(fdgpu.loc_params(**p), [dict(e, table=[loaded[index[i]] for i in e["table"]]) for e in evs])
+    for x, y in zip(a, b):
+        assert x["node"] == y["node"] and (x["exm"], x["eym"], x["ezm"]) == (y["exm"], y["eym"], y["ezm"])
+        assert _bits([x["stdmin"]], [y["stdmin"]]) == 0 and _bits(x["res"], y["res"]) == 0
+    short = str(tmp_path / "short.times")
+    host[0][:-1].astype("<f4").tofile(short)
+    with pytest.raises(fdgpu.FdgpuError) as e:
+        ctx.load_table(short, (NZ, NY, NX + 1))
+    assert e.value.code == -6
+    with pytest.raises(fdgpu.FdgpuError) as e:
+        ctx.write(loaded[0], str(tmp_path / "again.times"), None)
+    assert e.value.code == -5
+    for t in loaded:
+        ctx.release(t)

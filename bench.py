@@ -29,6 +29,9 @@ Other workloads (one JSON line each, same keys where they apply):
                   path): a 31x31x7 coarse model to the 301x301x61 fine grid
                   into two resident models per step, the reference c2f timed
                   beside it
+  --workload c2w  C2 with every table written to a file each step by the
+                  background writer, overlapping the next solve; the same
+                  steps with synchronous writes beside it
   --workload loc  sphfdloc's grid-search location (the row after c2f): 256
                   events of 24-40 picks over the 96 resident tables of the
                   default workload, all events per step, the oracle's location
@@ -251,6 +254,78 @@ def run_c2(args):
         out["nodes_differing_vs_reference"] = nd
     for i in last:
         ctx.release(i)
+    ctx.close()
+    return out
+
+
+# ------------------------------------------------------------------ C2 with files
+
+def run_c2w(args):
+    """C2 with every table written to a file in the reference layout each step:
+    the step solves the 96 tables and hands them to the background writer
+    (fdgpu_tables_write_async), whose device-to-host copies and file writes
+    overlap the next step's solve; a step ends once the previous step's files
+    are written.  Beside it: the same steps with the files written after each
+    solve (fdgpu_table_write, one table after another)."""
+    g, vp, vs, stations = workloads.c2()
+    ctx = fdgpu.Context(0)
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs = [fdgpu.TableSpec(workloads.table_par(g, st), m) for st in stations for m in (mp, ms)]
+    ntab = len(specs)
+    sched = tune(ctx, specs, args)
+    outdir = tempfile.mkdtemp(prefix="sphfd_tables_")
+    paths = [os.path.join(outdir, f"t{k:03d}.times") for k in range(ntab)]
+    try:
+        def step_overlapped(prev):
+            ids = ctx.solve(specs, fdgpu.BATCHED)
+            if prev:
+                ctx.write_wait()
+                for i in prev:
+                    ctx.release(i)
+            ctx.write_async(ids, paths)
+            return ids
+
+        prev = None
+        for _ in range(args.warmup):
+            prev = step_overlapped(prev)
+        walls = []
+        for s in range(args.steps):
+            t0 = time.perf_counter()
+            prev = step_overlapped(prev)
+            walls.append(time.perf_counter() - t0)
+            print(f"# overlapped step {s}: {walls[-1]:.3f} s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        ctx.write_wait()
+        drain = time.perf_counter() - t0
+        for i in prev:
+            ctx.release(i)
+        seq = []
+        for s in range(max(1, min(args.steps, 2))):
+            t0 = time.perf_counter()
+            ids = ctx.solve(specs, fdgpu.BATCHED)
+            for i, p in zip(ids, paths):
+                ctx.write(i, p, None)
+            seq.append(time.perf_counter() - t0)
+            for i in ids:
+                ctx.release(i)
+            print(f"# sequential step {s}: {seq[-1]:.3f} s", file=sys.stderr, flush=True)
+        file_bytes = sum(os.path.getsize(p) for p in paths)
+    finally:
+        shutil.rmtree(outdir, ignore_errors=True)
+    wall = float(np.mean(walls))
+    out = base_line(args, round(ntab / wall, 3), "station-solves/s (tables written to files)", round(wall * 1e3, 2),
+                    {"workload": "C2 with files: 96 tables 161x161x81 solved per step and written in the "
+                                 "reference layout by the background writer while the next step solves",
+                     "tables": ntab, "nodes_per_table": g.n})
+    out["metric"] = METRIC + " | with table files"
+    out["schedule"] = sched
+    out["sequential_station_solves_per_s"] = round(ntab / float(np.mean(seq)), 3)
+    out["bytes_written_per_step"] = file_bytes
+    out["final_drain_s"] = round(drain, 3)
+    out["cpu_baseline"] = None
+    out["roofline"] = None
+    ctx.release_model(mp)
+    ctx.release_model(ms)
     ctx.close()
     return out
 
@@ -656,7 +731,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "c2f", "loc"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2w", "c3", "c4", "c5", "c2f", "loc"])
     ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -677,7 +752,7 @@ def main():
         print("bench.py: this build measures one GPU; the station tables shard across GPUs with no "
               "communication on the hot path, so multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
-    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f, "loc": run_loc}[args.workload]
+    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f, "loc": run_loc, "c2w": run_c2w}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

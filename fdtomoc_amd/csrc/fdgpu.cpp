@@ -310,6 +310,7 @@ int fdgpu_open(int device, unsigned long long budget, fdgpu_ctx **out) {
 
 int fdgpu_close(fdgpu_ctx *c) {
     if (!c) return FDGPU_OK;
+    writer_shutdown(c);   // background writes finish before their tables go
     (void)hipSetDevice(c->device);
     {
         std::lock_guard<std::mutex> lk(c->mu);
@@ -587,6 +588,7 @@ int fdgpu_table_release(fdgpu_ctx *c, int table) {
     if (c->serve.on) return FDGPU_EBUSY;
     auto it = c->tables.find(table);
     if (it == c->tables.end()) return FDGPU_EINPUT;
+    if (writer_busy(c, table)) return FDGPU_EBUSY;   // a background write still reads it
     (void)hipSetDevice(c->device);
     release_table(c, it->second);
     c->tables.erase(it);

@@ -5,8 +5,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
+#include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fdtomoc_amd.h"
@@ -37,6 +42,36 @@ struct Table {
     std::vector<int> stats;
     fdprep::Prepared prep;
     bool from_file = false;    // loaded by fdgpu_table_load: no solve inputs behind it
+};
+
+// One background table write (fdgpu_tables_write_async): what the file
+// needs, captured at submission.
+struct WriteJob {
+    int table = 0;
+    const float *d_time = nullptr;
+    long long n = 0;
+    fdgpu_par par{};
+    fdprep::Prepared prep;
+    bool has_head = false;
+    unsigned char head[232] = {};
+    std::string path;
+};
+
+// The background writer: a thread with its own copy stream and two pinned
+// buffers, so that table k + 1 is copied while table k is written.
+struct Writer {
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv, idle;
+    std::deque<WriteJob> q;
+    int inflight = 0;               // jobs submitted and not yet written
+    int err = 0;                    // first error since the last wait
+    bool stop = false;
+    std::map<int, int> busy;        // table id -> its jobs in flight
+    hipStream_t stream = nullptr;
+    hipEvent_t done[2] = {nullptr, nullptr};
+    float *pinned[2] = {nullptr, nullptr};
+    size_t pinned_bytes = 0;
 };
 
 // First-fit allocator of byte ranges inside one block (HBM arena or pinned
@@ -171,12 +206,17 @@ struct fdgpu_ctx {
     double loc_timing[4] = {0.0, 0.0, 0.0, 0.0};
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     fdgpu_detail::Serve serve;
+    std::unique_ptr<fdgpu_detail::Writer> writer;   // created by the first background write
     std::mutex mu;
 };
 
 namespace fdgpu_detail {
 
 int hip_fail(hipError_t e);
+// Background writer: whether `table` has writes in flight; stopping it
+// (waits for every write, joins the thread, frees its buffers).
+bool writer_busy(fdgpu_ctx *c, int table);
+void writer_shutdown(fdgpu_ctx *c);
 int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes);
 void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
 size_t slot_bytes(long long F);

@@ -183,6 +183,8 @@ def lib():
         L.fdgpu_locate.argtypes = [C.c_void_p, C.POINTER(CLocParams), C.POINTER(CLocEvent), C.c_int]
         L.fdgpu_locate_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
         L.fdgpu_table_load.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.fdgpu_tables_write_async.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.fdgpu_write_wait.argtypes = [C.c_void_p]
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
@@ -297,6 +299,22 @@ class Context:
             out.append(dict(status=e.status, ngood=e.ngood, passes=e.passes, node=tuple(e.node), stdmin=e.stdmin,
                             avrmin=e.avrmin, exm=e.exm, eym=e.eym, ezm=e.ezm, isgood=good, res=res, tps=tc))
         return out
+
+    def write_async(self, tables: list, paths: list, vel_headers: list | None = None):
+        """Background writes of resident tables (fdgpu_tables_write_async);
+        fdgpu_write_wait (write_wait) reports their outcome."""
+        n = len(tables)
+        ids = np.ascontiguousarray(tables, dtype=np.int32)
+        enc = [C.create_string_buffer(os.fsencode(p)) for p in paths]
+        pp = (C.c_char_p * n)(*[C.cast(e, C.c_char_p) for e in enc])
+        hb, hp = [], None
+        if vel_headers is not None:
+            hb = [None if h is None else C.create_string_buffer(bytes(h), 232) for h in vel_headers]
+            hp = (C.c_void_p * n)(*[None if h is None else C.cast(h, C.c_void_p) for h in hb])
+        _check(lib().fdgpu_tables_write_async(self._h, ids.ctypes.data, pp, hp, n), "fdgpu_tables_write_async")
+
+    def write_wait(self):
+        _check(lib().fdgpu_write_wait(self._h), "fdgpu_write_wait")
 
     def load_table(self, path: str, shape: tuple) -> int:
         """A table file (headered or headerless, as FDtomoC's consumers read

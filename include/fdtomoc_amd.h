@@ -208,6 +208,18 @@ int fdgpu_table_write(fdgpu_ctx *ctx, int table, const char *path, const void *v
  * 0.  For fdgpu_locate and fdgpu_table_read; fdgpu_table_write refuses such a
  * table (no solve inputs behind it).  FDGPU_EIO: missing or short file. */
 int fdgpu_table_load(fdgpu_ctx *ctx, const char *path, int nx, int ny, int nz, int *table);
+/* Writes n resident tables as fdgpu_table_write does, in the background: a
+ * worker thread copies them to pinned host memory on a stream of its own and
+ * writes the files, so the copies and file writes overlap the caller's next
+ * fdgpu_solve (sphfd.c:5702-5803 off the critical path; no reference
+ * counterpart).  vel_headers: NULL, or n pointers each NULL or 232 bytes as
+ * fdgpu_table_write takes them; paths and headers are copied.  A table with
+ * writes in flight cannot be released (FDGPU_EBUSY). */
+int fdgpu_tables_write_async(fdgpu_ctx *ctx, const int *tables, const char *const *paths,
+                             const void *const *vel_headers, int n);
+/* Waits for every background write; returns the first error since the last
+ * wait (FDGPU_EIO for a file that could not be written), or 0. */
+int fdgpu_write_wait(fdgpu_ctx *ctx);
 /* Sweep counters of a table: passes, restarts, LEVEL faces, CHANGE faces,
  * LEVEL rounds, CHANGE rounds, fallbacks, node evaluations, deferrals, error
  * bits, last restart wall, then the first pass's head-wave counts (7). */

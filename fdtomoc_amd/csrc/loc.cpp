@@ -62,7 +62,7 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
     if (P.nx < 2 || P.ny < 2 || P.nz < 2 || P.kmin < 0 || P.ndiv < 1 || P.ndiv2 < 1 || P.ndiv > kMaxDiv ||
         P.ndiv2 > kMaxDiv)
         return FDGPU_EINPUT;
-    if ((long long)P.nx * P.ny * P.nz >= (1LL << 31)) return FDGPU_EINPUT;
+    if ((long long)P.nx * P.ny * P.nz >= (1LL << 30)) return FDGPU_EINPUT;   // 32-bit byte offsets
     if (!(P.df > 0.0) || !(P.dq > 0.0) || !(P.h > 0.0)) return FDGPU_EINPUT;
     if (nev == 0) return FDGPU_OK;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -139,6 +139,7 @@ int fdgpu_locate(fdgpu_ctx *c, const fdgpu_loc_params *lp, fdgpu_loc_event *ev, 
     g.nx = P.nx; g.ny = P.ny; g.nz = P.nz;
     g.kmin = P.kmin; g.ndiv = P.ndiv; g.ndiv2 = P.ndiv2;
     g.vpvs = P.vpvs;
+    g.tbytes = (unsigned)(4LL * P.nx * P.ny * P.nz);
     g.x0 = P.x0; g.y0 = P.y0; g.z0 = P.z0; g.df = P.df; g.dq = P.dq; g.h = P.h;
 
     std::vector<LocState> st((size_t)nev);

@@ -109,10 +109,52 @@ __device__ __forceinline__ void add2(float &avres, float &rsq, float r, float w)
 // the single float rounding (Figueroa's double-rounding bound).
 __device__ __forceinline__ float residual_unscaled(float obs, float t) { return obs - t; }
 
+// A pick's table as a buffer resource: its base in scalar registers and its
+// size as the bound (raw 32-bit access; dword 3 as for every gfx9 target).
+// A load then takes only the lane's byte offset: a refinement point's eight
+// corner loads per pick need no vector address arithmetic.  The host keeps tables below 2^30 nodes, so
+// byte offsets and sizes fit 32 bits.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const float *t, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)t, (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off4) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off4, 0, 0));
+}
+
 // Table value through the global address space (a pick's table pointer is
-// read from memory, so the compiler would otherwise use a flat load).
+// read from memory, so the compiler would otherwise use a flat load).  The
+// node search takes one value per table, so building a buffer resource per
+// pick would cost more scalar instructions than the address add it saves.
 __device__ __forceinline__ float gload(const float *t, long long n) {
     return *((const __attribute__((address_space(1))) float *)t + n);
+}
+
+// The byte offsets of a cell's eight corners in find_time's term order,
+// once per point for all of its picks' tables.
+__device__ __forceinline__ void corner_offsets(const Cell &c, long long sy, long long sz, unsigned o[8]) {
+    const unsigned b = (unsigned)c.base * 4u, y = (unsigned)sy * 4u, z = (unsigned)sz * 4u;
+    o[0] = b;
+    o[1] = b + 4u;
+    o[2] = b + y;
+    o[3] = b + z;
+    o[4] = b + y + 4u;
+    o[5] = b + z + 4u;
+    o[6] = b + z + y;
+    o[7] = b + z + y + 4u;
+}
+
+// interp() through a table's buffer resource: the same terms in the same order.
+__device__ __forceinline__ double interp_b(__amdgpu_buffer_rsrc_t r, const Cell &c, const unsigned o[8]) {
+    double tp = c.w[0] * (double)bload(r, o[0]);
+    tp = tp + c.w[1] * (double)bload(r, o[1]);
+    tp = tp + c.w[2] * (double)bload(r, o[2]);
+    tp = tp + c.w[3] * (double)bload(r, o[3]);
+    tp = tp + c.w[4] * (double)bload(r, o[4]);
+    tp = tp + c.w[5] * (double)bload(r, o[5]);
+    tp = tp + c.w[6] * (double)bload(r, o[6]);
+    tp = tp + c.w[7] * (double)bload(r, o[7]);
+    return tp;
 }
 
 // Ordering key of a trial that beats `bound` (the strict comparison of

@@ -148,11 +148,12 @@ __global__ __launch_bounds__(kThreads) void loc_refine_kernel(const LocGrid g, i
         double pt[3];
         locdev::box_point(b, q, pt);
         const locdev::Cell c = locdev::cell_of(g, pt[0], pt[1], pt[2]);
-        const long long sy = g.nx, sz = (long long)g.nx * g.ny;
+        unsigned off[8];
+        locdev::corner_offsets(c, g.nx, (long long)g.nx * g.ny, off);
         float avres = 0.0f, rsq = 0.0f;
 #pragma unroll 2
         for (int p = E.first; p < E.first + E.count; p++) {
-            double t = locdev::interp(sp[p].table, c, sy, sz);
+            double t = locdev::interp_b(locdev::table_rsrc(sp[p].table, g.tbytes), c, off);
             if (E.scaled && sc[p]) t *= (double)g.vpvs;
             locdev::add2(avres, rsq, (float)((double)sp[p].obs - t), sp[p].pwt);
         }

@@ -13,7 +13,7 @@ struct LocGrid {
     int nx, ny, nz;
     int kmin, ndiv, ndiv2;
     float vpvs;
-    int pad;
+    unsigned tbytes;   // bytes of one table (nx*ny*nz < 2^30)
     double x0, y0, z0, df, dq, h;
 };
 

@@ -148,7 +148,7 @@ void fdgpu_loc_params_defaults(fdgpu_loc_params *p);
 /* Locates n events (replaces sphfdloc's per-event loop, sphfdloc.c:752-1130,
  * and its table reads): each search pass runs on the device for every event
  * still being located; the outlier and retry decisions run on the host
- * between passes.  FDGPU_EINPUT for bad parameters, an unknown table or one
+ * between passes.  Grids below 2^30 nodes.  FDGPU_EINPUT for bad parameters, an unknown table or one
  * whose dimensions are not the grid's, or a phase other than P and S. */
 int fdgpu_locate(fdgpu_ctx *ctx, const fdgpu_loc_params *p, fdgpu_loc_event *events, int n);
 /* Measurements of the last fdgpu_locate (no reference counterpart): t[0]

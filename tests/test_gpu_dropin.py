@@ -140,3 +140,58 @@ def test_missing_key_exits_100(workdirs):
     _write_inputs(wd, {}, ["nx=10 ny=10 nz=10 h=1.0 timefile=x.t velfile=v.mod\n"])   # no fxs/fys/fzs
     r = _run(EXE, wd)
     assert r.returncode == 100
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference executable not built")
+def test_location_over_reference_table_files(workdirs):
+    """sphfdloc's inputs as the reference produces them: the reference sphfd
+    writes P and S tables of 8 stations to files; the files are loaded as the
+    consumers read them (fdgpu_table_load) and 12 events located over them
+    (fdgpu_locate), equal field for field to the oracle's location over the
+    same file payloads."""
+    ref_wd, _ = workdirs
+    g, vp = _model(25, 23, 14, 1.0, 11)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    head = tables.make_header(g.nx, g.ny, g.nz, g.x0, g.y0, g.z0, g.df_deg(), g.dq_deg(), g.h)
+    hbytes = np.asarray(head).tobytes()
+    files = {"VP.mod": hbytes + vp.astype("<f4").tobytes(), "VS.mod": hbytes + vs.astype("<f4").tobytes()}
+    sts = synth.random_surface_stations(g, np.random.default_rng(12), 8)
+    pars = []
+    for k, st in enumerate(sts):
+        pars.append(_par(g, st, f"timefile=S{k}.ptimes\nvelfile=VP.mod\n"))
+        pars.append(_par(g, st, f"timefile=S{k}.stimes\nvelfile=VS.mod\n"))
+    _write_inputs(ref_wd, files, pars)
+    r = _run(REF, ref_wd)
+    assert r.returncode == 0, r.stderr[-2000:]
+    names = [f"S{k}.{p}times" for k in range(8) for p in "ps"]
+    ctx = fdgpu.Context(0)
+    try:
+        ids = [ctx.load_table(os.path.join(ref_wd, "out", nm), (g.nz, g.ny, g.nx)) for nm in names]
+        payload = [tables.read_grid(os.path.join(ref_wd, "out", nm)).data.reshape(g.nz, g.ny, g.nx) for nm in names]
+        for i, t in zip(ids, payload):
+            assert np.array_equal(ctx.read(i, (g.nz, g.ny, g.nx)).view(np.uint32), t.view(np.uint32))
+        degrad = float(np.float32(0.017453292))
+        colat = float(np.float32(1.570796)) - synth.glath(g.y0 * degrad, g.z0)[0]
+        p = dict(ob.LOC_DEFAULTS, nx=g.nx, ny=g.ny, nz=g.nz, x0=g.x0 * degrad, y0=colat, z0=g.z0, h=g.h,
+                 dq=g.h / 6371.0, df=abs(g.h / (6371.0 * np.sin(colat))), ndiv=10, ndiv2=8)
+        rng = np.random.default_rng(13)
+        evs = []
+        for e in range(12):
+            i, j, k = int(rng.integers(1, g.nx - 1)), int(rng.integers(1, g.ny - 1)), int(rng.integers(3, g.nz - 1))
+            q = rng.choice(16, size=int(rng.integers(9, 16)), replace=False)
+            obs = np.array([payload[int(x)][k, j, i] + 1.5 + rng.normal(0, 0.04) for x in q], dtype=np.float32)
+            ph = "".join("P" if x % 2 == 0 else "S" for x in q)
+            evs.append(dict(table=[ids[int(x)] for x in q], tabs=[payload[int(x)] for x in q], obstime=obs,
+                            pwt=np.ones(len(q), np.float32), phase=ph, isgood=np.ones(len(q), np.int32)))
+        got = ctx.locate(fdgpu.loc_params(**p), evs)
+        grid = ob.loc_grid(**p)
+        for ev, res in zip(evs, got):
+            o = ob.locate(grid, ev["tabs"], ev["obstime"], ev["pwt"], ev["phase"], ev["isgood"])
+            for f in ("status", "ngood", "passes", "node", "exm", "eym", "ezm"):
+                assert res[f] == o[f], f
+            assert np.float32(res["stdmin"]).view(np.uint32) == np.float32(o["stdmin"]).view(np.uint32)
+            assert np.array_equal(res["res"].view(np.uint32), o["res"].view(np.uint32))
+        for i in ids:
+            ctx.release(i)
+    finally:
+        ctx.close()

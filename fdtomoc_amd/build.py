@@ -31,7 +31,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [("sweep_kernel.hip", True), ("coarse_kernel.hip", True), ("fdgpu.cpp", False), ("sphfd_io.cpp", False),
            ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False), ("coarse.cpp", False),
            ("loc_kernel.hip", True), ("loc.cpp", False), ("table_io.cpp", False),
-           ("table_writer.cpp", False)]
+           ("table_writer.cpp", False), ("loc_geo.cpp", False)]
 
 
 def _run(cmd, verbose):

@@ -85,6 +85,24 @@ def loc_params(**kw) -> CLocParams:
     return p
 
 
+def loc_params_from_grid(nx: int, ny: int, nz: int, x0: float, y0: float, z0: float, h: float,
+                         dq: float = 0.0, df: float = 0.0):
+    """(fdgpu_loc_params, z0r): sphfdloc's search grid from the spec's grid
+    (degrees, geodetic latitude, km), as sphfdloc.c:382-393 sets it up."""
+    p, r = CLocParams(), C.c_double()
+    _check(lib().fdgpu_loc_params_from_grid(nx, ny, nz, x0, y0, z0, h, dq, df, C.byref(p), C.byref(r)),
+           "fdgpu_loc_params_from_grid")
+    return p, r.value
+
+
+def loc_geographic(exm: float, eym: float, ezm: float, z0: float, z0r: float):
+    """(latitude deg, longitude deg, depth km) of a hypocentre as sphfdloc prints it."""
+    la, lo, d = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().fdgpu_loc_geographic(exm, eym, ezm, z0, z0r, C.byref(la), C.byref(lo), C.byref(d)),
+           "fdgpu_loc_geographic")
+    return la.value, lo.value, d.value
+
+
 @dataclasses.dataclass
 class Coarse:
     """A coarse model: node counts, fine intervals per coarse interval, the
@@ -182,6 +200,9 @@ def lib():
         L.fdgpu_loc_params_defaults.restype = None
         L.fdgpu_locate.argtypes = [C.c_void_p, C.POINTER(CLocParams), C.POINTER(CLocEvent), C.c_int]
         L.fdgpu_locate_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.fdgpu_loc_params_from_grid.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [
+            C.POINTER(CLocParams), C.POINTER(C.c_double)]
+        L.fdgpu_loc_geographic.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_double)] * 3
         L.fdgpu_table_load.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
         L.fdgpu_tables_write_async.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.fdgpu_write_wait.argtypes = [C.c_void_p]

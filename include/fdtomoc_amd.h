@@ -145,6 +145,17 @@ typedef struct {
 
 /* sphfdloc's parameter defaults (sphfdloc.c:111-128); grid fields zero. */
 void fdgpu_loc_params_defaults(fdgpu_loc_params *p);
+/* The defaults plus the search grid as sphfdloc sets it up from the spec's
+ * grid (sphfdloc.c:382-393): origin longitude/latitude in degrees (latitude
+ * geodetic), depth z0 and spacing h in km, dq/df in degrees (0: from h).
+ * z0r receives the radius of the grid's origin (km), for fdgpu_loc_geographic. */
+int fdgpu_loc_params_from_grid(int nx, int ny, int nz, double x0_deg, double y0_deg, double z0, double h,
+                               double dq_deg, double df_deg, fdgpu_loc_params *p, double *z0r);
+/* A hypocentre (exm, eym, ezm of fdgpu_loc_event) as sphfdloc prints it
+ * (sphfdloc.c:1131-1133): longitude and geodetic latitude in degrees and the
+ * ellipsoidal depth in km (WGS84, geographic_method.c glathinv). */
+int fdgpu_loc_geographic(double exm, double eym, double ezm, double z0, double z0r, double *lat_deg,
+                         double *lon_deg, double *depth);
 /* Locates n events (replaces sphfdloc's per-event loop, sphfdloc.c:752-1130,
  * and its table reads): each search pass runs on the device for every event
  * still being located; the outlier and retry decisions run on the host

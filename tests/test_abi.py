@@ -96,3 +96,28 @@ def test_loc_params_defaults_without_gpu():
     assert p.vpvs == np.float32(1.78) and p.resthres == np.float32(0.5)
     assert p.resthrep == np.float32(5.0) and p.stdmax == np.float32(15.0)
     assert (p.nx, p.ny, p.nz, p.x0, p.h) == (0, 0, 0, 0.0, 0.0)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_loc_geography_without_gpu():
+    """fdgpu_loc_params_from_grid and fdgpu_loc_geographic (host only): the
+    grid set-up equals the independent Python set-up of the same formulas
+    (sphfdloc.c:382-393) bit for bit; the hypocentre conversion inverts the
+    forward one (glathinv(glath(lat, depth)) = (lat, depth) to 1e-9 degrees
+    and 1e-6 km over latitudes -85..85 and depths -3..200 km); at the equator
+    the radius is the WGS84 semi-major axis less the depth."""
+    import numpy as np
+    from fdtomoc_amd import fdgpu, synth, workloads
+    g = synth.centred_grid(41, 37, 21, 1.0)
+    p, z0r = fdgpu.loc_params_from_grid(g.nx, g.ny, g.nz, g.x0, g.y0, g.z0, g.h)
+    ref = workloads.loc_params(g)
+    for f in ("x0", "y0", "z0", "h", "dq", "df"):
+        assert getattr(p, f) == ref[f], f
+    assert (p.nx, p.ny, p.nz, p.kmin, p.ndiv) == (41, 37, 21, 2, 20)
+    for lat in np.linspace(-85.0, 85.0, 35):
+        for dep in (-3.0, 0.0, 10.0, 55.5, 200.0):
+            q, r = fdgpu.loc_params_from_grid(5, 5, 5, 121.25, float(lat), dep, 1.0)
+            la, lo, d = fdgpu.loc_geographic(q.x0, q.y0, dep, dep, r)
+            assert abs(la - lat) < 1e-9 and abs(d - dep) < 1e-6 and abs(lo - 121.25) < 1e-12, (lat, dep)
+    _, r = fdgpu.loc_params_from_grid(5, 5, 5, 0.0, 0.0, 12.0, 1.0)
+    assert abs(r - (6378.137 - 12.0)) < 1e-9

@@ -67,6 +67,11 @@ BYTES_PER_VISIT = 12          # slowness read + time read + time write per node 
 # rocprofv3 counted for one launch of the default 96-table workload
 # (profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md)
 TRAFFIC_BYTES_C2 = 323.92e9
+# FP64 lane-operations of one launch of the default workload, from VALU
+# instruction counters (64 x (ADD + MUL + 2 FMA + TRANS), an upper bound)
+# (profiles/sphfd_sweep_kernel_c2_96tables_fp64.md)
+FP64_OPS_C2 = 8.557e12
+FP64_PEAK_TFLOPS = 78.6
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 # HBM bytes per node-search launch of the location workload (average over
 # its 5 passes, FETCH_SIZE x 1.99992 + WRITE_SIZE, copy-calibrated)
@@ -242,6 +247,11 @@ def run_c2(args):
                          "what": "batched solve of all 96 tables (submit to results resident)"}
     out["roofline"] = roofline(visits, float(kern.mean()), TRAFFIC_BYTES_C2,
                                "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md")
+    fp64 = FP64_OPS_C2 / float(kern.mean()) / 1e12
+    out["fp64_rate"] = {"achieved_max": round(fp64, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(fp64 / FP64_PEAK_TFLOPS, 5),
+                        "ops_per_launch": FP64_OPS_C2,
+                        "source": "profiles/sphfd_sweep_kernel_c2_96tables_fp64.md"}
     out["cpu_baseline"] = None
     if not args.no_cpu and os.path.exists(REF_EXE):
         cores = host_cores(args.cpu_threads)

@@ -87,3 +87,16 @@ def test_release_waits_for_writes_and_errors_reported(ctx, tmp_path):
     ctx.write_wait()
     ctx.release(ids[1])
     ctx.release_model(m)
+
+
+def test_close_finishes_queued_writes(tmp_path):
+    """Closing a context with background writes queued completes them first:
+    every file exists in full afterwards."""
+    from fdtomoc_amd import fdgpu, synth
+    c = fdgpu.Context(0)
+    g = synth.centred_grid(21, 19, 10, 1.0)
+    m = c.model(synth.gradient_velocity(g, 5.0, 7.0, 10.0).astype(np.float32))
+    ids = c.solve(_specs(m, g, [600, 601, 602]), fdgpu.BATCHED)
+    c.write_async(ids * 10, [str(tmp_path / f"c{k}.times") for k in range(30)])
+    c.close()
+    assert all(os.path.getsize(tmp_path / f"c{k}.times") == 232 + 4 * g.n for k in range(30))

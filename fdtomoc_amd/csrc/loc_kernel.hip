@@ -29,6 +29,7 @@
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kRowsPerLaunch = 32768;   // grid rows of one launch (the y dimension stops at 65535)
 
 // Block-wide minimum of the lanes' keys, folded into *dst.
 __device__ __forceinline__ void fold_min(unsigned long long key, unsigned long long *dst) {
@@ -241,9 +242,13 @@ extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_p
     for (int stage = 1; stage <= 2; stage++) {
         const long long side = 2LL * (stage == 1 ? g.ndiv : g.ndiv2) + 1;
         const long long pts = side * side * side;
-        hipLaunchKernelGGL(loc_refine_kernel, dim3((unsigned)((pts + kThreads - 1) / kThreads), (unsigned)nactive),
-                           dim3(kThreads), 0, stream, g, stage == 2 ? 1 : 0, d_spicks, d_sevents, d_sscale, d_active,
-                           d_state);
+        // one grid row per event, at most kRowsPerLaunch rows per launch
+        for (int first = 0; first < nactive; first += kRowsPerLaunch) {
+            const int rows = nactive - first < kRowsPerLaunch ? nactive - first : kRowsPerLaunch;
+            hipLaunchKernelGGL(loc_refine_kernel, dim3((unsigned)((pts + kThreads - 1) / kThreads), (unsigned)rows),
+                               dim3(kThreads), 0, stream, g, stage == 2 ? 1 : 0, d_spicks, d_sevents, d_sscale,
+                               d_active + first, d_state);
+        }
         hipLaunchKernelGGL(loc_commit_kernel, dim3(small), dim3(64), 0, stream, g, stage, d_picks, d_events,
                            d_active, nactive, d_state, d_res, d_tps);
     }

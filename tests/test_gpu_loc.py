@@ -260,3 +260,45 @@ def test_loaded_table_files(ctx, tables, tmp_path):
     assert e.value.code == -5
     for t in loaded:
         ctx.release(t)
+
+
+def test_more_events_than_grid_rows(ctx):
+    """70,000 events in one call (more than one launch's 65,535 grid rows in
+    the refinements) on a small grid: a sample of them equal to the oracle,
+    and every event located."""
+    from fdtomoc_amd import fdgpu, synth
+    nx, ny, nz = 9, 8, 7
+    g = synth.centred_grid(nx, ny, nz, 1.0)
+    m = ctx.model(synth.gradient_velocity(g, 5.0, 7.0, 7.0).astype(np.float32))
+    ids = ctx.solve(_table_specs(m, g, 10), fdgpu.BATCHED)
+    host = [ctx.read(i, (nz, ny, nx)) for i in ids]
+    p = dict(ob.LOC_DEFAULTS, nx=nx, ny=ny, nz=nz, x0=2.0, y0=1.0, z0=0.0, h=1.0, dq=1.0 / 6371.0,
+             df=1.0 / (6371.0 * math.sin(1.0)), ndiv=2, ndiv2=2, kmin=1)
+    rng = np.random.default_rng(91)
+    srcs = rng.integers([0, 0, 1], [nx, ny, nz], size=(70000, 3))
+    noise = rng.normal(0, 0.02, size=(70000, 10)).astype(np.float32)
+    obs = np.stack([host[q][srcs[:, 2], srcs[:, 1], srcs[:, 0]] for q in range(10)], 1) + 1.0 + noise
+    evs = [dict(table=ids, obstime=obs[e], pwt=np.ones(10, np.float32), phase="P" * 10) for e in range(70000)]
+    got = ctx.locate(fdgpu.loc_params(**p), evs)
+    assert all(r["status"] in (0, 1, 2) and r["passes"] >= 1 for r in got)
+    grid = ob.loc_grid(**p)
+    for e in range(0, 70000, 997):
+        o = ob.locate(grid, host, obs[e], np.ones(10, np.float32), "P" * 10, np.ones(10, np.int32))
+        r = got[e]
+        assert (r["node"], r["exm"], r["eym"], r["ezm"], r["passes"]) == (o["node"], o["exm"], o["eym"], o["ezm"],
+                                                                          o["passes"]), e
+        assert _bits([r["stdmin"]], [o["stdmin"]]) == 0 and _bits(r["res"], o["res"]) == 0, e
+    for i in ids:
+        ctx.release(i)
+    ctx.release_model(m)
+
+
+def _table_specs(m, g, count):
+    from fdtomoc_amd import fdgpu, synth
+    out = []
+    for s in range(count):
+        st = synth.random_surface_stations(g, np.random.default_rng(2000 + s), 1)[0]
+        par = ob.Par(nx=g.nx, ny=g.ny, nz=g.nz, h=g.h, x0=g.x0, y0=g.y0, z0=g.z0, fxs=st.lon, fys=st.lat,
+                     fzs=st.depth, reverse=50)
+        out.append(fdgpu.TableSpec(par_dict(par), m))
+    return out

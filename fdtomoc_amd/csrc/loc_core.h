@@ -130,6 +130,13 @@ __device__ __forceinline__ float gload(const float *t, long long n) {
     return *((const __attribute__((address_space(1))) float *)t + n);
 }
 
+// The values of nodes n and n + 1 (n even: 8-byte aligned in a table) in one load.
+typedef float float_pair __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float_pair gload2(const float *t, long long n) {
+    return *(const __attribute__((address_space(1))) float_pair *)((const __attribute__((address_space(1))) float *)t +
+                                                                   n);
+}
+
 // The byte offsets of a cell's eight corners in find_time's term order,
 // once per point for all of its picks' tables.
 __device__ __forceinline__ void corner_offsets(const Cell &c, long long sy, long long sz, unsigned o[8]) {

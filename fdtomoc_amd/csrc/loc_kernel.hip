@@ -95,6 +95,44 @@ __device__ __forceinline__ void node_sums(const LocGrid &g, const LocSPick *__re
     }
 }
 
+// Sums at the nodes n and n + 1 (n even, both in the table): one 8-byte load
+// per pick serves both; s = {avres(n), rsq(n), avres(n + 1), rsq(n + 1)}.
+template <bool SCALED>
+__device__ __forceinline__ void pair_sums(const LocGrid &g, const LocSPick *__restrict__ sp,
+                                          const unsigned char *__restrict__ sc, int first, int count, long long n,
+                                          float s[4]) {
+    constexpr int U = 8;
+    const int end = first + count;
+    int p = first;
+    for (; p + U <= end; p += U) {
+        locdev::float_pair t[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) t[u] = locdev::gload2(sp[p + u].table, n);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const float obs = sp[p + u].obs, w = sp[p + u].pwt;
+            const bool scl = SCALED && sc[p + u];
+            const float ra = scl ? (float)((double)obs - (double)t[u].x * (double)g.vpvs)
+                                 : locdev::residual_unscaled(obs, t[u].x);
+            const float rb = scl ? (float)((double)obs - (double)t[u].y * (double)g.vpvs)
+                                 : locdev::residual_unscaled(obs, t[u].y);
+            locdev::add2(s[0], s[1], ra, w);
+            locdev::add2(s[2], s[3], rb, w);
+        }
+    }
+    for (; p < end; p++) {
+        const locdev::float_pair t = locdev::gload2(sp[p].table, n);
+        const float obs = sp[p].obs, w = sp[p].pwt;
+        const bool scl = SCALED && sc[p];
+        const float ra = scl ? (float)((double)obs - (double)t.x * (double)g.vpvs) : locdev::residual_unscaled(obs, t.x);
+        const float rb = scl ? (float)((double)obs - (double)t.y * (double)g.vpvs) : locdev::residual_unscaled(obs, t.y);
+        locdev::add2(s[0], s[1], ra, w);
+        locdev::add2(s[2], s[3], rb, w);
+    }
+}
+
+constexpr int kPairTile = 2 * kThreads;   // nodes of a node-search block: two per lane
+
 constexpr int kXcds = 8;   // workgroups are dealt to the accelerator dies in turn
 
 // Block b runs on die b % kXcds.  The blocks a die holds at once should share
@@ -111,16 +149,32 @@ __global__ __launch_bounds__(kThreads) void loc_nodes_kernel(const LocGrid g, co
     const long long tile = (long long)(local / ngroups) * kXcds + x;
     const int grp = (int)(local % ngroups);
     const long long nxy = (long long)g.nx * g.ny;
-    const long long n0 = nxy * g.kmin;
-    if (n0 + tile * kThreads >= nxy * g.nz) return;   // the whole block: beyond the last tile
-    const long long n = n0 + tile * kThreads + threadIdx.x;
-    const bool in = n < nxy * g.nz;
+    const long long n0 = nxy * g.kmin, total = nxy * g.nz;
+    // node pairs from an even index: a lane's pair is one aligned 8-byte load
+    const long long nbase = n0 & ~1LL;
+    if (nbase + tile * kPairTile >= total) return;   // the whole block: beyond the last tile
+    const long long n = nbase + tile * kPairTile + 2 * (long long)threadIdx.x;
+    const bool in_a = n >= n0 && n < total, in_b = n + 1 < total;   // n + 1 >= n0 always
     const int a0 = (int)((long long)grp * nactive / ngroups), a1 = (int)((long long)(grp + 1) * nactive / ngroups);
     for (int a = a0; a < a1; a++) {
         const int ev = active[a];
         const LocSEvent E = sev[ev];
         unsigned long long key = locdev::kNone;
-        if (in) {
+        if (in_b) {
+            float sums[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (E.scaled)
+                pair_sums<true>(g, sp, sc, E.first, E.count, n, sums);
+            else
+                pair_sums<false>(g, sp, sc, E.first, E.count, n, sums);
+            const unsigned long long kb = locdev::key_of(locdev::deviation(sums[2], sums[3], E.facs, E.den),
+                                                         __builtin_inff(), (unsigned)(n + 1 - n0));
+            key = kb;
+            if (in_a) {
+                const unsigned long long ka = locdev::key_of(locdev::deviation(sums[0], sums[1], E.facs, E.den),
+                                                             __builtin_inff(), (unsigned)(n - n0));
+                key = ka < kb ? ka : kb;
+            }
+        } else if (in_a) {   // the grid's last node, without a partner
             float avres = 0.0f, rsq = 0.0f;
             if (E.scaled)
                 node_sums<true>(g, sp, sc, E.first, E.count, n, avres, rsq);
@@ -231,7 +285,8 @@ extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_p
         // groups of at least 16 events, at most 16: a die then holds about
         // 8 x 32 CUs / ngroups tiles at once, their table lines within its L2
         const int ngroups = nactive >= 256 ? 16 : nactive >= 16 ? nactive / 16 : 1;
-        const long long tiles = (nodes + kThreads - 1) / kThreads;
+        const long long nbase = (long long)g.nx * g.ny * g.kmin & ~1LL;
+        const long long tiles = ((long long)g.nx * g.ny * g.nz - nbase + kPairTile - 1) / kPairTile;
         const long long per_die = (tiles + kXcds - 1) / kXcds;
         hipLaunchKernelGGL(loc_nodes_kernel, dim3((unsigned)(per_die * ngroups * kXcds)), dim3(kThreads), 0, stream, g,
                            d_spicks, d_sevents, d_sscale, d_active, nactive, ngroups, d_state);

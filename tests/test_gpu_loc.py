@@ -264,10 +264,11 @@ def test_loaded_table_files(ctx, tables, tmp_path):
 
 def test_more_events_than_grid_rows(ctx):
     """70,000 events in one call (more than one launch's 65,535 grid rows in
-    the refinements) on a small grid: a sample of them equal to the oracle,
-    and every event located."""
+    the refinements) on a small grid with an odd node count and an odd first
+    searched node (the node search pairs nodes from an even index): a sample
+    of them equal to the oracle, and every event located."""
     from fdtomoc_amd import fdgpu, synth
-    nx, ny, nz = 9, 8, 7
+    nx, ny, nz = 9, 7, 7
     g = synth.centred_grid(nx, ny, nz, 1.0)
     m = ctx.model(synth.gradient_velocity(g, 5.0, 7.0, 7.0).astype(np.float32))
     ids = ctx.solve(_table_specs(m, g, 10), fdgpu.BATCHED)

@@ -76,7 +76,7 @@ REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 # HBM bytes per node-search launch of the location workload (average over
 # its 5 passes, FETCH_SIZE x 1.99992 + WRITE_SIZE, copy-calibrated)
 # (profiles/loc_nodes_kernel_256events_96tables.md)
-TRAFFIC_BYTES_LOC = 1.0731e9
+TRAFFIC_BYTES_LOC = 4.7591e9
 
 
 # ------------------------------------------------------------------ helpers

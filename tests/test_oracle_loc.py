@@ -145,3 +145,46 @@ def test_status_thresholds_and_s_scaling():
     assert r["node"] == src and r["stdmin"] < 1e-5
     r = ob.locate(_grid(21, 17, 12, ivs=1), tabs, sobs, np.ones(n), ph, np.ones(n))
     assert r["stdmin"] > 0.1
+
+
+def test_deviation_formula_closed_form():
+    """Constant tables give every trial point the same residuals r = obs - c,
+    so the first node wins and the refinements (ndiv = 1: points on nodes,
+    binary spacings) cannot beat it; the deviation and the mean residual then
+    equal sphfdloc's weighted variance (sphfdloc.c:786-799) evaluated
+    independently in float64, with every pick counted in the divisor on the
+    first pass: sqrt(|sum w r^2 - (sum w r)^2 / sum w| / ((n - 4) sum w / n))
+    and sum w r / sum w."""
+    rng = np.random.default_rng(17)
+    nx, ny, nz, n = 4, 3, 5, 11
+    c = rng.uniform(2.0, 20.0, n)
+    tabs = [np.full((nz, ny, nx), np.float32(v)) for v in c]
+    r = rng.normal(0.0, 0.2, n)
+    obs = (np.float32(c) + np.float32(r)).astype(np.float32)
+    w = rng.choice([1.0, 4.0, 25.0, 100.0], n).astype(np.float32)
+    g = _grid(nx, ny, nz, ndiv=1, ndiv2=1, kmin=1, resthres=100.0)
+    out = ob.locate(g, tabs, obs, w, "P" * n, np.ones(n))
+    rr = obs.astype(np.float64) - np.float32(c).astype(np.float64)
+    wd = w.astype(np.float64)
+    sw, swr, swr2 = wd.sum(), (wd * rr).sum(), (wd * rr * rr).sum()
+    dev = np.sqrt(abs(swr2 - swr * swr / sw) / ((n - 4) * sw / n))
+    assert out["node"] == (0, 0, 1) and out["passes"] == 1
+    assert out["stdmin"] == pytest.approx(dev, rel=2e-5)
+    assert out["avrmin"] == pytest.approx(swr / sw, rel=2e-5, abs=1e-7)
+    assert np.allclose(out["res"], rr, atol=1e-6)
+    # one pick flagged unusable and 50 s off (so the retry does not restore
+    # it): absent from the sums but counted in the first pass's divisor
+    # (sphfdloc.c:683), which is the only pass
+    obs2 = obs.copy()
+    obs2[3] += np.float32(50.0)
+    flags = np.ones(n, dtype=np.int32)
+    flags[3] = 0
+    g2 = _grid(nx, ny, nz, ndiv=1, ndiv2=1, kmin=1, resthres=1.0, resthrep=0.01)
+    out = ob.locate(g2, tabs, obs2, w, "P" * n, flags)
+    keep = np.arange(n) != 3
+    sw, swr, swr2 = wd[keep].sum(), (wd * rr)[keep].sum(), (wd * rr * rr)[keep].sum()
+    dev_n = np.sqrt(abs(swr2 - swr * swr / sw) / ((n - 4) * sw / n))
+    dev_n1 = np.sqrt(abs(swr2 - swr * swr / sw) / ((n - 5) * sw / (n - 1)))
+    assert out["passes"] == 1 and out["isgood"][3] == 0 and out["ngood"] == n - 1
+    assert out["stdmin"] == pytest.approx(dev_n, rel=2e-5)
+    assert abs(dev_n - dev_n1) > 1e-3 * dev_n

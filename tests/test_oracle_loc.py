@@ -188,3 +188,26 @@ def test_deviation_formula_closed_form():
     assert out["passes"] == 1 and out["isgood"][3] == 0 and out["ngood"] == n - 1
     assert out["stdmin"] == pytest.approx(dev_n, rel=2e-5)
     assert abs(dev_n - dev_n1) > 1e-3 * dev_n
+
+
+def test_refinements_land_exactly_on_a_lattice_source():
+    """Tables linear in the node indices (small integer gradients, exact in
+    float and reproduced exactly by find_time), binary spacings and
+    divisions (ndiv = ndiv2 = 4: steps 1/4 and 1/16), and a source on the
+    second refinement's lattice: the residuals vanish exactly at the source
+    and nowhere else, so the node search, the first box (the best node +- 1
+    in quarter steps) and the second box (+- one quarter in sixteenth
+    steps) must land on it exactly, with deviation 0."""
+    rng = np.random.default_rng(23)
+    nx, ny, nz, n = 12, 10, 9, 10
+    grads = rng.integers(-3, 4, size=(n, 3)).astype(np.float64)
+    grads[:3] = np.eye(3) * 2 + 1          # well spread directions
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    tabs = [(gv[0] * i + gv[1] * j + gv[2] * k + 40.0).astype(np.float32) for gv in grads]
+    for src in ((5.3125, 4.5625, 4.1875), (2.75, 6.0625, 3.9375), (8.0625, 2.25, 5.5)):
+        obs = np.array([gv @ np.array(src) + 40.0 for gv in grads], dtype=np.float32)
+        assert np.array_equal(obs.astype(np.float64), np.array([gv @ np.array(src) + 40.0 for gv in grads]))
+        g = _grid(nx, ny, nz, ndiv=4, ndiv2=4, resthres=100.0)
+        out = ob.locate(g, tabs, obs, np.ones(n), "P" * n, np.ones(n))
+        assert (out["exm"], out["eym"], out["ezm"]) == src, (src, out["exm"], out["eym"], out["ezm"])
+        assert out["stdmin"] == 0.0

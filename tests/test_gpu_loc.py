@@ -303,3 +303,31 @@ def _table_specs(m, g, count):
                      fzs=st.depth, reverse=50)
         out.append(fdgpu.TableSpec(par_dict(par), m))
     return out
+
+
+def test_device_lands_exactly_on_lattice_sources(ctx, tmp_path):
+    """Tables linear in the node indices (written as files and loaded),
+    binary spacings and divisions: the device's three stages land exactly on
+    sources of the second refinement's lattice with deviation 0, as the
+    oracle does (its pin of the refinement boxes)."""
+    from fdtomoc_amd import fdgpu
+    rng = np.random.default_rng(23)
+    nx, ny, nz, n = 12, 10, 9, 10
+    grads = rng.integers(-3, 4, size=(n, 3)).astype(np.float64)
+    grads[:3] = np.eye(3) * 2 + 1
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ids = []
+    for q, gv in enumerate(grads):
+        path = str(tmp_path / f"lin{q}.times")
+        (gv[0] * i + gv[1] * j + gv[2] * k + 40.0).astype("<f4").tofile(path)
+        ids.append(ctx.load_table(path, (nz, ny, nx)))
+    p = dict(ob.LOC_DEFAULTS, nx=nx, ny=ny, nz=nz, x0=0.0, y0=0.0, z0=0.0, h=1.0, dq=1.0, df=1.0, ndiv=4, ndiv2=4,
+             resthres=100.0)
+    srcs = ((5.3125, 4.5625, 4.1875), (2.75, 6.0625, 3.9375), (8.0625, 2.25, 5.5))
+    evs = [dict(table=ids, obstime=np.array([gv @ np.array(s) + 40.0 for gv in grads], dtype=np.float32),
+                pwt=np.ones(n, np.float32), phase="P" * n) for s in srcs]
+    got = ctx.locate(fdgpu.loc_params(**p), evs)
+    for s, r in zip(srcs, got):
+        assert (r["exm"], r["eym"], r["ezm"]) == s and r["stdmin"] == 0.0
+    for t in ids:
+        ctx.release(t)

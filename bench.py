@@ -601,7 +601,7 @@ def run_c2f(args):
         "kernel_nodes_per_s": round(n / kern_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(alg / kern_s / 1e9, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / kern_s / 1e9 / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "c2f_prep_kernel + c2f_fine_kernel", "kernel_ms": round(kern_s * 1e3, 4),
+                     "kernel": "c2f_prep_kernel + c2f_fine_kernel + c2f_range_kernel", "kernel_ms": round(kern_s * 1e3, 4),
                      "algorithmic_bytes": alg},
     }
     if not args.no_cpu and os.path.exists(C2F_EXE):

@@ -70,13 +70,15 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
     const long long ncoarse = 2LL * co->nxc * co->nyc * co->nzc;
     const long long nax = (long long)nx + ny + nz;
     const long long n = (long long)nx * ny * nz;
-    // one scratch block: coarse values | coordinates | axis entries | range | fine velocities
+    // one scratch block: coarse values | coordinates | axis entries | range |
+    // workgroup partial ranges | fine velocities
     const size_t b_co = align256(sizeof(float) * (size_t)ncoarse);
     const size_t b_g = align256(sizeof(float) * g.size());
     const size_t b_ax = align256(sizeof(C2fAxis) * (size_t)nax);
     const size_t b_rg = 256;
+    const size_t b_part = align256(sizeof(unsigned) * 8 * (size_t)kC2fMaxBlocks);
     const size_t b_fine = fine ? align256(sizeof(float) * 2 * (size_t)n) : 0;
-    const size_t bytes = b_co + b_g + b_ax + b_rg + b_fine;
+    const size_t bytes = b_co + b_g + b_ax + b_rg + b_part + b_fine;
     if (bytes > c->coarse_bytes) {
         FD_HIP(hipStreamSynchronize(c->stream));
         dev_free(c, c->d_coarse, c->coarse_bytes);
@@ -91,7 +93,8 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
     float *d_g = (float *)(blk + b_co);
     C2fAxis *d_ax = (C2fAxis *)(blk + b_co + b_g);
     unsigned *d_rg = (unsigned *)(blk + b_co + b_g + b_ax);
-    float *d_fine = fine ? (float *)(blk + b_co + b_g + b_ax + b_rg) : nullptr;
+    unsigned *d_part = (unsigned *)(blk + b_co + b_g + b_ax + b_rg);
+    float *d_fine = fine ? (float *)(blk + b_co + b_g + b_ax + b_rg + b_part) : nullptr;
     const unsigned rg0[8] = {0xffffffffu, 0u, 0u, 0u, 0xffffffffu, 0u, 0u, 0u};
     unsigned rg[8];
     bool ok = hipMemcpyAsync(d_co, co->velocity, sizeof(float) * (size_t)ncoarse, hipMemcpyHostToDevice,
@@ -104,7 +107,7 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
     grid.nx = nx; grid.ny = ny; grid.nz = nz;
     grid.x0 = co->x0; grid.y0 = co->y0; grid.z0 = co->z0; grid.h = co->h;
     ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
-    ok = ok && fd_launch_c2f(&grid, d_co, d_ax, d_fine, mp.d_slow, ms.d_slow, d_rg, c->stream) == hipSuccess;
+    ok = ok && fd_launch_c2f(&grid, d_co, d_ax, d_fine, mp.d_slow, ms.d_slow, d_part, d_rg, c->stream) == hipSuccess;
     ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
     ok = ok && hipMemcpyAsync(rg, d_rg, sizeof rg, hipMemcpyDeviceToHost, c->stream) == hipSuccess;
     if (fine)

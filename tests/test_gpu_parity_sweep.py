@@ -59,3 +59,35 @@ def test_ninety_six_option_mixes_in_one_launch():
             ctx.release_model(m)
     finally:
         ctx.close()
+
+
+def test_larger_option_mixes():
+    """Eight cases of 40..72 nodes per axis with the same random option mixes
+    (head waves and several reverse passes are more common at these sizes):
+    bit for bit and pass for pass equal to the oracle."""
+    from fdtomoc_amd import fdgpu
+    ctx = fdgpu.Context(0)
+    try:
+        cases = []
+        for s in range(8):
+            par, v = make_case(3000 + s, nmin=40, nmax=72)
+            rng = np.random.default_rng(3000 + s + 7777)
+            for key in ("iminus", "iplus", "jminus", "jplus", "kminus", "kplus"):
+                if rng.random() < 0.2:
+                    setattr(par, key, 2)
+            cases.append((par, v))
+        models = [ctx.model(v) for _, v in cases]
+        ids = ctx.solve([fdgpu.TableSpec(par_dict(p), m) for (p, _), m in zip(cases, models)], fdgpu.BATCHED)
+        for k, ((par, v), tid) in enumerate(zip(cases, ids)):
+            t_ref, st = ob.solve(par, ob.slowness(v))
+            t = ctx.read(tid, v.shape)
+            gst = ctx.stats(tid)
+            print(f"case {3000 + k} {v.shape} passes {st['passes']}", flush=True)
+            assert gst["passes"] == st["passes"] and gst["error"] & ~1 == 0
+            assert int((t.view(np.uint32) != t_ref.view(np.uint32)).sum()) == 0, 3000 + k
+        for tid in ids:
+            ctx.release(tid)
+        for m in models:
+            ctx.release_model(m)
+    finally:
+        ctx.close()

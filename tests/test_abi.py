@@ -121,3 +121,41 @@ def test_loc_geography_without_gpu():
             assert abs(la - lat) < 1e-9 and abs(d - dep) < 1e-6 and abs(lo - 121.25) < 1e-12, (lat, dep)
     _, r = fdgpu.loc_params_from_grid(5, 5, 5, 0.0, 0.0, 12.0, 1.0)
     assert abs(r - (6378.137 - 12.0)) < 1e-9
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror in fdtomoc_amd/fdgpu.py has the size and the field
+    offsets of its struct in include/fdtomoc_amd.h, as a C compiler lays it
+    out (a drifted mirror would corrupt calls silently)."""
+    import shutil
+    import subprocess
+    from fdtomoc_amd import fdgpu
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    pairs = [("fdgpu_par", fdgpu.CPar), ("fdgpu_serve_config", fdgpu.CServeConfig),
+             ("fdgpu_table_desc", fdgpu.CTable), ("fdgpu_coarse", fdgpu.CCoarse),
+             ("fdgpu_loc_params", fdgpu.CLocParams), ("fdgpu_loc_event", fdgpu.CLocEvent)]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fdtomoc_amd.h"', "int main(void) {"]
+    for cname, cls in pairs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, "-o", str(exe), str(src)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, cls in pairs:
+        assert got[(cname, "size")] == _ctypes_sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def _ctypes_sizeof(cls):
+    import ctypes
+    return ctypes.sizeof(cls)

@@ -282,9 +282,9 @@ extern "C" hipError_t fd_launch_loc_pass(const LocGrid *grid, const LocPick *d_p
     hipLaunchKernelGGL(loc_init_kernel, dim3(small), dim3(64), 0, stream, d_active, nactive, d_state);
     if (node0) (void)hipEventRecord(node0, stream);
     if (nodes > 0) {
-        // groups of at least 16 events, at most 16: a die then holds about
+        // at most 16 groups of at least 8 events: a die then holds about
         // 8 x 32 CUs / ngroups tiles at once, their table lines within its L2
-        const int ngroups = nactive >= 256 ? 16 : nactive >= 16 ? nactive / 16 : 1;
+        const int ngroups = nactive >= 128 ? 16 : nactive >= 8 ? nactive / 8 : 1;
         const long long nbase = (long long)g.nx * g.ny * g.kmin & ~1LL;
         const long long tiles = ((long long)g.nx * g.ny * g.nz - nbase + kPairTile - 1) / kPairTile;
         const long long per_die = (tiles + kXcds - 1) / kXcds;

@@ -85,6 +85,45 @@ def loc_params(**kw) -> CLocParams:
     return p
 
 
+class LocBatch:
+    """Events as fdgpu_loc_event records with their arrays, built once, for
+    repeated fdgpu_locate calls (Context.locate_batch); reset() restores the
+    use flags the outlier passes change."""
+
+    def __init__(self, events: list):
+        self.n = len(events)
+        self.arr = (CLocEvent * self.n)()
+        self._keep, self._outs, self._good0 = [], [], []
+        for e, ev in zip(self.arr, events):
+            n = len(ev["table"])
+            tab = np.ascontiguousarray(ev["table"], dtype=np.int32)
+            obs = np.ascontiguousarray(ev["obstime"], dtype=np.float32)
+            pwt = np.ascontiguousarray(ev["pwt"], dtype=np.float32)
+            ph = np.frombuffer(ev["phase"].encode(), dtype=np.uint8).copy()
+            good = np.ascontiguousarray(ev.get("isgood", np.ones(n)), dtype=np.int32).copy()
+            res, tc = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+            if not (len(obs) == len(pwt) == len(ph) == len(good) == n):
+                raise ValueError("pick arrays of different lengths")
+            self._keep += [tab, obs, pwt, ph]
+            e.npick = n
+            e.table, e.obstime, e.pwt = tab.ctypes.data, obs.ctypes.data, pwt.ctypes.data
+            e.phase, e.isgood, e.res, e.tcalc = ph.ctypes.data, good.ctypes.data, res.ctypes.data, tc.ctypes.data
+            self._outs.append((good, res, tc))
+            self._good0.append(good.copy())
+
+    def reset(self):
+        for (good, _, _), g0 in zip(self._outs, self._good0):
+            good[:] = g0
+
+    def results(self) -> list:
+        out = []
+        for e, (good, res, tc) in zip(self.arr, self._outs):
+            out.append(dict(status=e.status, ngood=e.ngood, passes=e.passes, node=tuple(e.node), stdmin=e.stdmin,
+                            avrmin=e.avrmin, exm=e.exm, eym=e.eym, ezm=e.ezm, isgood=good.copy(), res=res.copy(),
+                            tps=tc.copy()))
+        return out
+
+
 def loc_params_from_grid(nx: int, ny: int, nz: int, x0: float, y0: float, z0: float, h: float,
                          dq: float = 0.0, df: float = 0.0):
     """(fdgpu_loc_params, z0r): sphfdloc's search grid from the spec's grid
@@ -297,29 +336,14 @@ class Context:
         'P'/'S') and optionally `isgood`; returns one dict per event with the
         status, counts, node, deviation, mean residual, hypocentre, and the
         final use flags, residuals and computed times per pick."""
-        keep, arr = [], (CLocEvent * len(events))()
-        outs = []
-        for e, ev in zip(arr, events):
-            n = len(ev["table"])
-            tab = np.ascontiguousarray(ev["table"], dtype=np.int32)
-            obs = np.ascontiguousarray(ev["obstime"], dtype=np.float32)
-            pwt = np.ascontiguousarray(ev["pwt"], dtype=np.float32)
-            ph = np.frombuffer(ev["phase"].encode(), dtype=np.uint8).copy()
-            good = np.ascontiguousarray(ev.get("isgood", np.ones(n)), dtype=np.int32).copy()
-            res, tc = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
-            if not (len(obs) == len(pwt) == len(ph) == len(good) == n):
-                raise ValueError("pick arrays of different lengths")
-            keep += [tab, obs, pwt, ph, good, res, tc]
-            e.npick = n
-            e.table, e.obstime, e.pwt = tab.ctypes.data, obs.ctypes.data, pwt.ctypes.data
-            e.phase, e.isgood, e.res, e.tcalc = ph.ctypes.data, good.ctypes.data, res.ctypes.data, tc.ctypes.data
-            outs.append((good, res, tc))
-        _check(lib().fdgpu_locate(self._h, C.byref(params), arr, len(events)), "fdgpu_locate")
-        out = []
-        for e, (good, res, tc) in zip(arr, outs):
-            out.append(dict(status=e.status, ngood=e.ngood, passes=e.passes, node=tuple(e.node), stdmin=e.stdmin,
-                            avrmin=e.avrmin, exm=e.exm, eym=e.eym, ezm=e.ezm, isgood=good, res=res, tps=tc))
-        return out
+        batch = LocBatch(events)
+        self.locate_batch(params, batch)
+        return batch.results()
+
+    def locate_batch(self, params: CLocParams, batch: "LocBatch"):
+        """fdgpu_locate on a prepared batch (its events' C records built once):
+        the results are in the batch afterwards (LocBatch.results)."""
+        _check(lib().fdgpu_locate(self._h, C.byref(params), batch.arr, batch.n), "fdgpu_locate")
 
     def write_async(self, tables: list, paths: list, vel_headers: list | None = None):
         """Background writes of resident tables (fdgpu_tables_write_async);

@@ -85,6 +85,16 @@ def loc_params(**kw) -> CLocParams:
     return p
 
 
+class PreparedCoarse:
+    """A coarse model's fdgpu_coarse record and arrays, built once; change
+    `velocity` in place between updates (same shape)."""
+
+    def __init__(self, coarse: "Coarse"):
+        self._keep = []
+        self.c = coarse.to_c(self._keep)
+        self.velocity = self._keep[-1]   # the float32 payload the record points at
+
+
 class LocBatch:
     """Events as fdgpu_loc_event records with their arrays, built once, for
     repeated fdgpu_locate calls (Context.locate_batch); reset() restores the
@@ -388,6 +398,12 @@ class Context:
                                               None if fine is None else fine.ctypes.data),
                "fdgpu_models_from_coarse")
         return (mp.value, ms.value, fine) if want_fine else (mp.value, ms.value)
+
+    def update_models_from_prepared(self, model_p: int, model_s: int, prepared: "PreparedCoarse"):
+        """fdgpu_models_update_from_coarse on a coarse model whose C record was
+        built once (repeated updates, as an inversion's iterations make)."""
+        _check(lib().fdgpu_models_update_from_coarse(self._h, model_p, model_s, C.byref(prepared.c), None),
+               "fdgpu_models_update_from_coarse")
 
     def update_models_from_coarse(self, model_p: int, model_s: int, coarse: Coarse, want_fine: bool = False):
         keep = []

@@ -73,10 +73,13 @@ TRAFFIC_BYTES_C2 = 323.92e9
 FP64_OPS_C2 = 8.557e12
 FP64_PEAK_TFLOPS = 78.6
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
+# HBM bytes of one c2f model update (its three kernels, FETCH_SIZE x 1.99992 +
+# WRITE_SIZE) (profiles/c2f_fine_kernel_301x301x61.md)
+TRAFFIC_BYTES_C2F = 51.39e6
 # HBM bytes per node-search launch of the location workload (average over
 # its 5 passes, FETCH_SIZE x 1.99992 + WRITE_SIZE, copy-calibrated)
 # (profiles/loc_nodes_kernel_256events_96tables.md)
-TRAFFIC_BYTES_LOC = 4.7591e9
+TRAFFIC_BYTES_LOC = 2.6258e9
 
 
 # ------------------------------------------------------------------ helpers
@@ -603,7 +606,8 @@ def run_c2f(args):
                    "fine_nodes": n, "coarse_nodes": co.nxc * co.nyc * co.nzc},
         "kernel_nodes_per_s": round(n / kern_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(alg / kern_s / 1e9, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(alg / kern_s / 1e9 / HBM_PEAK_GBS, 6), "traffic": None,
+                     "unit": "GB/s", "frac": round(alg / kern_s / 1e9 / HBM_PEAK_GBS, 6), "traffic": TRAFFIC_BYTES_C2F,
+                     "traffic_source": "profiles/c2f_fine_kernel_301x301x61.md",
                      "kernel": "c2f_prep_kernel + c2f_fine_kernel + c2f_range_kernel", "kernel_ms": round(kern_s * 1e3, 4),
                      "algorithmic_bytes": alg},
     }

@@ -106,18 +106,24 @@ struct Scratch {
 };
 
 // LDS scratch of a workgroup: radix histograms during a sort, then the LEVEL
-// schedule's byte-packed pending counters and successor masks (faces up to
-// about 83K nodes with the wide build's arena; larger faces use global
-// scratch).  Beside it, the geometry row and layer tables of the table being
-// solved: geo_rows_doubles(ny) + geo_layers_doubles(nz) doubles, at most
-// kGeoLds (the host sends larger grids to a build that holds them, or refuses
-// them).
+// schedule's byte-packed pending counters and successor masks (2 bytes per
+// face node: faces up to 52,000 nodes with the wide build's 26,000 words,
+// 26,800 with the narrow build's 13,400; larger faces keep them in global
+// scratch), or the CHANGE schedule's
+// queued-follower bits (one bit per node).  LW bounds the words the schedules
+// may use for per-node state (LW <= AW; host test builds set it to 0 to run
+// every face through the global-scratch paths).  Beside it, the geometry row
+// and layer tables of the table being solved: geo_rows_doubles(ny) +
+// geo_layers_doubles(nz) doubles, at most kGeoLds (the host sends larger
+// grids to a build that holds them, or refuses them).
 constexpr int kArenaWords = 26000;
 constexpr int kGeoLdsDoubles = 6800;
 
-template <int AW, int GL = kGeoLdsDoubles>
+template <int AW, int GL = kGeoLdsDoubles, int LW = AW>
 struct SharedT {
     static constexpr int kArena = AW;
+    static constexpr int kNodeWords = LW;
+    static_assert(LW <= AW, "per-node state must fit the arena");
     static constexpr int kGeoLds = GL;
     double geol[GL];     // geometry row and layer tables
     int n[3];            // list sizes
@@ -327,7 +333,7 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
     // pending counters, one byte per node packed four to a word, and successor
     // masks (bit d: the neighbour in direction d follows the node)
     const int nwords = (Fn + 3) / 4;
-    const bool in_lds = (long long)nwords * 4 + Fn <= (long long)SH::kArena * 4;
+    const bool in_lds = (long long)nwords * 4 + Fn <= (long long)SH::kNodeWords * 4;
     unsigned *pend = in_lds ? (unsigned *)sh.arena : (unsigned *)W.pend;
     unsigned char *succ = in_lds ? (unsigned char *)(sh.arena + nwords) : W.flag;
     int *lists[3] = {W.list0, W.list1, W.list2};
@@ -460,7 +466,7 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
     // followers queued in the current round: a bit per node in LDS when the
     // face fits, else a round stamp per node in scratch
     const int nbits = (Fn + 31) / 32;
-    const bool qlds = nbits <= SH::kArena;
+    const bool qlds = nbits <= SH::kNodeWords;
     unsigned *qbits = (unsigned *)sh.arena;
     if (!qlds)
         for (int p = tid; p < Fn; p += T) W.stamp[p] = 0;

@@ -23,19 +23,24 @@ def lib():
     if _lib is None:
         L = C.CDLL(EMU_SO)
         L.emu_solve.argtypes = [C.POINTER(ob._Par), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
-                                C.c_int]
+                                C.c_int, C.c_int]
         L.emu_solve.restype = C.c_int
         _lib = L
     return _lib
 
 
-def solve(par: "ob.Par", slow: np.ndarray, nthreads: int = 4, extra=None, change_budget: int = 8):
+def solve(par: "ob.Par", slow: np.ndarray, nthreads: int = 4, extra=None, change_budget: int = 8,
+          global_state: bool = False):
+    """global_state: keep every face's schedule state (LEVEL pending counters
+    and successor masks, CHANGE queue marks) in global scratch, the device's
+    path for faces too large for LDS."""
     slow = np.ascontiguousarray(slow, dtype=np.float32)
     time = np.empty_like(slow)
     stats = np.zeros(len(STAT_NAMES), dtype=np.int32)
     ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float32)
     rc = lib().emu_solve(C.byref(par.to_c()), slow.ctypes.data, time.ctypes.data,
-                         None if ex is None else ex.ctypes.data, nthreads, stats.ctypes.data, change_budget)
+                         None if ex is None else ex.ctypes.data, nthreads, stats.ctypes.data, change_budget,
+                         1 if global_state else 0)
     if rc != 0:
         raise RuntimeError(f"emu_solve failed: {rc}")
     return time, dict(zip(STAT_NAMES, stats.tolist()))

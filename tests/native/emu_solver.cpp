@@ -48,8 +48,31 @@ struct EmuPar {
     int invh, invq, invf, invx0, invy0, invz0;
 };
 
+// Workgroup state whose schedules keep no per-node state in the arena: every
+// LEVEL face takes its pending counters and successor masks from global
+// scratch (Scratch::pend, Scratch::flag) and every CHANGE face its queue marks
+// (Scratch::stamp), the paths the device takes for faces too large for LDS.
+using SharedGlobal = fdsweep::SharedT<fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles, 0>;
+
+template <class SH>
+static void run_threads(int nthreads, const fdsweep::TableJob &J, const fdsweep::Scratch &W) {
+    std::unique_ptr<SH> shp(new SH());
+    SH &sh = *shp;
+    std::memset(&sh, 0, sizeof sh);
+    std::barrier<> bar(nthreads);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([&, t]() {
+            HostExec x{t, nthreads, &bar};
+            fdsweep::solve_table(x, sh, J, W);
+        });
+    for (auto &t : th) t.join();
+}
+
+// global_state: 0 per-node schedule state in the arena where a face fits
+// (as the device builds), 1 always in global scratch.
 extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const float *extra, int nthreads,
-                         int *stats, int change_budget) {
+                         int *stats, int change_budget, int global_state) {
     fdprep::ParValues p;
     p.nx = pc->nx; p.ny = pc->ny; p.nz = pc->nz;
     p.h = pc->h; p.dq = pc->dq; p.df = pc->df; p.x0 = pc->x0; p.y0 = pc->y0; p.z0 = pc->z0;
@@ -103,16 +126,7 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     J.change_budget = change_budget;
     std::vector<double> geo((size_t)fdsweep::geo_doubles(p.ny, p.nz));
     J.geo_fill = geo.data();
-    std::unique_ptr<fdsweep::Shared> shp(new fdsweep::Shared());
-    fdsweep::Shared &sh = *shp;
-    std::memset(&sh, 0, sizeof sh);
-    std::barrier<> bar(nthreads);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nthreads; t++)
-        th.emplace_back([&, t]() {
-            HostExec x{t, nthreads, &bar};
-            fdsweep::solve_table(x, sh, J, W);
-        });
-    for (auto &t : th) t.join();
+    if (global_state) run_threads<SharedGlobal>(nthreads, J, W);
+    else run_threads<fdsweep::Shared>(nthreads, J, W);
     return 0;
 }

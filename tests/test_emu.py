@@ -25,7 +25,7 @@ def test_emulated_kernel_matches_oracle(emu, seed):
     par, v = make_case(seed)
     s = ob.slowness(v)
     t_ref, st = ob.solve(par, s)
-    # 8, 16 and 24 threads also take the split evaluation of small LEVEL rounds
+    # thread counts from one lane to more lanes than most rounds have ready nodes
     nthreads = [1, 2, 3, 4, 8, 16, 24][seed % 7]
     # the per-face schedule knob must not change a bit: 0 (CHANGE never gives
     # up), 1 (most reverse-pass faces handed to LEVEL), 8 (default)
@@ -33,6 +33,24 @@ def test_emulated_kernel_matches_oracle(emu, seed):
     t, est = emu.solve(par, s, nthreads, change_budget=budget)
     assert est["error"] == 0
     assert est["passes"] == st["passes"] and est["restarts"] == st["restarts"]
+    assert int((t.view(np.uint32) != t_ref.view(np.uint32)).sum()) == 0
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_global_scratch_schedule_state_matches_oracle(emu, seed):
+    """Faces too large for LDS keep the LEVEL pending counters and successor
+    masks and the CHANGE queue marks in global scratch (on the device: faces
+    above 52,000 nodes on the wide build, 26,800 on the narrow one).  Here
+    every face takes that path, on the same cases, bit for bit."""
+    par, v = make_case(seed + 300)
+    s = ob.slowness(v)
+    t_ref, st = ob.solve(par, s)
+    nthreads = [1, 3, 8, 24][seed % 4]
+    budget = [0, 1, 8][seed % 3]
+    t, est = emu.solve(par, s, nthreads, change_budget=budget, global_state=True)
+    assert est["error"] == 0
+    assert est["passes"] == st["passes"] and est["restarts"] == st["restarts"]
+    assert est["faces_level"] > 0
     assert int((t.view(np.uint32) != t_ref.view(np.uint32)).sum()) == 0
 
 

@@ -1,25 +1,31 @@
 """Benchmark of the MI355X sphfd path.
 
-Default workload (the configuration the headline metric is quoted on, C2):
-48 stations uniform-random on the top surface of a 161 x 161 x 81 grid at
-1 km centred at 23.5N 121E, P and S tables (Vs = Vp/1.73) -> 96 station
-solves per step, FP32 1D gradient 5.0 -> 8.0 km/s over 0-64 km (continued
-below) times a +-5 % checkerboard of 20 km cells, reverse=50 as make_par_file
-writes.  A step is one batched solve of all 96 tables (one persistent launch);
-the velocity models are resident in HBM before timing starts.
+Default workload (the configuration the north-star target is stated on, C3):
+256 station solves whose boxes are clipped to per-station radii uniform in
+40-200 km -- windows from 41 x 41 x 41 to 201 x 201 x 101 nodes of one
+resident 401 x 401 x 101 model at 1 km centred at 23.5N 121E (FP32 gradient
+5.0 -> 8.0 km/s over 0-64 km times three seeded +-8 % Gaussian anomalies of
+sigma 10-30 km), reverse=50.  A step is one batched solve of all 256 windows
+(one hipGraph-captured persistent launch); the model is resident in HBM
+before timing starts, and the per-face schedule is picked by the autotuner
+during the untimed warm-up.
 
 The line reports station-solves/s and grid nodes/s of the batched path, the
-plain path (one table per launch) on a sample, the HBM roofline of the sweep
-kernel and the reference sphfd (compiled from its own sources, oracle/_ref)
-timed on the host, one process per table, on a bounded sample of the same
-tables, whose outputs are also compared with the GPU tables bit for bit.
+plain path (one table per launch) on a sample, p50/p99 step latency, the HBM
+roofline of the sweep kernel (kernel time from HIP events on its stream), the
+analytic FP64 error of the same grid in the unperturbed gradient medium, and
+the reference sphfd (compiled from its own sources, oracle/_ref) timed on the
+host's cores, one single-threaded process per table, on a size-stratified
+sample of the same windows (the largest included), whose outputs are also
+compared with the GPU tables bit for bit.
 
 Other workloads (one JSON line each, same keys where they apply):
   --workload c1   one 65x65x33 gradient table: plain GPU latency, the
                   reference on one core, error against the analytic
                   linear-gradient time
-  --workload c3   256 uneven windows (41^3 .. 201x201x101) of one resident
-                  model in one launch, plain sample, reference sample
+  --workload c2   48 stations x (P, S) = 96 tables of 161x161x81 (gradient x
+                  +-5 % checkerboard) in one launch, plain sample, reference
+                  sample
   --workload c4   serving: a Poisson stream of requests (81^3 .. 257x257x129,
                   each with its own updated velocities), 200 GB HBM arena,
                   p50/p99 latency and stations/s
@@ -37,7 +43,7 @@ Other workloads (one JSON line each, same keys where they apply):
                   default workload, all events per step, the oracle's location
                   timed on one core on a sample of the events
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
     python bench.py --calibrate-copy MIB     # one known-size copy, for HBM counter scaling
 """
 from __future__ import annotations
@@ -58,19 +64,15 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-from fdtomoc_amd import autotune, fdgpu, synth, tables, workloads  # noqa: E402
+from fdtomoc_amd import analytic, autotune, fdgpu, synth, tables, workloads  # noqa: E402
 
 METRIC = "stations/s & nodes/s batched vs plain at <=1e-4 s err; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VISIT = 12          # slowness read + time read + time write per node visit
-# HBM bytes (FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0, copy-calibrated) that
-# rocprofv3 counted for one launch of the default 96-table workload
-# (profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md)
-TRAFFIC_BYTES_C2 = 323.92e9
-# FP64 lane-operations of one launch of the default workload, from VALU
-# instruction counters (64 x (ADD + MUL + 2 FMA + TRANS), an upper bound)
-# (profiles/sphfd_sweep_kernel_c2_96tables_fp64.md)
-FP64_OPS_C2 = 8.557e12
+# FP64 lane-operations of one launch of the C2 workload at change budget 0,
+# from VALU instruction counters (64 x (ADD + MUL + 2 FMA + TRANS), an upper
+# bound) (profiles/sphfd_sweep_kernel_c2_96tables_fp64.md)
+FP64_OPS_C2 = {"change_budget": 0, "ops": 8.557e12, "source": "profiles/sphfd_sweep_kernel_c2_96tables_fp64.md"}
 FP64_PEAK_TFLOPS = 78.6
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "sphfd_ref")
 # HBM bytes of one c2f model update (its three kernels, FETCH_SIZE x 1.99992 +
@@ -84,8 +86,37 @@ TRAFFIC_BYTES_LOC = 2.6258e9
 
 # ------------------------------------------------------------------ helpers
 
+def host_info() -> dict:
+    """The host's CPUs as this process sees them: nproc, the CPUs it may run
+    on, the cgroup CPU quota and the CPU model."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cgroup_cpus": None,
+            "model": None}
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["cgroup_cpus"] = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
+
+
 def host_cores(requested: int) -> int:
-    return requested or min(os.cpu_count() or 1, 16)
+    """All the cores this process may use (CPU affinity, capped by the cgroup
+    quota) unless --cpu-threads says otherwise."""
+    if requested:
+        return requested
+    info = host_info()
+    n = info["affinity"]
+    if info["cgroup_cpus"]:
+        n = min(n, max(1, int(info["cgroup_cpus"])))
+    return n
 
 
 def cpu_reference(items, cores: int):
@@ -136,7 +167,7 @@ def cpu_block(secs, sizes, cores, what):
     reference processes = tables / (sum of per-table seconds / cores)."""
     busy = float(np.sum(secs)) / cores
     return {"value": round(len(secs) / busy, 5), "unit": "station-solves/s", "cores": cores, "kind": "reference",
-            "nodes_per_s": round(float(np.sum(sizes)) / busy, 1),
+            "nodes_per_s": round(float(np.sum(sizes)) / busy, 1), "host": host_info(),
             "sample": what + f"; compiled reference sphfd, one single-threaded process per table, {cores} at a "
                              f"time; wall time per table incl. file I/O, rate = tables / (sum of times / {cores})"}
 
@@ -146,6 +177,23 @@ def base_line(args, value, unit, ms_per_step, config, dtype="f64"):
             "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype,
             "data": "synthetic (seeded models and stations, reference binary layout)", "config": config}
+
+
+# HBM bytes per launch (FETCH_SIZE x calibration + WRITE_SIZE x calibration)
+# counted by rocprofv3 for one launch of a workload at the per-face schedule it
+# was profiled with; attached to a bench line only when the run uses that schedule.
+PROFILED_TRAFFIC = {
+    # FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0 at change budget 0
+    "c2": {"change_budget": 0, "bytes": 323.92e9, "source": "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md"},
+    "c3": {"change_budget": 2, "bytes": 398.23e9, "source": "profiles/sphfd_sweep_kernel_c3_256windows.md"},
+}
+
+
+def profiled_traffic(workload: str, sched: dict):
+    p = PROFILED_TRAFFIC.get(workload)
+    if p is None or p["change_budget"] != sched.get("change_budget"):
+        return None, None
+    return p["bytes"], p["source"]
 
 
 def roofline(visits, kern_s, traffic=None, source=None):
@@ -203,19 +251,28 @@ def plain_rate(ctx, specs, sample):
     return len(sample) / dt
 
 
-def host_geometry(par: dict) -> dict:
-    """Geometry of a table in radians and km, as the reference derives it
-    (src/sphfd.c:415-509): geocentric colatitude and top radius from glath,
-    dq/df from h when unset, the source's lon/colatitude/radius."""
-    D = synth.DEGRAD
-    lat0, z0r = synth.glath(par["y0"] * D, par.get("z0", 0.0))
-    y0 = synth.HPI - lat0
-    h = par["h"]
-    dq = par.get("dq", 0.0) * D or h / synth.REARTH
-    df = par.get("df", 0.0) * D or abs(h / (synth.REARTH * math.sin(y0)))
-    lats, rs = synth.glath(par["fys"] * D, par["fzs"])
-    return {"h": h, "dq": dq, "df": df, "x0": par["x0"] * D, "y0": y0, "z0r": z0r,
-            "fxss": par["fxs"] * D, "fyss": synth.HPI - lats, "fzss": rs}
+def analytic_error(ctx, par: dict, what: str, sample: int = 20000) -> dict:
+    """FD error of the workload's grid: one table of the same grid and source
+    solved on the device (untimed) in the unperturbed gradient medium
+    Vp = 5.0 + 3.0 z/64 km/s (z the depth k h below the grid top), against
+    the analytic FP64 spherical ray-integral time (fdtomoc_amd/analytic.py) on
+    a seeded sample of nodes.  The GPU tables are bit-identical to the
+    reference's, so this is the reference's own discretisation error."""
+    nz, ny, nx = par["nz"], par["ny"], par["nx"]
+    depth = np.arange(nz, dtype=np.float64) * par["h"]
+    v = np.ascontiguousarray(np.broadcast_to((5.0 + 3.0 * depth / 64.0)[:, None, None], (nz, ny, nx)),
+                             dtype=np.float32)
+    m = ctx.model(v)
+    (tid,) = ctx.solve([fdgpu.TableSpec(par, m)], fdgpu.BATCHED)
+    t = ctx.read(tid, v.shape)
+    ctx.release(tid)
+    ctx.release_model(m)
+    geom = analytic.geometry(par)
+    e = analytic.table_error(t, geom, lambda r, q, f: analytic.gradient_time(geom, 5.0, 3.0 / 64.0, geom["z0r"],
+                                                                             r, q, f), sample=sample)
+    e.update({"max": round(e["max"], 5), "mean": round(e["mean"], 5), "unit": "s",
+              "what": what + ", Vp 5.0 + 3.0 z/64 km/s, vs the FP64 spherical linear-gradient ray-integral time"})
+    return e
 
 
 # ------------------------------------------------------------------ C2 (default)
@@ -248,13 +305,14 @@ def run_c2(args):
     out["latency_ms"] = {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
                          "p99": round(float(np.percentile(wall, 99)) * 1e3, 2),
                          "what": "batched solve of all 96 tables (submit to results resident)"}
-    out["roofline"] = roofline(visits, float(kern.mean()), TRAFFIC_BYTES_C2,
-                               "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md")
-    fp64 = FP64_OPS_C2 / float(kern.mean()) / 1e12
-    out["fp64_rate"] = {"achieved_max": round(fp64, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(fp64 / FP64_PEAK_TFLOPS, 5),
-                        "ops_per_launch": FP64_OPS_C2,
-                        "source": "profiles/sphfd_sweep_kernel_c2_96tables_fp64.md"}
+    out["roofline"] = roofline(visits, float(kern.mean()), *profiled_traffic("c2", sched))
+    if sched.get("change_budget") == FP64_OPS_C2["change_budget"]:
+        fp64 = FP64_OPS_C2["ops"] / float(kern.mean()) / 1e12
+        out["fp64_rate"] = {"achieved_max": round(fp64, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(fp64 / FP64_PEAK_TFLOPS, 5), "ops_per_launch": FP64_OPS_C2["ops"],
+                            "source": FP64_OPS_C2["source"]}
+    if not args.no_analytic:
+        out["analytic_error"] = analytic_error(ctx, specs[0].par, "the first station's 161x161x81 grid")
     out["cpu_baseline"] = None
     if not args.no_cpu and os.path.exists(REF_EXE):
         cores = host_cores(args.cpu_threads)
@@ -353,8 +411,9 @@ def run_c1(args):
     specs = [fdgpu.TableSpec(par, m)]
     wall, kern, last = timed_steps(ctx, specs, args, fdgpu.PLAIN)
     t = ctx.read(last[0], v.shape)
-    exact = workloads.gradient_time_flat(g, host_geometry(par), 5.0, 3.0 / 64.0)
-    err = np.abs(t.astype(np.float64) - exact)
+    geom = analytic.geometry(par)
+    err = analytic.table_error(t, geom, lambda r, q, f: analytic.gradient_time(geom, 5.0, 3.0 / 64.0, geom["z0r"],
+                                                                               r, q, f))
     out = base_line(args, round(1.0 / float(wall.mean()), 3), "station-solves/s",
                     round(float(wall.mean()) * 1e3, 2),
                     {"workload": "C1: one station, 65x65x33 at 1 km, Vp 5.0 + 3.0 z/64 km/s, plain launch",
@@ -363,8 +422,9 @@ def run_c1(args):
                          "p99": round(float(np.percentile(wall, 99)) * 1e3, 2), "what": "one plain solve"}
     st_ = ctx.stats(last[0])
     out["roofline"] = roofline(g.n * (1 + st_["restarts"]), float(kern.mean()))
-    out["analytic_gradient_error_s"] = {"max": float(err.max()), "mean": float(err.mean()),
-                                        "what": "vs straight-chord flat-layer linear-gradient time (FP64)"}
+    err.update({"unit": "s", "what": "every node of the C1 table vs the FP64 spherical linear-gradient "
+                                     "ray-integral time (the workload's own medium)"})
+    out["analytic_error"] = err
     out["cpu_baseline"] = None
     if not args.no_cpu and os.path.exists(REF_EXE):
         secs, refs = cpu_reference([(par, v, None)], 1)
@@ -407,14 +467,20 @@ def run_c3(args):
     out["plain_sample"] = f"{len(psel)} random tables, one launch each, rate scaled by mean size"
     out["latency_ms"] = {"p50": round(float(np.percentile(wall, 50)) * 1e3, 2),
                          "p99": round(float(np.percentile(wall, 99)) * 1e3, 2), "what": "batched solve of 256 windows"}
-    out["roofline"] = roofline(visits, float(kern.mean()))
+    out["roofline"] = roofline(visits, float(kern.mean()), *profiled_traffic("c3", sched))
+    big = wins[int(np.argmax(sizes))]
+    if not args.no_analytic:
+        out["analytic_error"] = analytic_error(ctx, big.par, "the largest window's grid and station "
+                                                             f"({big.par['nx']}x{big.par['ny']}x{big.par['nz']})")
     out["cpu_baseline"] = None
     if not args.no_cpu and os.path.exists(REF_EXE):
         cores = host_cores(args.cpu_threads)
-        n = max(1, cores if args.cpu_sample < 0 else args.cpu_sample)
-        # a size-stratified sample from the smaller three quarters keeps the CPU leg bounded
-        order = np.argsort(sizes)[: (3 * len(sizes)) // 4]
-        pick = [int(order[int(q)]) for q in np.linspace(0, len(order) - 1, n)]
+        n = max(2, cores if args.cpu_sample < 0 else args.cpu_sample)
+        # a sample stratified over the whole size range, smallest and largest
+        # window included: its mean size is the workload's, so its rate in
+        # stations/s stands for the whole batch
+        order = np.argsort(sizes, kind="stable")
+        pick = [int(order[int(round(q))]) for q in np.linspace(0, len(order) - 1, n)]
         items = []
         for k in pick:
             w = wins[k]
@@ -422,10 +488,13 @@ def run_c3(args):
             p = w.par
             items.append((p, np.ascontiguousarray(v[oz:oz + p["nz"], oy:oy + p["ny"], ox:ox + p["nx"]]), None))
         secs, refs = cpu_reference(items, cores)
-        cb = cpu_block(secs, sizes[pick], cores, f"{n} windows stratified by size from the smaller 3/4")
+        cb = cpu_block(secs, sizes[pick], cores, f"{n} of the 256 windows at evenly spaced size ranks, the "
+                                                 f"smallest and the largest included (sample mean "
+                                                 f"{float(np.mean(sizes[pick])):.0f} nodes, workload mean "
+                                                 f"{float(sizes.mean()):.0f})")
         out["cpu_baseline"] = cb
-        out["gpu_over_cpu"] = round(out["nodes_per_s"] / cb["nodes_per_s"], 2)
-        out["gpu_over_cpu_basis"] = "nodes/s (the tables differ in size)"
+        out["gpu_over_cpu"] = round(stations_s / cb["value"], 2)
+        out["gpu_over_cpu_nodes"] = round(out["nodes_per_s"] / cb["nodes_per_s"], 2)
         err, nd = compare([ctx.read(last[k], refs[q].shape) for q, k in enumerate(pick)], refs)
         out["max_abs_err_vs_reference_s"] = err
         out["nodes_differing_vs_reference"] = nd
@@ -755,11 +824,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2w", "c3", "c4", "c5", "c2f", "loc"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2w", "c3", "c4", "c5", "c2f", "loc"])
     ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-analytic", action="store_true", help="skip the untimed analytic-error solve (profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-face schedule")
     ap.add_argument("--schedule", type=int, default=-1, help="set the per-face schedule (CHANGE budget) instead of tuning")
     ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")

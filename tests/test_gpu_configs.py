@@ -1,0 +1,240 @@
+"""GPU parity at the five benchmark workloads' real sizes (fdtomoc_amd.workloads):
+
+  C1  one 65x65x33 table, plain and batched, against the oracle live
+  C2  the full 96-table launch of 161x161x81: eight tables against the
+      oracle, properties on a quarter of them
+  C3  all 256 windows in one launch: the four smallest, two in between and
+      the largest against the oracle, properties on a quarter of them
+  C4  a Poisson stream of serving requests at real sizes, with the stream's
+      largest request (256x256x129, whose first pass sweeps 53,352-node
+      BOTTOM faces: LEVEL state in global scratch), each equal to the oracle
+      and to the same table solved alone
+  C5  321x321x161 tables (first-pass faces of 101,761 nodes on the bottom,
+      51,681 on the sides) on both workgroup builds against the oracle
+
+and the global-scratch LEVEL path on faces just above the LDS limits of each
+build (52,000 nodes wide, 26,800 narrow).  Every comparison is of float32 bit
+patterns.  The oracle's results for the large tables are recorded in
+tests/golden/config_tables.npz (tests/golden/make_config_golden.py, which
+also checks them against the reference compiled from its sources); a table
+whose regenerated input does not reproduce the recorded input hash is solved
+by the oracle live instead.
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import numpy as np
+import pytest
+
+import config_cases as cc
+import oracle_bind as ob
+from cases import par_dict
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config_tables.npz")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from fdtomoc_amd import fdgpu
+    c = fdgpu.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def expected():
+    return cc.Expected(GOLDEN)
+
+
+def _bits(a, b):
+    return int((a.view(np.uint32) != b.view(np.uint32)).sum())
+
+
+def _properties(t: np.ndarray, par: dict) -> dict:
+    """Size-independent properties of a solved table: every node reached,
+    no negative time, the minimum inside the source cube, and causality --
+    each node outside the source cube is no earlier than the earliest of its
+    26 neighbours (every stencil result is >= the times it was computed from,
+    and times only decrease afterwards)."""
+    from scipy import ndimage
+    p = ob.Par(**par)
+    geo = ob.geometry(p)
+    xs, ys, zs, nc = geo["xs"], geo["ys"], geo["zs"], p.ncube
+    fp = np.ones((3, 3, 3), bool)
+    fp[1, 1, 1] = False
+    nmin = ndimage.minimum_filter(t, footprint=fp, mode="nearest")
+    cube = np.zeros(t.shape, bool)
+    cube[max(zs - nc, 0):zs + nc + 1, max(ys - nc, 0):ys + nc + 1, max(xs - nc, 0):xs + nc + 1] = True
+    k, j, i = np.unravel_index(int(np.argmin(t)), t.shape)
+    return dict(reached=bool((t < 1e9).all()), nonneg=bool((t >= 0).all()),
+                min_in_cube=bool(cube[k, j, i]), causal=bool((t[~cube] >= nmin[~cube]).all()))
+
+
+def _assert_properties(t, par, label):
+    p = _properties(t, par)
+    assert all(p.values()), f"{label}: {p}"
+
+
+def _check(expected, key, par, vel, table, stats):
+    passes, ndiff, how = expected.check(key, par, vel, table)
+    print(f"{key} {table.shape}: passes {stats['passes']}/{passes}, {ndiff} nodes differ ({how})", flush=True)
+    assert stats["error"] & ~1 == 0
+    assert stats["passes"] == passes
+    assert ndiff == 0
+
+
+# ----------------------------------------------------------------------- C1
+
+def test_c1_full_size(ctx):
+    from fdtomoc_amd import fdgpu, workloads
+    for seed in range(3):
+        g, v, st = workloads.c1(seed)
+        par = ob.Par(**workloads.table_par(g, st))
+        t_ref, ost = ob.solve(par, ob.slowness(v))
+        m = ctx.model(v)
+        for mode in (fdgpu.PLAIN, fdgpu.BATCHED):
+            (tid,) = ctx.solve([fdgpu.TableSpec(par_dict(par), m)], mode)
+            t = ctx.read(tid, v.shape)
+            gst = ctx.stats(tid)
+            ctx.release(tid)
+            print(f"C1 seed {seed} mode {mode}: passes {gst['passes']}/{ost['passes']}, "
+                  f"{_bits(t, t_ref)} nodes differ", flush=True)
+            assert gst["error"] & ~1 == 0 and gst["passes"] == ost["passes"]
+            assert _bits(t, t_ref) == 0
+        ctx.release_model(m)
+
+
+# ----------------------------------------------------------------------- C2
+
+def test_c2_full_launch(ctx, expected):
+    from fdtomoc_amd import fdgpu
+    g, vp, vs, pars = cc.c2()
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs = [fdgpu.TableSpec(p, mp if q % 2 == 0 else ms) for q, p in enumerate(pars)]
+    assert len(specs) == 96
+    t0 = time.time()
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    print(f"C2: 96 tables in one launch, {time.time() - t0:.2f} s", flush=True)
+    for q in cc.C2_CHECK:
+        _check(expected, f"c2_t{q}", pars[q], vp if q % 2 == 0 else vs, ctx.read(ids[q], vp.shape),
+               ctx.stats(ids[q]))
+    for q, tid in enumerate(ids):
+        assert ctx.stats(tid)["error"] & ~1 == 0
+        if q % 4 == 0:   # properties on a quarter of the tables keep the test short
+            _assert_properties(ctx.read(tid, vp.shape), pars[q], f"C2 table {q}")
+        ctx.release(tid)
+    ctx.release_model(mp)
+    ctx.release_model(ms)
+
+
+# ----------------------------------------------------------------------- C3
+
+def test_c3_all_windows(ctx, expected):
+    from fdtomoc_amd import fdgpu
+    G, v, wins, check = cc.c3()
+    m = ctx.model(v)
+    t0 = time.time()
+    ids = ctx.solve([fdgpu.TableSpec(w.par, m, w.origin) for w in wins], fdgpu.BATCHED)
+    print(f"C3: 256 windows in one launch, {time.time() - t0:.2f} s", flush=True)
+
+    def shape(w):
+        return (w.par["nz"], w.par["ny"], w.par["nx"])
+    for q in check:
+        _check(expected, f"c3_w{q}", wins[q].par, cc.c3_window(v, wins[q]), ctx.read(ids[q], shape(wins[q])),
+               ctx.stats(ids[q]))
+    for q, tid in enumerate(ids):
+        assert ctx.stats(tid)["error"] & ~1 == 0
+        if q % 4 == 0:
+            _assert_properties(ctx.read(tid, shape(wins[q])), wins[q].par, f"C3 window {q}")
+        ctx.release(tid)
+    ctx.release_model(m)
+
+
+# ----------------------------------------------------------------------- C4
+
+def test_c4_stream_at_real_sizes(ctx, expected):
+    """Requests of C4's stream (Poisson arrivals at 3/s replayed 5x faster, own
+    velocities each), submitted at their arrival times while earlier ones run."""
+    from fdtomoc_amd import fdgpu
+    chosen, vels = cc.c4()
+    assert max(r.shape[1] * r.shape[2] for r in chosen) > 52000
+    ctx.serve_start((257, 257, 129))
+    try:
+        ids = []
+        t_start = time.time()
+        base = chosen[0].t_arrival
+        for r, v in sorted(zip(chosen, vels), key=lambda rv: rv[0].t_arrival):
+            wait = (r.t_arrival - base) * 0.2 - (time.time() - t_start)
+            if wait > 0:
+                time.sleep(min(wait, 0.5))
+            ids.append((r, v, ctx.serve_submit(fdgpu.TableSpec(r.par, -1), v)))
+        got = {}
+        for r, v, rid in ids:
+            ctx.serve_wait(rid, 600.0)
+            got[id(r)] = (ctx.serve_read(rid, v.shape), ctx.serve_stats(rid))
+            ctx.serve_release(rid)
+    finally:
+        ctx.serve_stop()
+    for q, (r, v) in enumerate(zip(chosen, vels)):
+        t, st = got[id(r)]
+        _check(expected, f"c4_r{q}", r.par, v, t, st)
+    # alone: the first request and the largest, each in its own batched launch
+    for q in (0, len(chosen) - 1):
+        mm = ctx.model(vels[q])
+        (tid,) = ctx.solve([fdgpu.TableSpec(chosen[q].par, mm)])
+        assert _bits(ctx.read(tid, vels[q].shape), got[id(chosen[q])][0]) == 0
+        ctx.release(tid)
+        ctx.release_model(mm)
+
+
+# ----------------------------------------------------------------------- C5
+
+@pytest.mark.parametrize("lanes", [256, 512])
+def test_c5_full_size_tables(ctx, expected, lanes):
+    """C5's 321x321x161 grid, P and S of one station in one launch: on the
+    narrow build every large face (101,761-node bottom faces, 51,681-node
+    sides) keeps its LEVEL state in global scratch, on the wide build the
+    bottom faces do."""
+    from fdtomoc_amd import fdgpu
+    g, vp, vs, pars = cc.c5()
+    s = 0 if lanes == 256 else 1
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    ctx.set_workgroup(lanes)
+    try:
+        t0 = time.time()
+        ids = ctx.solve([fdgpu.TableSpec(pars[s], mp), fdgpu.TableSpec(pars[s], ms)], fdgpu.BATCHED)
+        print(f"C5 ({lanes} lanes): P and S tables in one launch, {time.time() - t0:.1f} s", flush=True)
+    finally:
+        ctx.set_workgroup(0)
+    for tid, name, vel in zip(ids, "PS", (vp, vs)):
+        _check(expected, f"c5_s{s}_{name}", pars[s], vel, ctx.read(tid, vp.shape), ctx.stats(tid))
+        ctx.release(tid)
+    ctx.release_model(mp)
+    ctx.release_model(ms)
+
+
+# ----------------------------------------------- faces above the LDS limits
+
+@pytest.mark.parametrize("lanes,n_top,nz", [(512, 232, 45), (256, 170, 34)])
+def test_faces_above_lds_limit(ctx, expected, lanes, n_top, nz):
+    """First-pass BOTTOM faces of 53,824 nodes (wide build, LDS limit 52,000)
+    and 28,900 nodes (narrow build, limit 26,800): LEVEL pending counters and
+    successor masks in global scratch, bit for bit against the oracle."""
+    from fdtomoc_amd import fdgpu
+    assert n_top * n_top > (52000 if lanes == 512 else 26800)
+    cases = [(seed,) + cc.large_face(n_top, nz, seed) for (n, _, seed) in cc.LARGE_FACE if n == n_top]
+    models = [ctx.model(v) for _, _, v in cases]
+    ctx.set_workgroup(lanes)
+    try:
+        ids = ctx.solve([fdgpu.TableSpec(p, m) for (_, p, _), m in zip(cases, models)], fdgpu.BATCHED)
+    finally:
+        ctx.set_workgroup(0)
+    for (seed, p, v), tid in zip(cases, ids):
+        _check(expected, f"lf_{n_top}_{seed}", p, v, ctx.read(tid, v.shape), ctx.stats(tid))
+        ctx.release(tid)
+    for m in models:
+        ctx.release_model(m)

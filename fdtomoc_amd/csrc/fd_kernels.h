@@ -28,6 +28,14 @@ inline long long fd_geo_lds_capacity(int variant) {
     return variant == FD_NARROW ? kNarrowGeoLds : fdsweep::kGeoLdsDoubles;
 }
 
+// Size of the device array behind fdgpu_solve_noalloc (8 MiB: one table whose
+// largest face has up to about 130,000 nodes).
+#define FD_NOALLOC_SCRATCH_BYTES (8u << 20)
+// Device address of that array on the current device (hipGetSymbolAddress).
+extern "C" hipError_t fd_noalloc_scratch(void **ptr);
+// One table by one wide workgroup, descriptor and scratch layout by value.
+extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::Scratch *W, hipStream_t stream);
+
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream);
 // Kernel address of a sweep build, for occupancy queries.

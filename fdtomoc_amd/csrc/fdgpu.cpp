@@ -272,6 +272,7 @@ const char *fdgpu_strerror(int code) {
     case FDGPU_EKERNEL: return "sweep kernel reported an internal error";
     case FDGPU_EBUSY: return "busy: serving arena, staging or queue full, or call not allowed while serving";
     case FDGPU_ETIMEOUT: return "timed out waiting for a request";
+    case FDGPU_ESCRATCH: return "table too large for the fixed device scratch of fdgpu_solve_noalloc";
     }
     return "unknown error";
 }
@@ -300,7 +301,8 @@ int fdgpu_open(int device, unsigned long long budget, fdgpu_ctx **out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
-        hipMalloc(&c->d_queue, sizeof(int)) != hipSuccess) {
+        hipMalloc(&c->d_queue, sizeof(int)) != hipSuccess ||
+        fd_noalloc_scratch((void **)&c->d_fixed) != hipSuccess) {
         fdgpu_close(c);
         return FDGPU_EHIP;
     }
@@ -536,6 +538,95 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         c->tables[id] = std::move(T);
     }
     return kerr ? FDGPU_EKERNEL : FDGPU_OK;
+}
+
+int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, int *stats, int nstats) {
+    if (!c || !d || !d_time || (nstats > 0 && !stats)) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;   // the request loop holds every CU on the stream
+    if (!c->d_fixed) return FDGPU_EHIP;
+    FD_HIP(hipSetDevice(c->device));
+    auto mit = c->models.find(d->model);
+    if (mit == c->models.end()) return FDGPU_EINPUT;
+    const Model &m = mit->second;
+    Table T;
+    std::vector<float> preset;
+    int rc = prepare_table(*d, m.nx, m.ny, m.nz, T, preset);
+    if (rc) return rc;
+    if (fd_geo_lds_need(T.par.ny, T.par.nz) > fd_geo_lds_capacity(FD_WIDE)) return FDGPU_EINPUT;
+    // layout of the fixed array: the host-prepared arrays, counters, last
+    // full-plane marks, geometry tables, then one workgroup's face scratch
+    const long long maxF = fdprep::max_face(T.par.nx, T.par.ny, T.par.nz);
+    const int maxdim = std::max(T.par.nx, std::max(T.par.ny, T.par.nz));
+    const size_t b_sin = align256(sizeof(double) * T.prep.sinq.size());
+    const size_t b_idx = align256(sizeof(int) * T.prep.init_idx.size());
+    const size_t b_dist = align256(sizeof(double) * T.prep.init_dist.size());
+    const size_t b_val = align256(sizeof(float) * T.prep.init_val.size());
+    const size_t b_st = align256(sizeof(int) * fdsweep::kStatCount);
+    const size_t b_lf = align256(sizeof(int) * 6 * (size_t)maxdim);
+    const size_t b_geo = align256(sizeof(double) * (size_t)fdsweep::geo_doubles(T.par.ny, T.par.nz));
+    const size_t b_host = b_sin + b_idx + b_dist + b_val;
+    const size_t need = b_host + b_st + b_lf + b_geo + slot_bytes(maxF);
+    if (need > FD_NOALLOC_SCRATCH_BYTES) return FDGPU_ESCRATCH;
+    char *base = c->d_fixed;
+    std::vector<char> aux(b_host, 0);
+    memcpy(aux.data(), T.prep.sinq.data(), sizeof(double) * T.prep.sinq.size());
+    memcpy(aux.data() + b_sin, T.prep.init_idx.data(), sizeof(int) * T.prep.init_idx.size());
+    memcpy(aux.data() + b_sin + b_idx, T.prep.init_dist.data(), sizeof(double) * T.prep.init_dist.size());
+    memcpy(aux.data() + b_sin + b_idx + b_dist, T.prep.init_val.data(), sizeof(float) * T.prep.init_val.size());
+    FD_HIP(hipMemcpyAsync(base, aux.data(), b_host, hipMemcpyHostToDevice, c->stream));
+    if (T.par.srctype != 1)
+        FD_HIP(hipMemcpyAsync(d_time, preset.data(), sizeof(float) * (size_t)T.n, hipMemcpyHostToDevice, c->stream));
+    fdsweep::TableJob J;
+    memset(&J, 0, sizeof J);
+    J.nx = T.par.nx; J.ny = T.par.ny; J.nz = T.par.nz;
+    J.h = T.prep.h; J.dq = T.prep.dq; J.df = T.prep.df; J.y0 = T.prep.y0; J.z0r = T.prep.z0r;
+    J.sinq = (const double *)base;
+    J.s_sy = m.nx;
+    J.s_sz = (long long)m.nx * m.ny;
+    J.slow = m.d_slow + (long long)T.oz * J.s_sz + (long long)T.oy * J.s_sy + T.ox;
+    J.time = d_time;
+    J.init_mode = T.prep.init_mode;
+    J.src_sidx = (long long)T.prep.zs * J.s_sz + (long long)T.prep.ys * J.s_sy + T.prep.xs;
+    J.ninit = (int)T.prep.init_idx.size();
+    J.init_idx = (const int *)(base + b_sin);
+    J.init_dist = (const double *)(base + b_sin + b_idx);
+    J.init_val = (const float *)(base + b_sin + b_idx + b_dist);
+    memcpy(J.box, T.prep.box, sizeof J.box);
+    memcpy(J.dflag, T.prep.dflag, sizeof J.dflag);
+    memcpy(J.grow, T.prep.grow, sizeof J.grow);
+    J.reverse = T.prep.reverse; J.headpref = T.prep.headpref; J.maxrad = T.prep.maxrad;
+    J.radius0 = T.prep.radius0; J.srcwall = T.prep.srcwall;
+    int *d_stats = (int *)(base + b_host);
+    J.stats = d_stats;
+    J.mod_stamp = nullptr;   // no per-node stamps: the reverse-pass skip is off (same results)
+    J.skip_ok = 0;
+    J.last_full = (int *)(base + b_host + b_st);
+    J.maxdim = maxdim;
+    J.change_budget = c->change_budget;
+    J.geo_fill = (double *)(base + b_host + b_st + b_lf);
+    // one slot of face scratch, laid out as fill_slots lays out each slot
+    char *sb = base + b_host + b_st + b_lf + b_geo;
+    const size_t f = align256(sizeof(float) * (size_t)maxF);
+    fdsweep::Scratch W;
+    size_t o = 0;
+    auto take = [&](size_t n) { char *p = sb + o; o += n; return p; };
+    W.O = (float *)take(f); W.key = (unsigned *)take(f); W.key2 = (unsigned *)take(f);
+    W.val = (int *)take(f); W.val2 = (int *)take(f); W.rank = (int *)take(f); W.ord = (int *)take(f);
+    W.pend = (int *)take(f); W.stat = (int *)take(f); W.tout0 = (float *)take(f); W.tout4 = (float *)take(f);
+    W.list0 = (int *)take(f); W.list1 = (int *)take(f); W.list2 = (int *)take(f); W.stamp = (int *)take(f);
+    W.flag = (unsigned char *)take(align256((size_t)maxF));
+    bool ok = hipEventRecord(c->ev0, c->stream) == hipSuccess;
+    ok = ok && fd_launch_one(&J, &W, c->stream) == hipSuccess;
+    ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) return FDGPU_EHIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
+    int st[fdsweep::kStatCount];
+    FD_HIP(hipMemcpy(st, d_stats, sizeof st, hipMemcpyDeviceToHost));
+    for (int q = 0; q < nstats && q < fdsweep::kStatCount; q++) stats[q] = st[q];
+    return (st[fdsweep::kStatError] & ~1) ? FDGPU_EKERNEL : FDGPU_OK;
 }
 
 double fdgpu_last_kernel_ms(fdgpu_ctx *c) { return c ? c->last_ms : 0.0; }

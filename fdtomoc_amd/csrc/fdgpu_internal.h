@@ -205,6 +205,9 @@ struct fdgpu_ctx {
     // the last fdgpu_locate: passes, pass ms, node-search ms, node-search table bytes
     double loc_timing[4] = {0.0, 0.0, 0.0, 0.0};
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
+    // this device's copy of the fixed array behind fdgpu_solve_noalloc
+    // (sweep_kernel.hip `scratch`), looked up once at fdgpu_open
+    char *d_fixed = nullptr;
     fdgpu_detail::Serve serve;
     std::unique_ptr<fdgpu_detail::Writer> writer;   // created by the first background write
     std::mutex mu;

@@ -9,6 +9,9 @@
 //                        (serve_shared.h), each solved exactly like a batched
 //                        table
 //   slowness_kernel      1/v in FP32 (sphfd.c:689-690), model upload
+//   sphfd_one_kernel     one table, its descriptor passed by value, for the
+//                        entry point that allocates nothing (its device state
+//                        lives in the file-scope `scratch` array below)
 //
 // Compiled with -ffp-contract=off: FP64 stencil arithmetic must round
 // operation by operation like the reference's x86-64 build.
@@ -159,7 +162,33 @@ __global__ void slowness_kernel(const float *__restrict__ vel, float *__restrict
         slow[i] = 1.0f / vel[i];
 }
 
+// One table solved by one wide workgroup, with the descriptor and the
+// scratch layout passed by value: the kernel of fdgpu_solve_noalloc.
+__global__ void __launch_bounds__(kWideThreads, 2) sphfd_one_kernel(fdsweep::TableJob J, fdsweep::Scratch W) {
+    __shared__ fdsweep::Shared sh;
+    DevExec x;
+    fdsweep::solve_table(x, sh, J, W);
+}
+
 }  // namespace
+
+// Device memory of fdgpu_solve_noalloc: the table's host-prepared arrays,
+// counters, geometry tables and per-face scratch, carved out by the host for
+// each call.  Every GPU the library runs on holds its own copy until the
+// process ends; calls on one GPU share it, so calls that overlap on one GPU
+// race.  Host code never uses this name as an address (on the host it is a
+// stand-in symbol): fd_noalloc_scratch returns the device address.
+static __device__ __align__(256) unsigned char scratch[FD_NOALLOC_SCRATCH_BYTES];
+
+extern "C" hipError_t fd_noalloc_scratch(void **ptr) {
+    // the current device's copy of the array
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::Scratch *W, hipStream_t stream) {
+    hipLaunchKernelGGL(sphfd_one_kernel, dim3(1), dim3(kWideThreads), 0, stream, *J, *W);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream) {

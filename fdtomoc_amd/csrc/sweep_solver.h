@@ -62,7 +62,8 @@ struct TableJob {
     int grow[6];               // kminus kplus iplus iminus jminus jplus
     int reverse, headpref, maxrad, radius0, srcwall;
     int *stats;                // kStat* counters (device)
-    int *mod_stamp;            // nx*ny*nz: face counter of each node's last change
+    int *mod_stamp;            // nx*ny*nz: face counter of each node's last change (NULL:
+                               // not kept; then skip_ok must be 0)
     int *last_full;            // 6 * maxdim: face counter at the start of the last
                                // full-plane face per (face type, plane)
     int maxdim;                // max(nx, ny, nz)
@@ -418,7 +419,7 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
             }
             if (o.stored) {
                 J.time[node] = o.tnew;
-                J.mod_stamp[node] = fcur;
+                if (J.mod_stamp) J.mod_stamp[node] = fcur;
                 if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
             }
             W.tout0[rp] = o.tout0;
@@ -522,7 +523,7 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
             __builtin_memcpy(&uc, &cv, 4);
             if (un != uc) {
                 J.time[node] = nv;
-                J.mod_stamp[node] = fcur;
+                if (J.mod_stamp) J.mod_stamp[node] = fcur;
                 W.list2[x.atomic_add(&sh.n[2], 1)] = p;
             }
         }
@@ -732,7 +733,8 @@ template <class X, class SH>
 FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
     const int T = x.nthr(), tid = x.tid();
     const long long N = (long long)J.nx * J.ny * J.nz;
-    for (long long q = tid; q < N; q += T) J.mod_stamp[q] = 0;
+    if (J.mod_stamp)
+        for (long long q = tid; q < N; q += T) J.mod_stamp[q] = 0;
     for (int q = tid; q < 6 * J.maxdim; q += T) J.last_full[q] = 0;
     if (J.init_mode != 1) {
         for (long long q = tid; q < N; q += T) J.time[q] = 1.0e10f;

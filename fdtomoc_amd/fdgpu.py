@@ -205,7 +205,7 @@ class FdgpuError(RuntimeError):
         self.code = code
 
 
-EBUSY, ETIMEOUT = -10, -11
+EBUSY, ETIMEOUT, ESCRATCH = -10, -11, -12
 SERVE_INFO = ["on", "outstanding", "arena_used", "arena_bytes", "launches", "workgroups"]
 
 
@@ -228,6 +228,7 @@ def lib():
         L.fdgpu_model_update.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.fdgpu_model_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_solve.argtypes = [C.c_void_p, C.POINTER(CTable), C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.fdgpu_solve_noalloc.argtypes = [C.c_void_p, C.POINTER(CTable), C.c_void_p, C.c_void_p, C.c_int]
         L.fdgpu_last_kernel_ms.argtypes = [C.c_void_p]
         L.fdgpu_last_kernel_ms.restype = C.c_double
         L.fdgpu_table_read.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_ulonglong]
@@ -437,6 +438,18 @@ class Context:
         ids = (C.c_int * max(n, 1))()
         _check(lib().fdgpu_solve(self._h, arr, n, mode, ids), "fdgpu_solve")
         return list(ids[:n])
+
+    def solve_noalloc(self, spec: TableSpec, d_time: int) -> dict:
+        """fdgpu_solve_noalloc: one table into the caller's device buffer at
+        address d_time (par nx*ny*nz floats), no device allocation; returns
+        the table's counters."""
+        d = CTable()
+        keep = []
+        self._fill(d, spec, keep)
+        s = np.zeros(len(STAT_NAMES), dtype=np.int32)
+        _check(lib().fdgpu_solve_noalloc(self._h, C.byref(d), C.c_void_p(d_time), s.ctypes.data, len(STAT_NAMES)),
+               "fdgpu_solve_noalloc")
+        return dict(zip(STAT_NAMES, s.tolist()))
 
     def set_schedule(self, change_budget: int):
         """Per-face schedule knob (fdgpu_set_schedule); results do not depend on it."""

@@ -39,7 +39,9 @@ enum {
     FDGPU_EBUSY = -10,      /* serving: arena/staging/queue full (retry after releasing
                                finished requests), or a call that allocates or frees
                                device memory made while the request loop runs */
-    FDGPU_ETIMEOUT = -11    /* serving: request not finished within the wait's timeout */
+    FDGPU_ETIMEOUT = -11,   /* serving: request not finished within the wait's timeout */
+    FDGPU_ESCRATCH = -12    /* fdgpu_solve_noalloc: the table's device state exceeds the
+                               library's fixed 8 MiB array */
 };
 
 /* Par-file values of one table (keys of src/sphfd.c:383-530), before the
@@ -185,6 +187,23 @@ enum { FDGPU_BATCHED = 0, FDGPU_PLAIN = 1 };
  * tables from a queue (largest first).  FDGPU_PLAIN: one launch per table, in
  * order.  The tables stay resident; ids are written to table_ids[0..n). */
 int fdgpu_solve(fdgpu_ctx *ctx, const fdgpu_table_desc *tables, int n, int mode, int *table_ids);
+
+/* One table into a caller's device buffer, with no device allocation (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same solve as fdgpu_solve, bit for bit, by one
+ * workgroup: the input is resident model d->model (created beforehand), the
+ * output d->par.nx*ny*nz floats written to `d_time` (device memory of the
+ * caller; for srctype 2/3 the preset is copied into it first).  Every other
+ * piece of device state -- the table's prepared arrays, counters, geometry
+ * tables and face scratch -- lives in one fixed 8 MiB array of the library,
+ * one per GPU, held until the process ends: a table whose largest face needs
+ * more (about 130,000 nodes) is refused with FDGPU_ESCRATCH.  Calls on one GPU
+ * share that array and must not overlap (they race); the reverse-pass skip
+ * is off here (it needs a per-node array), results are unchanged.  stats:
+ * NULL or nstats counters as fdgpu_table_stats returns them.  Synchronous;
+ * FDGPU_EBUSY while serving is on. */
+int fdgpu_solve_noalloc(fdgpu_ctx *ctx, const fdgpu_table_desc *d, float *d_time, int *stats, int nstats);
 
 /* Device time of the last fdgpu_solve's (or coarse-model call's) kernel
  * launches, in milliseconds, measured with HIP events on the launch stream. */

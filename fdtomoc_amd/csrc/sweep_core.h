@@ -820,6 +820,101 @@ FD_INLINE bool needs_carry(const Nbhd &N, const Geo &G, int i, int j, int k, flo
     return any2 && !self;
 }
 
+// One trial's stencil value (trial TI of face F at node (j, k)); for a 2D
+// trial t0/t4 are the carried registers its causality test reads.  The
+// arithmetic of every kind, in the reference's order; eval_node and the
+// trial-parallel LEVEL rounds both evaluate trials through this.
+template <int F, int TI>
+FD_INLINE double trial_value(const Nbhd &N, const Geo &G, int j, int k, double t0, double t4) {
+    constexpr Trial T = FaceSpec<F>::trials[TI];
+    const float slow = N.s[nb_index<F>(0, 0, 0)];
+    if constexpr (T.kind == K3D) {
+        double t[7], s[8];
+        static_for<0, 8>([&](auto P) {
+            constexpr P3 o = P < 7 ? T.pt[P] : P3{0, 0, 0};
+            constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
+            if constexpr (P < 7) t[P] = N.t[q];
+            s[P] = N.s[q];
+        });
+        return st_3d<F, TI>(t, s, G, j, k);
+    } else if constexpr (T.kind == KNE) {
+        double t[8], s[8];
+        static_for<0, 5>([&](auto P) {
+            constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
+            t[P] = N.t[q];
+            s[P] = N.s[q];
+        });
+        s[5] = slow;
+        static_assert(T.d[3].dz == 0 && T.d[3].dy == 0 && T.d[4].dz == 0 && T.d[4].dy == 0, "node distances");
+        constexpr int pa = geo_pair(T.d[3].form, T.d[4].form);
+        double a, ya;
+        pair_a<pa>(G, N, k, a, ya);
+        return st_ne(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_y(G, N, T.d[2], k),
+                     dist_y(G, N, T.d[3], k), dist_y(G, N, T.d[4], k), a, ya);
+    } else if constexpr (T.kind == K2D) {
+        double t[8], s[8];
+        t[0] = t0;
+        t[4] = t4;
+        s[0] = slow;
+        static_for<0, 3>([&](auto P) {
+            constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
+            t[P + 1] = N.t[q];
+            s[P + 1] = N.s[q];
+        });
+        static_assert(T.d[0].dz == 0 && T.d[0].dy == 0 && T.d[1].dz == 0 && T.d[1].dy == 0, "node distances");
+        constexpr int pa = geo_pair(T.d[0].form, T.d[1].form);
+        double a, ya;
+        pair_a<pa>(G, N, k, a, ya);
+        return st_2d(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_y(G, N, T.d[2], k),
+                     dist_y(G, N, T.d[3], k), a, ya);
+    } else if constexpr (T.kind == KNF) {
+        double t[8], s[8];
+        static_for<0, 5>([&](auto P) {
+            constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
+            t[P] = N.t[q];
+            s[P] = N.s[q];
+        });
+        s[5] = slow;
+        return st_nf(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_of(G, N, T.d[2], k));
+    } else {  // K1D
+        constexpr P3 o = T.pt[0];
+        constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
+        double step = .5 * (slow + N.s[q]);   // float sum, then double
+        static_for<0, 3>([&](auto Fi) {
+            if constexpr (T.fac[Fi].form != XNONE) step = step * factor_of(G, N, T.fac[Fi], k);
+        });
+        return (double)N.t[q] + step;
+    }
+}
+
+// The distance an improving head-wave trial divides by (fhead = (guess - tr)
+// / (den * slow)): d01 of a 2D trial, the product of the step factors of a 1D
+// trial.
+template <int F, int TI>
+FD_INLINE double trial_den(const Nbhd &N, const Geo &G, int k) {
+    constexpr Trial T = FaceSpec<F>::trials[TI];
+    if constexpr (T.kind == K2D) {
+        return dist_of(G, N, T.d[0], k);
+    } else {
+        double den = 1.0;
+        static_for<0, 3>([&](auto Fi) {
+            if constexpr (T.fac[Fi].form != XNONE) {
+                const double fv = factor_of(G, N, T.fac[Fi], k);
+                den = (Fi == 0) ? fv : den * fv;
+            }
+        });
+        return den;
+    }
+}
+
+// t[0] and t[4] as a 3D, new-edge or new-face trial leaves them.
+template <int F, int TI>
+FD_INLINE void trial_t04(const Nbhd &N, float &t0, float &t4) {
+    constexpr Trial T = FaceSpec<F>::trials[TI];
+    t0 = N.t[nb_index<F>(T.pt[0].dx, T.pt[0].dy, T.pt[0].dz)];
+    t4 = N.t[nb_index<F>(T.pt[4].dx, T.pt[4].dy, T.pt[4].dz)];
+}
+
 // The node's full trial sequence (e.g. TOP sphfd.c:1035-1760), on registers.
 // tin0/tin4 are the t[0]/t[4] left by earlier nodes; used only if out.used_in.
 template <int F>
@@ -841,92 +936,20 @@ FD_INLINE void eval_node(const Nbhd &N, const Geo &G, int i, int j, int k, float
             if (!(guess > kBig)) return;
         }
         if (!trial_open<F, TI>(G, N, i, j, k)) return;
-        if constexpr (T.kind == K3D) {
-            double t[7], s[8];
-            static_for<0, 8>([&](auto P) {
-                constexpr P3 o = P < 7 ? T.pt[P] : P3{0, 0, 0};
-                constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
-                if constexpr (P < 7) t[P] = N.t[q];
-                s[P] = N.s[q];
-            });
-            const double tr = st_3d<F, TI>(t, s, G, j, k);
-            t0 = t[0];
-            t4 = t[4];
+        const double tr = trial_value<F, TI>(N, G, j, k, t0, t4);
+        if constexpr (T.kind == K3D || T.kind == KNE || T.kind == KNF) {
+            float a, b;
+            trial_t04<F, TI>(N, a, b);
+            t0 = a;
+            t4 = b;
             own04 = true;
-            self = true;
-            if (tr < guess) guess = tr;
-        } else if constexpr (T.kind == KNE) {
-            double t[8], s[8];
-            static_for<0, 5>([&](auto P) {
-                constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
-                t[P] = N.t[q];
-                s[P] = N.s[q];
-            });
-            s[5] = slow;
-            static_assert(T.d[3].dz == 0 && T.d[3].dy == 0 && T.d[4].dz == 0 && T.d[4].dy == 0, "node distances");
-            constexpr int pa = geo_pair(T.d[3].form, T.d[4].form);
-            double a, ya;
-            pair_a<pa>(G, N, k, a, ya);
-            const double tr = st_ne(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_y(G, N, T.d[2], k),
-                                    dist_y(G, N, T.d[3], k), dist_y(G, N, T.d[4], k), a, ya);
-            t0 = t[0];
-            t4 = t[4];
-            own04 = true;
-            self = true;
-            if (tr < guess) guess = tr;
+            if constexpr (T.kind != KNF) self = true;
         } else if constexpr (T.kind == K2D) {
-            double t[8], s[8];
-            t[0] = t0;
-            t[4] = t4;
-            s[0] = slow;
-            static_for<0, 3>([&](auto P) {
-                constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
-                t[P + 1] = N.t[q];
-                s[P + 1] = N.s[q];
-            });
             if (!own04) used_in = true;
-            static_assert(T.d[0].dz == 0 && T.d[0].dy == 0 && T.d[1].dz == 0 && T.d[1].dy == 0, "node distances");
-            constexpr int pa = geo_pair(T.d[0].form, T.d[1].form);
-            double a, ya;
-            pair_a<pa>(G, N, k, a, ya);
-            const Dist D01 = dist_y(G, N, T.d[0], k);
-            const double d01 = D01.d;
-            const double tr = st_2d(t, s, D01, dist_y(G, N, T.d[1], k), dist_y(G, N, T.d[2], k),
-                                    dist_y(G, N, T.d[3], k), a, ya);
-            if (tr < guess) {
-                if constexpr (T.fhead) fhead = (guess - tr) / (d01 * (double)slow);
-                guess = tr;
-            }
-        } else if constexpr (T.kind == KNF) {
-            double t[8], s[8];
-            static_for<0, 5>([&](auto P) {
-                constexpr int q = nb_index<F>(T.pt[P].dx, T.pt[P].dy, T.pt[P].dz);
-                t[P] = N.t[q];
-                s[P] = N.s[q];
-            });
-            s[5] = slow;
-            const double tr = st_nf(t, s, dist_y(G, N, T.d[0], k), dist_y(G, N, T.d[1], k), dist_of(G, N, T.d[2], k));
-            t0 = t[0];
-            t4 = t[4];
-            own04 = true;
-            if (tr < guess) guess = tr;
-        } else {  // K1D
-            constexpr P3 o = T.pt[0];
-            constexpr int q = nb_index<F>(o.dx, o.dy, o.dz);
-            double step = .5 * (slow + N.s[q]);   // float sum, then double
-            double den = 1.0;
-            static_for<0, 3>([&](auto Fi) {
-                if constexpr (T.fac[Fi].form != XNONE) {
-                    const double fv = factor_of(G, N, T.fac[Fi], k);
-                    step = step * fv;
-                    den = (Fi == 0) ? fv : den * fv;
-                }
-            });
-            const double tr = (double)N.t[q] + step;
-            if (tr < guess) {
-                if constexpr (T.fhead) fhead = (guess - tr) / (den * (double)slow);
-                guess = tr;
-            }
+        }
+        if (tr < guess) {
+            if constexpr (T.fhead) fhead = (guess - tr) / (trial_den<F, TI>(N, G, k) * (double)slow);
+            guess = tr;
         }
     });
     out.stored = guess < (double)told;

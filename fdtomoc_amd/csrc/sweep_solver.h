@@ -273,6 +273,7 @@ FD_INLINE bool radix_sort(X &x, SH &sh, unsigned *k0, int *v0, unsigned *k1, int
     const int hi = lo + chunk < F ? lo + chunk : F;
     int *hist = sh.arena;              // 16 * T digit-major counters
     int *part = sh.arena + 16 * T;     // T partial sums
+    int *csum = part + T;              // their sums in chunks of 16
     bool swapped = false;
     for (int d = 0; d < 8; d++) {
         const unsigned vary = (sh.kor ^ sh.kand) >> (4 * d);
@@ -287,9 +288,28 @@ FD_INLINE bool radix_sort(X &x, SH &sh, unsigned *k0, int *v0, unsigned *k1, int
         for (int q = 0; q < 16; q++) s += hist[tid * 16 + q];
         part[tid] = s;
         x.sync();
+        // exclusive scan of the T partial sums in chunks of 16: chunk sums,
+        // a scan of the chunk sums, then each chunk's prefixes (a serial depth
+        // of 16 + T/16 + 16 adds instead of T)
+        const int nch = (T + 15) / 16;
+        if (tid < nch) {
+            int a = 0;
+            for (int q = 16 * tid; q < 16 * tid + 16 && q < T; q++) a += part[q];
+            csum[tid] = a;
+        }
+        x.sync();
         if (tid == 0) {
             int run = 0;
-            for (int q = 0; q < T; q++) {
+            for (int c = 0; c < nch; c++) {
+                const int v = csum[c];
+                csum[c] = run;
+                run += v;
+            }
+        }
+        x.sync();
+        if (tid < nch) {
+            int run = csum[tid];
+            for (int q = 16 * tid; q < 16 * tid + 16 && q < T; q++) {
                 const int v = part[q];
                 part[q] = run;
                 run += v;

@@ -29,8 +29,9 @@ Other workloads (one JSON line each, same keys where they apply):
   --workload c4   serving: a Poisson stream of requests (81^3 .. 257x257x129,
                   each with its own updated velocities), 200 GB HBM arena,
                   p50/p99 latency and stations/s
-  --workload c5   one GPU's share of the 1000-station P+S set on
-                  321x321x161: --tables tables per launch, resident
+  --workload c5   the 1000-station P+S set on 321x321x161, all 2000 tables
+                  resident through 5 inversion iterations (models updated
+                  and tables re-solved in place), sample tables written
   --workload c2f  c2f's coarse-to-fine interpolation (the row beside the
                   path): a 31x31x7 coarse model to the 301x301x61 fine grid
                   into two resident models per step, the reference c2f timed
@@ -579,27 +580,121 @@ def run_c4(args):
 # ------------------------------------------------------------------ C5
 
 def run_c5(args):
+    """C5: 1000 stations x (P, S) = 2000 tables of 321 x 321 x 161 (66.4 MB
+    each, 132.7 GB), all kept resident in HBM through `--iterations`
+    simulated inversion iterations.  Iteration 0 solves every table
+    (fdgpu_solve); each later iteration replaces both velocity models in place
+    (a fresh random-layer perturbation of the gradient, as an inversion step
+    would) and re-solves every table in place (fdgpu_resolve: no allocation).
+    A step is one re-solve iteration (model update + all tables), in launches
+    of 500 tables.  Each iteration writes 8 tables in the reference layout
+    through the background writer (overlapping the next iteration) and checks
+    one table against the reference compiled from its sources, run
+    concurrently on the host (the CPU baseline)."""
     g, vp, vs, sts = workloads.c5()
+    ntab = min(args.tables, 2 * len(sts))
     ctx = fdgpu.Context(0)
     mp, ms = ctx.model(vp), ctx.model(vs)
     specs = []
     for st in sts:
-        specs.append(fdgpu.TableSpec(workloads.table_par(g, st), mp))
-        specs.append(fdgpu.TableSpec(workloads.table_par(g, st), ms))
-    specs = specs[: args.tables]
-    wall, kern, last = timed_steps(ctx, specs, args)
-    restarts = np.array([ctx.stats(i)["restarts"] for i in last])
-    stations_s = len(specs) / float(wall.mean())
-    out = base_line(args, round(stations_s, 3), "station-solves/s", round(float(wall.mean()) * 1e3, 2),
-                    {"workload": f"C5 share: {len(specs)} of the 2000 P+S tables on 321x321x161, gradient x "
-                                 f"random +-4% layers, one batched launch, tables resident",
-                     "tables": len(specs), "nodes_per_table": g.n, "reverse_passes_mean": float(restarts.mean())})
-    out["nodes_per_s"] = round(len(specs) * g.n / float(wall.mean()), 1)
-    out["roofline"] = roofline(int(np.sum(g.n * (1 + restarts))), float(kern.mean()))
-    out["c5_iteration_s_extrapolated"] = round(2000 / stations_s, 1)
-    out["resident_table_bytes"] = 4 * g.n * len(specs)
-    out["cpu_baseline"] = None
-    for i in last:
+        for m in (mp, ms):
+            specs.append(fdgpu.TableSpec(workloads.table_par(g, st), m))
+    specs = specs[:ntab]
+    chunk = 500
+    outdir = tempfile.mkdtemp(prefix="c5_tables_")
+    cores = host_cores(args.cpu_threads)
+    ref_pool = cf.ThreadPoolExecutor(max_workers=max(1, min(args.iterations, cores)))
+    ref_jobs, iters, ids = [], [], []
+    pending = set()   # tables whose background writes may still run
+    t_start = time.perf_counter()
+    try:
+        for it in range(args.iterations):
+            t0 = time.perf_counter()
+            if it > 0:
+                rng = np.random.default_rng(500 + it)
+                base = synth.gradient_velocity(g, 5.0, 8.0, 64.0)
+                vp = np.ascontiguousarray(base * synth.layered_perturbation(g, rng, 0.04, 2.0, 12.0), dtype=np.float32)
+                vs = (vp / np.float32(1.73)).astype(np.float32)
+                t0 = time.perf_counter()
+                ctx.update_model(mp, vp)
+                ctx.update_model(ms, vs)
+            kern = 0.0
+            for c0 in range(0, ntab, chunk):
+                if it == 0:
+                    ids += ctx.solve(specs[c0:c0 + chunk], fdgpu.BATCHED)
+                else:
+                    if any(c0 <= k < c0 + chunk for k in pending):
+                        ctx.write_wait()   # the files of these tables are written first
+                        pending.clear()
+                    ctx.resolve(ids[c0:c0 + chunk], fdgpu.BATCHED)
+                kern += ctx.last_kernel_ms() / 1e3
+                print(f"# iteration {it}: tables {c0}..{min(c0 + chunk, ntab) - 1} done at "
+                      f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            dt = time.perf_counter() - t0
+            restarts = np.array([ctx.stats(i)["restarts"] for i in ids])
+            iters.append({"iteration": it, "s": round(dt, 2), "kernel_s": round(kern, 2),
+                          "reverse_passes_mean": float(restarts.mean()), "call": "solve" if it == 0 else "resolve"})
+            print(f"# iteration {it}: {dt:.1f} s wall, {kern:.1f} s kernels, reverse passes mean "
+                  f"{restarts.mean():.2f}", file=sys.stderr, flush=True)
+            # one table against the reference on the host, concurrently
+            q = (it * 401) % ntab
+            if not args.no_cpu and os.path.exists(REF_EXE):
+                gpu_t = ctx.read(ids[q], (g.nz, g.ny, g.nx))
+                vel = vp if q % 2 == 0 else vs
+                ref_jobs.append((it, q, gpu_t, ref_pool.submit(cpu_reference, [(specs[q].par, vel.copy(), None)], 1)))
+            # a sample of the last launch's tables to files in the reference
+            # layout, in the background while the next iteration's first
+            # launches run
+            ctx.write_wait()
+            for f in os.listdir(outdir):
+                os.remove(os.path.join(outdir, f))
+            lo = (ntab - 1) // chunk * chunk
+            sample = sorted({lo + (it * 8 + k) * 37 % (ntab - lo) for k in range(8)})
+            ctx.write_async([ids[k] for k in sample], [os.path.join(outdir, f"t{k:04d}.times") for k in sample])
+            pending = set(sample)
+        ctx.write_wait()
+        files_ok = all(os.path.getsize(os.path.join(outdir, f)) == 232 + 4 * g.n for f in os.listdir(outdir))
+        total = time.perf_counter() - t_start
+        resolve = [x["s"] for x in iters if x["call"] == "resolve"]
+        step = float(np.mean(resolve)) if resolve else iters[0]["s"]
+        out = base_line(args, round(ntab / step, 3), "station-solves/s", round(step * 1e3, 1),
+                        {"workload": f"C5: {ntab} tables (1000 stations x P, S) of 321x321x161, gradient x random "
+                                     f"+-4% layers, all resident through {args.iterations} inversion iterations "
+                                     f"(model update in place + re-solve of every table in place per step)",
+                         "tables": ntab, "nodes_per_table": g.n, "iterations": args.iterations})
+        out["steps"] = len(resolve)
+        out["warmup"] = 1
+        out["nodes_per_s"] = round(ntab * g.n / step, 1)
+        out["iterations"] = iters
+        out["run_s"] = round(total, 1)
+        out["resident_table_bytes"] = 4 * g.n * ntab
+        out["sample_files_written_ok"] = bool(files_ok)
+        last = iters[-1]
+        out["roofline"] = roofline(int(ntab * g.n * (1 + last["reverse_passes_mean"])), last["kernel_s"])
+        out["cpu_baseline"] = None
+        if ref_jobs:
+            secs, nd, err = [], 0, 0.0
+            for it, q, gpu_t, fut in ref_jobs:
+                s1, refs = fut.result()
+                secs += s1
+                e, d = compare([gpu_t], refs)
+                nd += d
+                err = max(err, e)
+            busy = float(np.mean(secs))
+            out["cpu_baseline"] = {"value": round(cores / busy, 5), "unit": "station-solves/s", "cores": cores,
+                                   "kind": "reference", "nodes_per_s": round(cores * g.n / busy, 1),
+                                   "host": host_info(),
+                                   "sample": f"one table per iteration ({len(secs)} tables), compiled reference sphfd, "
+                                             f"one single-threaded process each, run concurrently with the GPU "
+                                             f"iterations; rate = {cores} cores / mean wall time per table "
+                                             f"({busy:.1f} s incl. file I/O)"}
+            out["gpu_over_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
+            out["max_abs_err_vs_reference_s"] = err
+            out["nodes_differing_vs_reference"] = nd
+    finally:
+        ref_pool.shutdown(wait=True)
+        shutil.rmtree(outdir, ignore_errors=True)
+    for i in ids:
         ctx.release(i)
     ctx.close()
     return out
@@ -834,7 +929,8 @@ def main():
     ap.add_argument("--schedule", type=int, default=-1, help="set the per-face schedule (CHANGE budget) instead of tuning")
     ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=3.0, help="c4: mean arrivals per second")
-    ap.add_argument("--tables", type=int, default=256, help="c5: tables per launch")
+    ap.add_argument("--tables", type=int, default=2000, help="c5: resident tables (1000 stations x P, S)")
+    ap.add_argument("--iterations", type=int, default=5, help="c5: inversion iterations (the first solves)")
     ap.add_argument("--events", type=int, default=256, help="loc: events per step")
     ap.add_argument("--loc-cpu-events", type=int, default=8, help="loc: events in the CPU baseline sample")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -116,7 +116,11 @@ int interpolate(fdgpu_ctx *c, const fdgpu_coarse *co, int nx, int ny, int nz, Mo
     ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
     float ms_ = 0.f;
     if (ok && hipEventElapsedTime(&ms_, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms_;
-    if (!ok) return FDGPU_EHIP;
+    if (!ok) {   // the slowness may be partly written: no skipping on either model
+        mp.vmin = mp.vmax = 0.0;
+        ms.vmin = ms.vmax = 0.0;
+        return FDGPU_EHIP;
+    }
     // velocity ranges bound the reverse-pass skip (fdprep::skip_safe); a
     // model with a non-positive or non-finite velocity never skips
     Model *m2[2] = {&mp, &ms};

@@ -42,13 +42,15 @@ void dev_free(fdgpu_ctx *c, void *p, size_t bytes) {
     c->used -= bytes;
 }
 
-size_t slot_bytes(long long F) {
+size_t slot_bytes(long long F, long long N, int D) {
     const size_t f = align256(sizeof(float) * (size_t)F);
-    return 15 * f + align256((size_t)F);
+    size_t b = 15 * f + align256((size_t)F);
+    if (N > 0) b += align256(sizeof(int) * (size_t)N) + align256(sizeof(int) * 6 * (size_t)D);
+    return b;
 }
 
-int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots) {
-    const size_t per = slot_bytes(F);
+int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long long N, int D) {
+    const size_t per = slot_bytes(F, N, D);
     const size_t f = align256(sizeof(float) * (size_t)F);
     std::vector<fdsweep::Scratch> h((size_t)ns);
     for (int s = 0; s < ns; s++) {
@@ -72,15 +74,28 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots) {
         W.list2 = (int *)take(f);
         W.stamp = (int *)take(f);
         W.flag = (unsigned char *)take(align256((size_t)F));
+        W.mstamp = nullptr;
+        W.lastf = nullptr;
+        if (N > 0) {
+            W.mstamp = (int *)take(align256(sizeof(int) * (size_t)N));
+            W.lastf = (int *)take(align256(sizeof(int) * 6 * (size_t)D));
+        }
     }
     FD_HIP(hipMemcpy(d_slots, h.data(), sizeof(fdsweep::Scratch) * (size_t)ns, hipMemcpyHostToDevice));
     return FDGPU_OK;
 }
 
-int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
-    if (nslots <= c->slots_cap && maxF <= c->slot_face_cap) return FDGPU_OK;
+// Slots for a launch: face scratch for faces of up to maxF nodes and the
+// change stamps of tables of up to maxN nodes (the stamps live only for the
+// duration of one table's solve, so they belong to the slot, not the table);
+// grown on demand, never shrunk.
+int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF, long long maxN, int maxD) {
+    if (nslots <= c->slots_cap && maxF <= c->slot_face_cap && maxN <= c->slot_node_cap && maxD <= c->slot_dim_cap)
+        return FDGPU_OK;
     const int ns = std::max(nslots, c->slots_cap);
     const long long F = std::max(maxF, c->slot_face_cap);
+    const long long N = std::max(maxN, c->slot_node_cap);
+    const int D = std::max(maxD, c->slot_dim_cap);
     (void)hipStreamSynchronize(c->stream);
     clear_graphs(c);
     dev_free(c, c->d_scratch, c->scratch_bytes);
@@ -89,16 +104,20 @@ int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF) {
     c->d_slots = nullptr;
     c->slots_cap = 0;
     c->slot_face_cap = 0;
-    const size_t per = slot_bytes(F);
+    c->slot_node_cap = 0;
+    c->slot_dim_cap = 0;
+    const size_t per = slot_bytes(F, N, D);
     int rc = dev_alloc(c, (void **)&c->d_scratch, per * (size_t)ns);
     if (rc) return rc;
     c->scratch_bytes = per * (size_t)ns;
     rc = dev_alloc(c, (void **)&c->d_slots, sizeof(fdsweep::Scratch) * (size_t)ns);
     if (rc) return rc;
-    rc = fill_slots(c->d_scratch, ns, F, c->d_slots);
+    rc = fill_slots(c->d_scratch, ns, F, c->d_slots, N, D);
     if (rc) return rc;
     c->slots_cap = ns;
     c->slot_face_cap = F;
+    c->slot_node_cap = N;
+    c->slot_dim_cap = D;
     return FDGPU_OK;
 }
 
@@ -239,6 +258,80 @@ void release_table(fdgpu_ctx *c, Table &t) {
     t.d_aux = nullptr;
 }
 
+// Solves tables whose device buffers and descriptors exist (T.job): the
+// per-solve fields (reverse-pass skip bound from the model's current
+// velocity range, schedule), slots and descriptors sized, one captured
+// persistent launch (FDGPU_BATCHED) or one launch per table (FDGPU_PLAIN),
+// then each table's counters.  Allocates only when the slots or the
+// descriptor array must grow past every earlier launch of the context.
+int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode) {
+    const int n = (int)tabs.size();
+    if (n == 0) return FDGPU_OK;
+    long long maxF = 1, maxN = 1, geo_need = 0;
+    int maxD = 1;
+    std::vector<fdsweep::TableJob> jobs((size_t)n);
+    for (int t = 0; t < n; t++) {
+        Table &T = *tabs[(size_t)t];
+        const Model &m = c->models[T.model];
+        maxF = std::max(maxF, fdprep::max_face(T.par.nx, T.par.ny, T.par.nz));
+        maxN = std::max(maxN, T.n);
+        maxD = std::max(maxD, T.job.maxdim);
+        geo_need = std::max(geo_need, fd_geo_lds_need(T.par.ny, T.par.nz));
+        fdsweep::TableJob &J = jobs[(size_t)t];
+        J = T.job;
+        J.skip_ok = fdprep::skip_safe(to_values(T.par), T.prep, m.vmin, m.vmax) ? 1 : 0;
+        J.change_budget = c->change_budget;
+    }
+    // queue order: largest table first (longest-processing-time-first)
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    if (mode == FDGPU_BATCHED)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tabs[(size_t)a]->n > tabs[(size_t)b]->n; });
+    std::vector<fdsweep::TableJob> qjobs((size_t)n);
+    for (int t = 0; t < n; t++) qjobs[(size_t)t] = jobs[(size_t)order[(size_t)t]];
+    // wide workgroups while every table gets a CU of its own, else narrow ones
+    int variant = FD_WIDE;
+    if (mode == FDGPU_BATCHED) {
+        if (c->lanes == kNarrowThreads) variant = FD_NARROW;
+        else if (c->lanes == 0 && n > c->num_cu * c->wg_per_cu[FD_WIDE]) variant = FD_NARROW;
+        if (geo_need > fd_geo_lds_capacity(variant)) variant = FD_WIDE;   // same results, larger LDS
+    }
+    int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu[variant]) : 1;
+    if (nslots < 1) nslots = 1;
+    int rc = ensure_scratch(c, nslots, maxF, maxN, maxD);
+    if (rc) return rc;
+    rc = ensure_jobs(c, n);
+    if (rc) return rc;
+    fdsweep::TableJob *d_jobs = c->d_jobs;
+    bool ok = hipMemcpyAsync(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess;
+    ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
+    if (mode == FDGPU_BATCHED) {
+        // one captured graph per launch shape: queue reset + persistent sweep
+        hipGraphExec_t g = nullptr;
+        ok = ok && sweep_graph(c, variant, n, nslots, &g) == FDGPU_OK;
+        ok = ok && hipGraphLaunch(g, c->stream) == hipSuccess;
+    } else {
+        for (int t = 0; t < n && ok; t++) {
+            ok = fd_launch_reset(c->d_queue, c->stream) == hipSuccess;
+            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
+        }
+    }
+    ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) return FDGPU_EHIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
+    int kerr = 0;
+    for (int t = 0; t < n; t++) {
+        Table &T = *tabs[(size_t)t];
+        T.stats.assign(fdsweep::kStatCount, 0);
+        FD_HIP(hipMemcpy(T.stats.data(), T.d_stats, sizeof(int) * fdsweep::kStatCount, hipMemcpyDeviceToHost));
+        if (T.stats[fdsweep::kStatError] & ~1) kerr = 1;   // bit0 (NaN keys) is reported, not fatal
+    }
+    return kerr ? FDGPU_EKERNEL : FDGPU_OK;
+}
+
 }  // namespace fdgpu_detail
 
 using namespace fdgpu_detail;
@@ -350,9 +443,14 @@ int fdgpu_model_create(fdgpu_ctx *c, int nx, int ny, int nz, const float *vel, i
     if (rc) return rc;
     const int id = c->next_id++;
     c->models[id] = m;
-    *model = id;
     rc = fdgpu_model_update_locked(c, id, vel);
-    return rc;
+    if (rc) {   // not registered: its memory goes with it
+        dev_free(c, m.d_slow, m.bytes);
+        c->models.erase(id);
+        return rc;
+    }
+    *model = id;
+    return FDGPU_OK;
 }
 
 int fdgpu_model_update(fdgpu_ctx *c, int model, const float *vel) {
@@ -383,7 +481,6 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     // prepare every table on the host first: no device work on bad input
     std::vector<Table> tabs((size_t)n);
     std::vector<std::vector<float>> presets((size_t)n);
-    long long maxF = 1, geo_need = 0;
     for (int t = 0; t < n; t++) {
         const fdgpu_table_desc &d = desc[t];
         auto mit = c->models.find(d.model);
@@ -391,13 +488,10 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         const Model &m = mit->second;
         int rc = prepare_table(d, m.nx, m.ny, m.nz, tabs[(size_t)t], presets[(size_t)t]);
         if (rc) return rc;
-        maxF = std::max(maxF, fdprep::max_face(d.par.nx, d.par.ny, d.par.nz));
-        geo_need = std::max(geo_need, fd_geo_lds_need(d.par.ny, d.par.nz));
+        // every table's geometry row/layer tables must fit the LDS of the wide build
+        if (fd_geo_lds_need(d.par.ny, d.par.nz) > fd_geo_lds_capacity(FD_WIDE)) return FDGPU_EINPUT;
     }
-    // every table's geometry row/layer tables must fit the LDS of the wide build
-    if (geo_need > fd_geo_lds_capacity(FD_WIDE)) return FDGPU_EINPUT;
-    // device buffers per table
-    std::vector<fdsweep::TableJob> jobs((size_t)n);
+    // device buffers per table: the time grid and the host-prepared arrays
     std::map<std::vector<unsigned long long>, int> geo_keep;
     int rc = FDGPU_OK;
     int made = 0;
@@ -409,19 +503,13 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         const size_t b_dist = align256(sizeof(double) * T.prep.init_dist.size());
         const size_t b_val = align256(sizeof(float) * T.prep.init_val.size());
         const size_t b_st = align256(sizeof(int) * fdsweep::kStatCount);
-        const int maxdim = std::max(T.par.nx, std::max(T.par.ny, T.par.nz));
-        const size_t b_ms = align256(sizeof(int) * (size_t)T.n);
-        const size_t b_lf = align256(sizeof(int) * 6 * (size_t)maxdim);
-        T.bytes = b_sin + b_idx + b_dist + b_val + b_st + b_ms + b_lf;
+        T.bytes = b_sin + b_idx + b_dist + b_val + b_st;
         rc = dev_alloc(c, (void **)&T.d_time, sizeof(float) * (size_t)T.n);
         if (rc) break;
         rc = dev_alloc(c, (void **)&T.d_aux, T.bytes);
         if (rc) { dev_free(c, T.d_time, sizeof(float) * (size_t)T.n); T.d_time = nullptr; break; }
         made++;
-        // only the leading parts come from the host; the stamps and last-face
-        // marks behind them are initialised by the kernel
-        const size_t b_host = b_sin + b_idx + b_dist + b_val + b_st;
-        std::vector<char> aux(b_host, 0);
+        std::vector<char> aux(T.bytes, 0);
         size_t o = 0;
         memcpy(aux.data() + o, T.prep.sinq.data(), sizeof(double) * T.prep.sinq.size());
         const double *d_sin = (const double *)(T.d_aux + o);
@@ -436,11 +524,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         const float *d_val = (const float *)(T.d_aux + o);
         o += b_val;
         T.d_stats = (int *)(T.d_aux + o);
-        o += b_st;
-        int *d_ms = (int *)(T.d_aux + o);
-        o += b_ms;
-        int *d_lf = (int *)(T.d_aux + o);
-        if (hipMemcpyAsync(T.d_aux, aux.data(), b_host, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        if (hipMemcpyAsync(T.d_aux, aux.data(), T.bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
             rc = FDGPU_EHIP;
             break;
         }
@@ -451,7 +535,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
             break;
         }
         (void)hipStreamSynchronize(c->stream);   // host staging buffers are freed at scope end
-        fdsweep::TableJob &J = jobs[(size_t)t];
+        fdsweep::TableJob &J = T.job;
         memset(&J, 0, sizeof J);
         J.nx = T.par.nx; J.ny = T.par.ny; J.nz = T.par.nz;
         J.h = T.prep.h; J.dq = T.prep.dq; J.df = T.prep.df; J.y0 = T.prep.y0; J.z0r = T.prep.z0r;
@@ -472,11 +556,9 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         J.reverse = T.prep.reverse; J.headpref = T.prep.headpref; J.maxrad = T.prep.maxrad;
         J.radius0 = T.prep.radius0; J.srcwall = T.prep.srcwall;
         J.stats = T.d_stats;
-        J.mod_stamp = d_ms;
-        J.last_full = d_lf;
-        J.maxdim = maxdim;
-        J.skip_ok = fdprep::skip_safe(to_values(T.par), T.prep, m.vmin, m.vmax) ? 1 : 0;
-        J.change_budget = c->change_budget;
+        J.mod_stamp = nullptr;   // the solving slot's stamps
+        J.last_full = nullptr;
+        J.maxdim = std::max(T.par.nx, std::max(T.par.ny, T.par.nz));
         rc = geometry_for(c, J, T.prep.sinq, geo_keep, &J.geo);
         if (rc) break;
     }
@@ -484,60 +566,32 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         for (int t = 0; t < made; t++) release_table(c, tabs[(size_t)t]);
     };
     if (rc) { unwind(); return rc; }
-    // queue order: largest table first (longest-processing-time-first)
-    std::vector<int> order((size_t)n);
-    std::iota(order.begin(), order.end(), 0);
-    if (mode == FDGPU_BATCHED)
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tabs[(size_t)a].n > tabs[(size_t)b].n; });
-    std::vector<fdsweep::TableJob> qjobs((size_t)n);
-    for (int t = 0; t < n; t++) qjobs[(size_t)t] = jobs[(size_t)order[(size_t)t]];
-    // wide workgroups while every table gets a CU of its own, else narrow ones
-    int variant = FD_WIDE;
-    if (mode == FDGPU_BATCHED) {
-        if (c->lanes == kNarrowThreads) variant = FD_NARROW;
-        else if (c->lanes == 0 && n > c->num_cu * c->wg_per_cu[FD_WIDE]) variant = FD_NARROW;
-        if (geo_need > fd_geo_lds_capacity(variant)) variant = FD_WIDE;   // same results, larger LDS
-    }
-    int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu[variant]) : 1;
-    if (nslots < 1) nslots = 1;
-    rc = ensure_scratch(c, nslots, maxF);
-    if (rc) { unwind(); return rc; }
-    rc = ensure_jobs(c, n);
-    if (rc) { unwind(); return rc; }
-    fdsweep::TableJob *d_jobs = c->d_jobs;
-    bool ok = hipMemcpyAsync(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice,
-                             c->stream) == hipSuccess;
-    ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
-    if (mode == FDGPU_BATCHED) {
-        // one captured graph per launch shape: queue reset + persistent sweep
-        hipGraphExec_t g = nullptr;
-        ok = ok && sweep_graph(c, variant, n, nslots, &g) == FDGPU_OK;
-        ok = ok && hipGraphLaunch(g, c->stream) == hipSuccess;
-    } else {
-        for (int t = 0; t < n && ok; t++) {
-            ok = fd_launch_reset(c->d_queue, c->stream) == hipSuccess;
-            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
-        }
-    }
-    ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
-    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
-    float ms = 0.f;
-    if (ok && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
-    if (!ok) { unwind(); return FDGPU_EHIP; }
-    int kerr = 0;
+    std::vector<Table *> run((size_t)n);
+    for (int t = 0; t < n; t++) run[(size_t)t] = &tabs[(size_t)t];
+    rc = run_tables(c, run, mode);
+    if (rc && rc != FDGPU_EKERNEL) { unwind(); return rc; }
     for (int t = 0; t < n; t++) {
-        Table &T = tabs[(size_t)t];
-        T.stats.assign(fdsweep::kStatCount, 0);
-        if (hipMemcpy(T.stats.data(), T.d_stats, sizeof(int) * fdsweep::kStatCount, hipMemcpyDeviceToHost) != hipSuccess) {
-            unwind();
-            return FDGPU_EHIP;
-        }
-        if (T.stats[fdsweep::kStatError] & ~1) kerr = 1;   // bit0 (NaN keys) is reported, not fatal
         const int id = c->next_id++;
         ids[t] = id;
-        c->tables[id] = std::move(T);
+        c->tables[id] = std::move(tabs[(size_t)t]);
     }
-    return kerr ? FDGPU_EKERNEL : FDGPU_OK;
+    return rc;
+}
+
+int fdgpu_resolve(fdgpu_ctx *c, const int *ids, int n, int mode) {
+    if (!c || (!ids && n > 0) || n < 0) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;
+    FD_HIP(hipSetDevice(c->device));
+    std::vector<Table *> run((size_t)n);
+    for (int t = 0; t < n; t++) {
+        auto it = c->tables.find(ids[t]);
+        if (it == c->tables.end() || it->second.from_file || it->second.par.srctype != 1) return FDGPU_EINPUT;
+        if (writer_busy(c, ids[t])) return FDGPU_EBUSY;   // a background write still reads it
+        if (c->models.find(it->second.model) == c->models.end()) return FDGPU_EINPUT;
+        run[(size_t)t] = &it->second;
+    }
+    return run_tables(c, run, mode);
 }
 
 int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, int *stats, int nstats) {
@@ -566,7 +620,7 @@ int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, 
     const size_t b_lf = align256(sizeof(int) * 6 * (size_t)maxdim);
     const size_t b_geo = align256(sizeof(double) * (size_t)fdsweep::geo_doubles(T.par.ny, T.par.nz));
     const size_t b_host = b_sin + b_idx + b_dist + b_val;
-    const size_t need = b_host + b_st + b_lf + b_geo + slot_bytes(maxF);
+    const size_t need = b_host + b_st + b_lf + b_geo + slot_bytes(maxF, 0, 0);
     if (need > FD_NOALLOC_SCRATCH_BYTES) return FDGPU_ESCRATCH;
     char *base = c->d_fixed;
     std::vector<char> aux(b_host, 0);
@@ -616,6 +670,8 @@ int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, 
     W.pend = (int *)take(f); W.stat = (int *)take(f); W.tout0 = (float *)take(f); W.tout4 = (float *)take(f);
     W.list0 = (int *)take(f); W.list1 = (int *)take(f); W.list2 = (int *)take(f); W.stamp = (int *)take(f);
     W.flag = (unsigned char *)take(align256((size_t)maxF));
+    W.mstamp = nullptr;
+    W.lastf = nullptr;
     bool ok = hipEventRecord(c->ev0, c->stream) == hipSuccess;
     ok = ok && fd_launch_one(&J, &W, c->stream) == hipSuccess;
     ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
@@ -725,7 +781,11 @@ int fdgpu_table_write(fdgpu_ctx *c, int table, const char *path, const void *vel
 
 }  // extern "C"
 
-// model upload (called with the context lock held)
+// model upload (called with the context lock held): the velocities are
+// copied into the model's own buffer and turned into slowness there, so an
+// update allocates nothing.  The velocity range bounds the reverse-pass skip
+// (skip_safe); it is stored only once the device holds the new slowness, and
+// cleared (no skipping) if the upload fails part-way.
 int fdgpu_model_update_locked(fdgpu_ctx *c, int model, const float *vel) {
     auto it = c->models.find(model);
     if (it == c->models.end()) return FDGPU_EINPUT;
@@ -738,15 +798,14 @@ int fdgpu_model_update_locked(fdgpu_ctx *c, int model, const float *vel) {
         vmin = v < vmin ? v : vmin;
         vmax = v > vmax ? v : vmax;
     }
+    bool ok = hipMemcpyAsync(m.d_slow, vel, m.bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+    ok = ok && fd_launch_slowness(m.d_slow, m.d_slow, nv, c->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) {
+        m.vmin = m.vmax = 0.0;
+        return FDGPU_EHIP;
+    }
     m.vmin = vmin;
     m.vmax = vmax;
-    float *d_vel = nullptr;
-    int rc = dev_alloc(c, (void **)&d_vel, m.bytes);
-    if (rc) return rc;
-    const long long n = (long long)m.nx * m.ny * m.nz;
-    bool ok = hipMemcpyAsync(d_vel, vel, m.bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
-    ok = ok && fd_launch_slowness(d_vel, m.d_slow, n, c->stream) == hipSuccess;
-    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
-    dev_free(c, d_vel, m.bytes);
-    return ok ? FDGPU_OK : FDGPU_EHIP;
+    return FDGPU_OK;
 }

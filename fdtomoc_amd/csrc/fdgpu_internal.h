@@ -36,9 +36,10 @@ struct Table {
     int ox = 0, oy = 0, oz = 0;
     long long n = 0;
     float *d_time = nullptr;
-    char *d_aux = nullptr;     // sinq | init_idx | init_dist | init_val | stats | stamps | last_full
+    char *d_aux = nullptr;     // sinq | init_idx | init_dist | init_val | stats
     size_t bytes = 0;
     int *d_stats = nullptr;
+    fdsweep::TableJob job{};   // its device descriptor, for re-solves in place (fdgpu_resolve)
     std::vector<int> stats;
     fdprep::Prepared prep;
     bool from_file = false;    // loaded by fdgpu_table_load: no solve inputs behind it
@@ -191,6 +192,8 @@ struct fdgpu_ctx {
     fdsweep::Scratch *d_slots = nullptr;
     int slots_cap = 0;
     long long slot_face_cap = 0;
+    long long slot_node_cap = 0;   // per-slot change stamps: nodes of the largest table
+    int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;
     fdsweep::TableJob *d_jobs = nullptr;   // descriptors of the current batch
     int jobs_cap = 0;
@@ -222,12 +225,15 @@ bool writer_busy(fdgpu_ctx *c, int table);
 void writer_shutdown(fdgpu_ctx *c);
 int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes);
 void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
-size_t slot_bytes(long long F);
+// One slot's scratch: face arrays for faces of up to F nodes, plus change
+// stamps for tables of up to N nodes and last full-plane marks for dimensions
+// up to D (none when N == 0).
+size_t slot_bytes(long long F, long long N = 0, int D = 0);
 // Destroys the captured batched launches (their pointers are about to change).
 void clear_graphs(fdgpu_ctx *c);
 // Carves `ns` scratch slots for faces of up to F nodes out of `base` and
 // uploads their descriptors to `d_slots`.
-int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots);
+int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long long N = 0, int D = 0);
 fdprep::ParValues to_values(const fdgpu_par &p);
 // Frees every cached geometry table (stream idle).
 void clear_geos(fdgpu_ctx *c);

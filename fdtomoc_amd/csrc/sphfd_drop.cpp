@@ -107,14 +107,16 @@ int solve_group(fdgpu_ctx *c, std::vector<Loaded *> &grp, int mode) {
     }
     std::vector<int> ids(grp.size());
     rc = fdgpu_solve(c, d.data(), (int)d.size(), mode, ids.data());
+    const bool registered = rc == 0 || rc == FDGPU_EKERNEL;   // ids assigned either way
     if (rc == 0) {
         for (size_t t = 0; t < grp.size() && rc == 0; t++) {
             const fdio::VHead h = fdio::table_header(grp[t]->p, grp[t]->g, fhead, has_head);
             rc = fdgpu_table_write(c, ids[t], grp[t]->timefile.c_str(), &h);
             if (rc) fprintf(stderr, "sphfd: cannot write %s\n", grp[t]->timefile.c_str());
         }
-        for (int id : ids) fdgpu_table_release(c, id);
     }
+    if (registered)
+        for (int id : ids) fdgpu_table_release(c, id);
     fdgpu_model_release(c, model);
     return rc;
 }

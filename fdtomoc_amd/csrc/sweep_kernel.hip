@@ -157,7 +157,9 @@ __global__ void reset_kernel(int *q) {
     if (threadIdx.x == 0) q[0] = 0;
 }
 
-__global__ void slowness_kernel(const float *__restrict__ vel, float *__restrict__ slow, long long n) {
+// slow may alias vel (an in-place update): each lane reads its element before
+// writing it.
+__global__ void slowness_kernel(const float *vel, float *slow, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         slow[i] = 1.0f / vel[i];
 }

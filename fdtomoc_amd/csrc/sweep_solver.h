@@ -63,9 +63,11 @@ struct TableJob {
     int reverse, headpref, maxrad, radius0, srcwall;
     int *stats;                // kStat* counters (device)
     int *mod_stamp;            // nx*ny*nz: face counter of each node's last change (NULL:
-                               // not kept; then skip_ok must be 0)
+                               // the solving slot's Scratch::mstamp, or not kept when that
+                               // is NULL too, and then skip_ok must be 0)
     int *last_full;            // 6 * maxdim: face counter at the start of the last
-                               // full-plane face per (face type, plane)
+                               // full-plane face per (face type, plane) (NULL: the
+                               // slot's Scratch::lastf)
     int maxdim;                // max(nx, ny, nz)
     int skip_ok;               // skipping unchanged reverse-pass nodes is exact here
     int change_budget;         // CHANGE evaluations per face node before a face goes to
@@ -104,7 +106,15 @@ struct Scratch {
     int *stamp;          // CHANGE: last round a node was queued
     unsigned char *flag; // CHANGE: bit0 head wave, bit1 used carried t0/t4, bit2 sets t0/t4, bit3 evaluated;
                          // LEVEL, faces too large for LDS: successor masks
+    int *mstamp;         // per-node change stamps of the table being solved (sized for the
+                         // largest table of the launch), or NULL
+    int *lastf;          // its last full-plane faces (6 * largest dimension), or NULL
 };
+
+// The stamps and last full-plane faces a table's solve uses: its own, else
+// the solving slot's (they live only for the duration of one solve).
+FD_INLINE int *stamps_of(const TableJob &J, const Scratch &W) { return J.mod_stamp ? J.mod_stamp : W.mstamp; }
+FD_INLINE int *lastfull_of(const TableJob &J, const Scratch &W) { return J.last_full ? J.last_full : W.lastf; }
 
 // LDS scratch of a workgroup: radix histograms during a sort, then the LEVEL
 // schedule's byte-packed pending counters and successor masks (2 bytes per
@@ -237,7 +247,7 @@ FD_INLINE void load_nbhd(const TableJob &J, const Geo &G, const FaceGeom &fg, co
 
 // Latest change stamp over a node's 3x3x2 neighbourhood (its every input).
 template <int F>
-FD_INLINE int max_stamp(const TableJob &J, int i, int j, int k) {
+FD_INLINE int max_stamp(const TableJob &J, const int *ms, int i, int j, int k) {
     using S = FaceSpec<F>;
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
     int m = 0;
@@ -255,7 +265,7 @@ FD_INLINE int max_stamp(const TableJob &J, int i, int j, int k) {
                 ci = ci < 0 ? 0 : (ci >= J.nx ? J.nx - 1 : ci);
                 cj = cj < 0 ? 0 : (cj >= J.ny ? J.ny - 1 : cj);
                 ck = ck < 0 ? 0 : (ck >= J.nz ? J.nz - 1 : ck);
-                const int v = J.mod_stamp[(long long)ck * nxy + (long long)cj * nx + ci];
+                const int v = ms[(long long)ck * nxy + (long long)cj * nx + ci];
                 m = v > m ? v : m;
             }
     return m;
@@ -439,7 +449,7 @@ FD_INLINE void face_level(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
             }
             if (o.stored) {
                 J.time[node] = o.tnew;
-                if (J.mod_stamp) J.mod_stamp[node] = fcur;
+                if (int *ms = stamps_of(J, W)) ms[node] = fcur;
                 if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
             }
             W.tout0[rp] = o.tout0;
@@ -522,7 +532,7 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
             // A node whose inputs have not changed since the start of the last
             // full-plane face of this type through its plane would reproduce
             // that evaluation, whose result the grid already holds.
-            if (round == 1 && fprev > 0 && max_stamp<F>(J, i, j, k) < fprev) {
+            if (round == 1 && fprev > 0 && max_stamp<F>(J, stamps_of(J, W), i, j, k) < fprev) {
                 W.flag[p] = 0;
                 nskip++;
                 continue;
@@ -543,7 +553,7 @@ FD_INLINE bool face_change(X &x, SH &sh, const TableJob &J, const Scratch &W, Ge
             __builtin_memcpy(&uc, &cv, 4);
             if (un != uc) {
                 J.time[node] = nv;
-                if (J.mod_stamp) J.mod_stamp[node] = fcur;
+                if (int *ms = stamps_of(J, W)) ms[node] = fcur;
                 W.list2[x.atomic_add(&sh.n[2], 1)] = p;
             }
         }
@@ -689,7 +699,7 @@ FD_INLINE void sweep_face(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
     const int na = FaceSpec<F>::a_ax == AXX ? J.nx : FaceSpec<F>::a_ax == AXY ? J.ny : J.nz;
     const int nb = FaceSpec<F>::b_ax == AXX ? J.nx : FaceSpec<F>::b_ax == AXY ? J.ny : J.nz;
     const bool full = fg.a0 == 0 && fg.wa == na && fg.b0 == 0 && fg.wb == nb;
-    int *lastf = J.last_full + F * J.maxdim + fg.fixed;
+    int *lastf = lastfull_of(J, W) + F * J.maxdim + fg.fixed;
     const int fprev = (full && J.skip_ok) ? *lastf : 0;
     // gather the pre-face values and the inner-layer keys
     unsigned kor = 0u, kand = 0xffffffffu;
@@ -753,9 +763,10 @@ template <class X, class SH>
 FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
     const int T = x.nthr(), tid = x.tid();
     const long long N = (long long)J.nx * J.ny * J.nz;
-    if (J.mod_stamp)
-        for (long long q = tid; q < N; q += T) J.mod_stamp[q] = 0;
-    for (int q = tid; q < 6 * J.maxdim; q += T) J.last_full[q] = 0;
+    if (int *ms = stamps_of(J, W))
+        for (long long q = tid; q < N; q += T) ms[q] = 0;
+    int *lf = lastfull_of(J, W);
+    for (int q = tid; q < 6 * J.maxdim; q += T) lf[q] = 0;
     if (J.init_mode != 1) {
         for (long long q = tid; q < N; q += T) J.time[q] = 1.0e10f;
         x.sync();

@@ -228,6 +228,7 @@ def lib():
         L.fdgpu_model_update.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.fdgpu_model_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_solve.argtypes = [C.c_void_p, C.POINTER(CTable), C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.fdgpu_resolve.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
         L.fdgpu_solve_noalloc.argtypes = [C.c_void_p, C.POINTER(CTable), C.c_void_p, C.c_void_p, C.c_int]
         L.fdgpu_last_kernel_ms.argtypes = [C.c_void_p]
         L.fdgpu_last_kernel_ms.restype = C.c_double
@@ -438,6 +439,12 @@ class Context:
         ids = (C.c_int * max(n, 1))()
         _check(lib().fdgpu_solve(self._h, arr, n, mode, ids), "fdgpu_solve")
         return list(ids[:n])
+
+    def resolve(self, tables: list[int], mode: int = BATCHED):
+        """fdgpu_resolve: re-solve resident tables in place against their
+        models' current velocities."""
+        ids = np.ascontiguousarray(tables, dtype=np.int32)
+        _check(lib().fdgpu_resolve(self._h, ids.ctypes.data, len(ids), mode), "fdgpu_resolve")
 
     def solve_noalloc(self, spec: TableSpec, d_time: int) -> dict:
         """fdgpu_solve_noalloc: one table into the caller's device buffer at

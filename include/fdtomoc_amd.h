@@ -74,7 +74,8 @@ const char *fdgpu_strerror(int code);
  * (src/sphfd.c:151-152).  Stored as slowness 1/v rounded to float as
  * src/sphfd.c:689-690 computes it.  `model` receives an id. */
 int fdgpu_model_create(fdgpu_ctx *ctx, int nx, int ny, int nz, const float *velocity, int *model);
-/* Replaces the model's velocities (e.g. after an inversion step). */
+/* Replaces the model's velocities (e.g. after an inversion step), in the
+ * model's own device buffer (no allocation). */
 int fdgpu_model_update(fdgpu_ctx *ctx, int model, const float *velocity);
 int fdgpu_model_release(fdgpu_ctx *ctx, int model);
 
@@ -204,6 +205,18 @@ int fdgpu_solve(fdgpu_ctx *ctx, const fdgpu_table_desc *tables, int n, int mode,
  * NULL or nstats counters as fdgpu_table_stats returns them.  Synchronous;
  * FDGPU_EBUSY while serving is on. */
 int fdgpu_solve_noalloc(fdgpu_ctx *ctx, const fdgpu_table_desc *d, float *d_time, int *stats, int nstats);
+
+/* Re-solves n resident tables in place against their models' current
+ * velocities (after fdgpu_model_update: an inversion iteration), reusing the
+ * tables' device memory -- sphfd's run over the same par files with updated
+ * velocity files (src/sphfd.c:200-257), without the per-table allocations,
+ * uploads and releases of a new fdgpu_solve.  Allocates nothing once an
+ * earlier solve or re-solve of the context has sized the per-workgroup
+ * scratch for tables this large.  Tables of srctype 2/3 (whose initial grid
+ * the solve overwrote) and loaded tables are refused (FDGPU_EINPUT); a table
+ * with background writes in flight returns FDGPU_EBUSY.  Same results as
+ * releasing and solving them again. */
+int fdgpu_resolve(fdgpu_ctx *ctx, const int *table_ids, int n, int mode);
 
 /* Device time of the last fdgpu_solve's (or coarse-model call's) kernel
  * launches, in milliseconds, measured with HIP events on the launch stream. */

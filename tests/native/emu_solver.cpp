@@ -100,6 +100,7 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     W.rank = rank.data(); W.ord = ord.data(); W.pend = pend.data(); W.stat = stat.data();
     W.tout0 = tout0.data(); W.tout4 = tout4.data();
     W.list0 = l0.data(); W.list1 = l1.data(); W.list2 = l2.data(); W.stamp = stamp.data(); W.flag = flag.data();
+    W.mstamp = nullptr; W.lastf = nullptr;
     fdsweep::TableJob J;
     std::memset(&J, 0, sizeof J);
     J.nx = p.nx; J.ny = p.ny; J.nz = p.nz;

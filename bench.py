@@ -211,14 +211,21 @@ def roofline(visits, kern_s, traffic=None, source=None):
 def tune(ctx, specs, args, sample=None):
     """Per-face schedule picked by the autotuner before timing (untimed, like
     the warmup); the tables are identical for every choice."""
+    if args.wave >= 0:
+        ctx.set_wave(args.wave)
     if args.schedule >= 0:
         ctx.set_schedule(args.schedule)
-        return {"change_budget": args.schedule, "tuned": False}
+        return {"wave": ctx.get_wave(), "change_budget": args.schedule, "tuned": False}
     if args.no_autotune:
-        return {"change_budget": ctx.get_schedule(), "tuned": False}
-    best, timings = autotune.tune_change_budget(ctx, specs, sample=sample)
-    print(f"# autotune change_budget: {best} {timings}", file=sys.stderr, flush=True)
-    return {"change_budget": best, "tuned": True, "candidates_s": {str(k): round(v, 3) for k, v in timings.items()}}
+        return {"wave": ctx.get_wave(), "change_budget": ctx.get_schedule(), "tuned": False}
+    if args.wave >= 0:
+        best, timings = autotune.tune_change_budget(ctx, specs, sample=sample)
+        cand = {str(k): round(v, 3) for k, v in timings.items()}
+    else:
+        (_, best), timings = autotune.tune_schedule(ctx, specs, sample=sample)
+        cand = {k: round(v, 3) for k, v in timings.items()}
+    print(f"# autotune wave {ctx.get_wave()} change_budget {best}: {cand}", file=sys.stderr, flush=True)
+    return {"wave": ctx.get_wave(), "change_budget": best, "tuned": True, "candidates_s": cand}
 
 
 def timed_steps(ctx, specs, args, mode=fdgpu.BATCHED):
@@ -927,6 +934,7 @@ def main():
     ap.add_argument("--no-analytic", action="store_true", help="skip the untimed analytic-error solve (profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-face schedule")
     ap.add_argument("--schedule", type=int, default=-1, help="set the per-face schedule (CHANGE budget) instead of tuning")
+    ap.add_argument("--wave", type=int, default=-1, help="wavefront flags (1 first pass, 3 all passes) instead of tuning")
     ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=3.0, help="c4: mean arrivals per second")
     ap.add_argument("--tables", type=int, default=2000, help="c5: resident tables (1000 stations x P, S)")

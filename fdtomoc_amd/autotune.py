@@ -1,4 +1,9 @@
-"""Autotuner of the sweep's per-face schedule.
+"""Autotuner of the sweep's schedules.
+
+Two knobs, both without effect on the results.  The wavefront flags
+(`fdgpu_set_wave`): every pass face by face (flags 0), the first pass as one
+dataflow wavefront over the table (flags 1), or every pass as a wavefront
+(flags 3); passes run face by face use the per-face schedule below.
 
 Reverse-pass faces first try the CHANGE schedule (evaluate the whole face at
 once, then re-evaluate what changed); when changes cascade through a face, the
@@ -12,6 +17,7 @@ sample of them), keeping the fastest.
 
     from fdtomoc_amd import autotune
     best, timings = autotune.tune_change_budget(ctx, specs)
+    (wave, budget), timings = autotune.tune_schedule(ctx, specs)
 """
 from __future__ import annotations
 
@@ -43,7 +49,7 @@ def tune_change_budget(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES,
             cost = 0.0
             for i in ids:
                 s = ctx.stats(i)
-                cost += (s["t_gather"] + s["t_sort"] + s["t_level"] + s["t_change"] + s["t_init"]) * tick
+                cost += (s["t_gather"] + s["t_sort"] + s["t_level"] + s["t_change"] + s["t_init"] + s["t_wave"]) * tick
         else:
             cost = wall
         for i in ids:
@@ -52,3 +58,25 @@ def tune_change_budget(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES,
     best = min(timings, key=timings.get)
     ctx.set_schedule(best)
     return best, timings
+
+
+def tune_schedule(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES, sample: int | None = None,
+                  objective: str = "wall") -> tuple[tuple[int, int], dict]:
+    """Times the passes face by face under each candidate budget (flags 0), the
+    first pass as a wavefront with reverse passes face by face (flags 1), and
+    every pass as a wavefront (flags 3); keeps the fastest on `ctx` and returns
+    ((wave flags, budget), {"w<flags>/b<budget>" or "w3": seconds})."""
+    ctx.set_wave(1)
+    best_b, t1 = tune_change_budget(ctx, specs, candidates, sample, objective)
+    timings = {f"w1/b{b}": v for b, v in t1.items()}
+    ctx.set_wave(0)
+    best_b0, t0 = tune_change_budget(ctx, specs, candidates, sample, objective)
+    timings.update({f"w0/b{b}": v for b, v in t0.items()})
+    ctx.set_wave(3)
+    _, t3 = tune_change_budget(ctx, specs, (best_b,), sample, objective)
+    timings["w3"] = t3[best_b]
+    options = {(1, best_b): t1[best_b], (0, best_b0): t0[best_b0], (3, best_b): t3[best_b]}
+    wave, budget = min(options, key=options.get)
+    ctx.set_wave(wave)
+    ctx.set_schedule(budget)
+    return (wave, budget), timings

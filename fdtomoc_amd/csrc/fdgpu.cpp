@@ -42,10 +42,19 @@ void dev_free(fdgpu_ctx *c, void *p, size_t bytes) {
     c->used -= bytes;
 }
 
+// Capacity of each per-round list of the wavefront passes: a round queues a
+// surface of the table, far below this; a pass that would overflow it
+// reports it and the table is solved face by face instead.
+long long wave_list_cap(long long N) { return std::min(N, (long long)1 << 22); }
+
 size_t slot_bytes(long long F, long long N, int D) {
     const size_t f = align256(sizeof(float) * (size_t)F);
     size_t b = 15 * f + align256((size_t)F);
-    if (N > 0) b += align256(sizeof(int) * (size_t)N) + align256(sizeof(int) * 6 * (size_t)D);
+    if (N > 0) {
+        b += align256(sizeof(int) * (size_t)N) + align256(sizeof(int) * 6 * (size_t)D);
+        // wavefront state: status, t[0], t[4], counter per node, two lists, the round word
+        b += 4 * align256(sizeof(int) * (size_t)N) + 2 * align256(sizeof(int) * (size_t)wave_list_cap(N)) + 256;
+    }
     return b;
 }
 
@@ -76,9 +85,24 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long 
         W.flag = (unsigned char *)take(align256((size_t)F));
         W.mstamp = nullptr;
         W.lastf = nullptr;
+        W.wst = nullptr; W.wt0 = nullptr; W.wt4 = nullptr;
+        W.wcnt = nullptr; W.wtmp = nullptr; W.wready = nullptr;
+        W.wcap = 0; W.wn = 0; W.wround = nullptr;
         if (N > 0) {
             W.mstamp = (int *)take(align256(sizeof(int) * (size_t)N));
             W.lastf = (int *)take(align256(sizeof(int) * 6 * (size_t)D));
+            const size_t nb = align256(sizeof(int) * (size_t)N), lb = align256(sizeof(int) * (size_t)wave_list_cap(N));
+            W.wst = (unsigned *)take(nb);
+            W.wt0 = (float *)take(nb);
+            W.wt4 = (float *)take(nb);
+            W.wcnt = (int *)take(nb);
+            W.wtmp = (int *)take(lb);
+            W.wready = (int *)take(lb);
+            W.wcap = wave_list_cap(N);
+            W.wn = N;
+            W.wround = (unsigned *)take(256);
+            // round word 0: the slot's first wavefront pass clears its status words
+            FD_HIP(hipMemset(W.wround, 0, sizeof(unsigned)));
         }
     }
     FD_HIP(hipMemcpy(d_slots, h.data(), sizeof(fdsweep::Scratch) * (size_t)ns, hipMemcpyHostToDevice));
@@ -264,7 +288,7 @@ void release_table(fdgpu_ctx *c, Table &t) {
 // persistent launch (FDGPU_BATCHED) or one launch per table (FDGPU_PLAIN),
 // then each table's counters.  Allocates only when the slots or the
 // descriptor array must grow past every earlier launch of the context.
-int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode) {
+int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no_wave) {
     const int n = (int)tabs.size();
     if (n == 0) return FDGPU_OK;
     long long maxF = 1, maxN = 1, geo_need = 0;
@@ -281,6 +305,9 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode) {
         J = T.job;
         J.skip_ok = fdprep::skip_safe(to_values(T.par), T.prep, m.vmin, m.vmax) ? 1 : 0;
         J.change_budget = c->change_budget;
+        // wavefront passes for tables the kernel initialises itself (a failed
+        // wavefront is solved again face by face, which needs no host preset)
+        J.wave = (J.init_mode != 1 && !no_wave) ? (c->wave & 3) : 0;
     }
     // queue order: largest table first (longest-processing-time-first)
     std::vector<int> order((size_t)n);
@@ -323,11 +350,22 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
     int kerr = 0;
+    std::vector<Table *> again;
     for (int t = 0; t < n; t++) {
         Table &T = *tabs[(size_t)t];
         T.stats.assign(fdsweep::kStatCount, 0);
         FD_HIP(hipMemcpy(T.stats.data(), T.d_stats, sizeof(int) * fdsweep::kStatCount, hipMemcpyDeviceToHost));
-        if (T.stats[fdsweep::kStatError] & ~1) kerr = 1;   // bit0 (NaN keys) is reported, not fatal
+        // bit 7: a wavefront pass could not finish (list capacity, no progress);
+        // the table is solved again face by face
+        if (!no_wave && (T.stats[fdsweep::kStatError] & 128)) again.push_back(&T);
+        else if (T.stats[fdsweep::kStatError] & ~1) kerr = 1;   // bit0 (NaN keys) is reported, not fatal
+    }
+    if (!again.empty()) {
+        const float ms0 = c->last_ms;
+        const int rc2 = run_tables(c, again, mode, true);
+        c->last_ms += ms0;
+        if (rc2 == FDGPU_EKERNEL) kerr = 1;
+        else if (rc2) return rc2;
     }
     return kerr ? FDGPU_EKERNEL : FDGPU_OK;
 }
@@ -568,7 +606,7 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
     if (rc) { unwind(); return rc; }
     std::vector<Table *> run((size_t)n);
     for (int t = 0; t < n; t++) run[(size_t)t] = &tabs[(size_t)t];
-    rc = run_tables(c, run, mode);
+    rc = run_tables(c, run, mode, false);
     if (rc && rc != FDGPU_EKERNEL) { unwind(); return rc; }
     for (int t = 0; t < n; t++) {
         const int id = c->next_id++;
@@ -591,7 +629,7 @@ int fdgpu_resolve(fdgpu_ctx *c, const int *ids, int n, int mode) {
         if (c->models.find(it->second.model) == c->models.end()) return FDGPU_EINPUT;
         run[(size_t)t] = &it->second;
     }
-    return run_tables(c, run, mode);
+    return run_tables(c, run, mode, false);
 }
 
 int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, int *stats, int nstats) {
@@ -672,6 +710,9 @@ int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, 
     W.flag = (unsigned char *)take(align256((size_t)maxF));
     W.mstamp = nullptr;
     W.lastf = nullptr;
+    W.wst = nullptr; W.wt0 = nullptr; W.wt4 = nullptr;   // no wavefront state: faces one by one
+    W.wcnt = nullptr; W.wtmp = nullptr; W.wready = nullptr;
+    W.wcap = 0; W.wn = 0; W.wround = nullptr;
     bool ok = hipEventRecord(c->ev0, c->stream) == hipSuccess;
     ok = ok && fd_launch_one(&J, &W, c->stream) == hipSuccess;
     ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
@@ -691,6 +732,20 @@ int fdgpu_set_schedule(fdgpu_ctx *c, int change_budget) {
     if (!c || change_budget < 0) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
     c->change_budget = change_budget;
+    return FDGPU_OK;
+}
+
+int fdgpu_set_wave(fdgpu_ctx *c, int flags) {
+    if (!c || flags < 0 || flags > 3) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->wave = flags;
+    return FDGPU_OK;
+}
+
+int fdgpu_get_wave(fdgpu_ctx *c, int *flags) {
+    if (!c || !flags) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *flags = c->wave;
     return FDGPU_OK;
 }
 

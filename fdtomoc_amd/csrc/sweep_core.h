@@ -26,10 +26,12 @@
 #include <hip/hip_runtime.h>
 #define FD_HD __host__ __device__
 #define FD_INLINE __host__ __device__ __forceinline__
+#define FD_NOINLINE __host__ __device__ __attribute__((noinline))
 #else
 #include <math.h>
 #define FD_HD
 #define FD_INLINE inline __attribute__((always_inline))
+#define FD_NOINLINE __attribute__((noinline))
 #endif
 
 namespace fdsweep {

@@ -33,6 +33,12 @@ struct DevExec {
     __device__ __forceinline__ unsigned atomic_sub_u(unsigned *p, unsigned v) const { return atomicSub(p, v); }
     __device__ __forceinline__ unsigned atomic_or(unsigned *p, unsigned v) const { return atomicOr(p, v); }
     __device__ __forceinline__ unsigned atomic_and(unsigned *p, unsigned v) const { return atomicAnd(p, v); }
+    __device__ __forceinline__ unsigned atomic_cas(unsigned *p, unsigned cmp, unsigned v) const {
+        return atomicCAS(p, cmp, v);
+    }
+    __device__ __forceinline__ unsigned long long atomic_max_u64(unsigned long long *p, unsigned long long v) const {
+        return atomicMax(p, v);
+    }
     __device__ __forceinline__ long long clock() const { return (long long)wall_clock64(); }
 };
 

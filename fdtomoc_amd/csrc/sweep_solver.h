@@ -74,6 +74,9 @@ struct TableJob {
                                // LEVEL (0: never); a schedule choice, results are identical
     const double *geo;         // geometry tables of this grid (geo_doubles(ny, nz)), or
     double *geo_fill;          // if set, a buffer the workgroup fills first and then uses
+    int wave;                  // passes run as one dataflow wavefront (sweep_wave.h) instead of
+                               // face by face: bit0 the first pass, bit1 reverse passes (the
+                               // host sets it only where the slot holds the wavefront state)
 };
 
 enum {
@@ -88,6 +91,16 @@ enum {
     kStatTimeLevelPre, kStatTimeLevelEval,
     kStatWalks,         // LEVEL: rank-order steps walked back to find carried t[0]/t[4]
     kStatTimeChange1,   // CHANGE: ticks of the first round (every node of the face)
+    // wavefront passes (sweep_wave.h)
+    kStatWavePasses,    // passes run as a wavefront
+    kStatRoundsWave,    // their rounds
+    kStatWaveSeeds,     // nodes ready at the start of a pass (no predecessors)
+    kStatWaveCarry,     // nodes that waited for the carried t[0]/t[4] of their predecessor
+    kStatWaveScans,     // face scans for such predecessors
+    kStatTimeWave,      // ticks spent in wavefront passes
+    kStatTimeWaveGroup, // of which grouping each round's nodes by face type
+    kStatTimeWaveEval,  // evaluations and queueing (phase 3)
+    kStatTimeWaveCarry, // carried-register requests (phase 4)
     kStatCount
 };
 
@@ -109,6 +122,17 @@ struct Scratch {
     int *mstamp;         // per-node change stamps of the table being solved (sized for the
                          // largest table of the launch), or NULL
     int *lastf;          // its last full-plane faces (6 * largest dimension), or NULL
+    // wavefront passes (sweep_wave.h), sized for the largest table of the launch
+    unsigned *wst;       // per node: bit0 final, bit1 set t[0]/t[4], bits 2.. round (final) or
+                         // last round queued (not final); rounds never repeat on a slot
+    float *wt0, *wt4;    // per node: t[0]/t[4] as the node left them (or its resolved carry-in)
+    int *wcnt;           // per node: earlier-face predecessors still to finalise (bits 0-7) and
+                         // in-face predecessors finalised (bits 8-15)
+    int *wtmp, *wready;  // per round: nodes queued for the next round, this round's by face type
+    long long wcap;      // capacity of each of the two lists
+    long long wn;        // nodes wst, wt0 and wt4 hold (0: no wavefront state)
+    unsigned *wround;    // the slot's next round number (one word; starts at 1 over zeroed
+                         // status words)
 };
 
 // The stamps and last full-plane faces a table's solve uses: its own, else
@@ -146,6 +170,8 @@ struct SharedT {
     float cin0, cin4;    // carried t[0]/t[4] entering the face
     int job;
     int face_ctr;        // faces swept so far in this table (stamps start at 1)
+    unsigned long long wbest;  // wavefront: result of a face scan
+    int wc[32];          // wavefront: counters and flags
     int st[kStatCount];  // counters of the current table, flushed to TableJob::stats at the end
     int arena[AW];
 };
@@ -757,6 +783,12 @@ FD_INLINE void sweep_face(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo
 }
 
 
+}  // namespace fdsweep
+
+#include "sweep_wave.h"
+
+namespace fdsweep {
+
 // ------------------------------------------------------------- one table
 
 template <class X, class SH>
@@ -826,6 +858,14 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
             for (int w = 0; w < 7; w++) sh.headw[w] = 0;
         x.sync();
         passes++;
+        if ((J.wave & (passes == 1 ? 1 : 2)) && W.wst && N <= W.wn && N < (1LL << 29) &&
+            wave_arena_words(nx, ny, nz) <= SH::kArena) {
+            const int b0[3][2] = {{x1, x2}, {y1, y2}, {z1, z2}};
+            const int f0[6] = {dx1, dx2, dy1, dy2, dz1, dz2};
+            // the whole pass at once; on failure the host solves the table face by face
+            if (!wave_pass(x, sh, J, W, G, b0, f0, rad0, radius)) break;
+            dx1 = dx2 = dy1 = dy2 = dz1 = dz2 = 0;
+        }
         while (rad0 && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
             for (int g = 1; g <= J.grow[0]; g++)
                 if (dz1) { FD_SETBOX(); sweep_face<TOP>(x, sh, J, W, G); if (z1 == 0) dz1 = 0; z1--; }

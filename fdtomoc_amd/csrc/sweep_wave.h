@@ -1,0 +1,693 @@
+// sweep_wave.h -- one pass of the expanding-box sweep (the first pass or a
+// reverse pass of sphfd.c:1008-5678) run as a dataflow wavefront over the
+// whole table instead of face by face.  Included by sweep_solver.h.
+//
+// The reference visits the faces of a pass in a fixed order and the nodes of a
+// face in ascending (inner-layer time, fill index) order; a node reads the
+// current values of its 3x3 in-face and 3x3 inner-layer neighbours.  So every
+// dependency is between two nodes of one 3x3x3 cube, and the sequential result
+// is reproduced by any order in which a node is evaluated once every
+// neighbour that precedes it is final and before any neighbour that follows it
+// is touched.  u precedes v when u's face comes first, or when both lie on one
+// face and u's (key, fill index) is lower.
+//
+// Face membership is a closed form.  Replaying the controller's box growth
+// (the same loop as solve_table) gives, for every plane of each axis, the face
+// event that sweeps it, -1 for planes inside the starting box and kWaveNever
+// for planes the pass never reaches.  A node belongs to the event with the
+// largest index among its three planes' events: at that event the other two
+// coordinates are already inside the box, and at the earlier ones they are
+// not.  The order key of a face node is its inner neighbour's time; the inner
+// neighbours of a node and of its in-face neighbours lie in the node's cube on
+// an earlier face, so they are final before the node can be found ready.
+//
+// Rounds (one workgroup, bulk-synchronous).  Every node of the pass has a
+// counter word: its earlier-face predecessors still to finalise (set at the
+// start of the pass from the event tables) and, beside it, how many of its
+// in-face predecessors have finalised.  A node evaluated in round R (exactly
+// as a LEVEL face evaluates it, sweep_core.h eval_node) updates the counters of
+// its neighbours that follow it; the update that leaves no earlier-face
+// predecessor and all in-face predecessors final queues the node for round
+// R + 1.  The number of in-face predecessors needs the order keys, which are
+// final by then.  Two neighbours are never ready in the same round (one of
+// them precedes the other), so a round reads only values final before it.
+//
+// Carried registers: fdsph2d also tests t[0]/t[4] as left by whichever node
+// ran before it in the sequential order (sphfd.c:6116-6121).  A node that
+// needs them (needs_carry) waits in a small request list until every key of
+// its face is final (per-face counters of keys not yet final), then a scan of
+// the face finds the node with the largest (key, fill) below its own: if that
+// node is final and set the registers, its t[0]/t[4] are the carry; if it did
+// not set them it passed its own carry on, so the search continues below it;
+// below the first node of a face it continues at the last node of the
+// previous face, and before the first face it is the pass's carry-in.
+#pragma once
+
+namespace fdsweep {
+
+constexpr int kWaveNever = 0x7fff0000;   // plane not swept in this pass (above every event)
+constexpr int kWaveCarryCap = 64;        // carried-register requests waiting at once
+constexpr int kWaveEvInts = 6;           // per event: 3 packed box pairs, type, face size, keys pending
+constexpr int kWaveCarryInts = 8;        // per request: node, event, bound (2), candidate, state, cand. event, -
+constexpr unsigned kWaveRoundLimit = 1u << 29;   // a slot's rounds are renumbered from 1 past this
+
+// Arena words the wavefront needs for a grid (planes' events, events, requests).
+FD_INLINE long long wave_arena_words(int nx, int ny, int nz) {
+    return (long long)(nx + ny + nz) * (1 + kWaveEvInts) + (long long)kWaveCarryCap * kWaveCarryInts;
+}
+
+// Control words in SharedT::wc.
+enum {
+    kWcNev = 0, kWcNCur, kWcNNext, kWcDone, kWcTotal, kWcErr, kWcScanBad, kWcWaiting, kWcResolved,
+    kWcTypeCnt = 16,   // 6 counters of the list being filled
+    kWcTypeOff = 22,   // 6 offsets into the list being evaluated
+};
+
+// status word: bit0 final, bit1 set t[0]/t[4], bits 2.. the round
+FD_INLINE bool wv_final_in(unsigned s, unsigned base) { return (s & 1u) && (s >> 2) >= base; }
+FD_INLINE bool wv_final_before(unsigned s, unsigned base, unsigned R) {
+    return (s & 1u) && (s >> 2) >= base && (s >> 2) < R;
+}
+
+// Views of the pass's event tables in the workgroup arena.
+struct WaveTabs {
+    int *mx, *my, *mz;   // plane -> event
+    int *ev;             // kWaveEvInts per event
+    int *car;            // kWaveCarryInts per request
+    FD_INLINE int type(int e) const { return ev[kWaveEvInts * e + 3]; }
+    FD_INLINE int fsize(int e) const { return ev[kWaveEvInts * e + 4]; }
+    FD_INLINE int *pk(int e) const { return ev + kWaveEvInts * e + 5; }
+    FD_INLINE void box(int e, int B[3][2]) const {
+        for (int a = 0; a < 3; a++) {
+            const unsigned w = (unsigned)ev[kWaveEvInts * e + a];
+            B[a][0] = (int)(w & 0xffffu) - 1;
+            B[a][1] = (int)(w >> 16) - 1;
+        }
+    }
+};
+
+FD_INLINE int imax3(int a, int b, int c) {
+    const int m = a > b ? a : b;
+    return m > c ? m : c;
+}
+
+// Face geometry of event e (as sweep_face computes it from the box).
+FD_INLINE void wave_face_geom(const WaveTabs &WT, int e, FaceGeom &fg) {
+    int B[3][2];
+    WT.box(e, B);
+    const int F = WT.type(e);
+    fg.face = F;
+    if (F == TOP || F == BOTTOM) {
+        fg.fixed = (F == TOP) ? B[2][0] : B[2][1];
+        fg.a0 = B[0][0] + 1; fg.wa = B[0][1] - 1 - fg.a0 + 1;
+        fg.b0 = B[1][0] + 1; fg.wb = B[1][1] - 1 - fg.b0 + 1;
+    } else if (F == EAST || F == WEST) {
+        fg.fixed = (F == EAST) ? B[0][1] : B[0][0];
+        fg.a0 = B[1][0] + 1; fg.wa = B[1][1] - 1 - fg.a0 + 1;
+        fg.b0 = B[2][0] + 1; fg.wb = B[2][1] - 1 - fg.b0 + 1;
+    } else {
+        fg.fixed = (F == NORTH) ? B[1][0] : B[1][1];
+        fg.a0 = B[0][0] + 1; fg.wa = B[0][1] - 1 - fg.a0 + 1;
+        fg.b0 = B[2][0] + 1; fg.wb = B[2][1] - 1 - fg.b0 + 1;
+    }
+    fg.F = (fg.wa > 0 && fg.wb > 0) ? fg.wa * fg.wb : 0;
+}
+
+// Normal axis and inward direction of a face type (FaceSpec<F>::nax / ndir).
+FD_INLINE int face_nax(int F) { return (F == TOP || F == BOTTOM) ? 2 : (F == EAST || F == WEST) ? 0 : 1; }
+FD_INLINE int face_ndir(int F) { return (F == TOP || F == WEST || F == NORTH) ? 1 : -1; }
+
+// Coordinates of fill index p of a face, and the node's fill index.
+FD_INLINE void face_coords(const FaceGeom &fg, int p, int &i, int &j, int &k) {
+    const int a = p % fg.wa, b = p / fg.wa;
+    const int A = fg.a0 + a, B = fg.b0 + b;
+    if (fg.face == TOP || fg.face == BOTTOM) { i = A; j = B; k = fg.fixed; }
+    else if (fg.face == EAST || fg.face == WEST) { i = fg.fixed; j = A; k = B; }
+    else { i = A; j = fg.fixed; k = B; }
+}
+FD_INLINE int face_fill(const FaceGeom &fg, int i, int j, int k) {
+    if (fg.face == TOP || fg.face == BOTTOM) return (j - fg.b0) * fg.wa + (i - fg.a0);
+    if (fg.face == EAST || fg.face == WEST) return (k - fg.b0) * fg.wa + (j - fg.a0);
+    return (k - fg.b0) * fg.wa + (i - fg.a0);
+}
+
+// Appends ready node v (face type F) to the next round's list.
+template <class X, class SH>
+FD_INLINE void wave_queue(X &x, SH &sh, const Scratch &W, long long v, int F) {
+    const int pos = x.atomic_add(&sh.wc[kWcNNext], 1);
+    x.atomic_add(&sh.wc[kWcTypeCnt + F], 1);
+    if (pos < W.wcap) W.wtmp[pos] = (int)((unsigned)v | ((unsigned)F << 29));
+}
+
+// Node index -> coordinates (indices below 2^29).
+FD_INLINE void node_ijk(const TableJob &J, long long v, int &i, int &j, int &k) {
+    const unsigned nx = (unsigned)J.nx, nxy = (unsigned)J.nx * (unsigned)J.ny, u = (unsigned)v;
+    k = (int)(u / nxy);
+    const unsigned r = u - (unsigned)k * nxy;
+    j = (int)(r / nx);
+    i = (int)(r - (unsigned)j * nx);
+}
+
+// Events of the planes through a node's 3x3x3 cube (-2: outside the grid).
+FD_INLINE void cube_events(const TableJob &J, const WaveTabs &WT, int i, int j, int k, int ex[3], int ey[3],
+                           int ez[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int ci = i + d - 1, cj = j + d - 1, ck = k + d - 1;
+        ex[d] = (ci >= 0 && ci < J.nx) ? WT.mx[ci] : -2;
+        ey[d] = (cj >= 0 && cj < J.ny) ? WT.my[cj] : -2;
+        ez[d] = (ck >= 0 && ck < J.nz) ? WT.mz[ck] : -2;
+    }
+}
+
+// Earlier-face predecessors of a node of event e: its neighbours swept by an
+// earlier event of the pass.
+FD_INLINE int earlier_preds(const int ex[3], const int ey[3], const int ez[3], int e) {
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        if (q == 13) continue;
+        const int dx = q % 3, dy = (q / 3) % 3, dz = q / 9;
+        if (ex[dx] == -2 || ey[dy] == -2 || ez[dz] == -2) continue;
+        const int eu = imax3(ex[dx], ey[dy], ez[dz]);
+        n += (eu >= 0 && eu < e) ? 1 : 0;
+    }
+    return n;
+}
+
+// In-face predecessors of node (i, j, k) of event e: in-plane neighbours on
+// the same face with a lower (key, fill).  Needs the keys final: the node's
+// earlier-face predecessors include every key source read here.
+FD_INLINE int inface_preds(const TableJob &J, const WaveTabs &WT, int i, int j, int k, int e, const int ex[3],
+                           const int ey[3], const int ez[3]) {
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    FaceGeom fg;
+    wave_face_geom(WT, e, fg);
+    const int F = fg.face, nax = face_nax(F), nd = face_ndir(F);
+    const int aax = (F == EAST || F == WEST) ? 1 : 0, bax = (F == TOP || F == BOTTOM) ? 1 : 2;
+    const int n3[3] = {J.nx, J.ny, J.nz};
+    float kt[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        int c[3] = {i, j, k};
+        c[aax] += q % 3 - 1;
+        c[bax] += q / 3 - 1;
+        c[nax] += nd;
+#pragma unroll
+        for (int a = 0; a < 3; a++) c[a] = c[a] < 0 ? 0 : (c[a] >= n3[a] ? n3[a] - 1 : c[a]);
+        kt[q] = J.time[(long long)c[2] * nxy + (long long)c[1] * nx + c[0]];
+    }
+    const unsigned long long kv = ((unsigned long long)order_key(kt[4]) << 32) | (unsigned)face_fill(fg, i, j, k);
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        if (q == 4) continue;
+        int c[3] = {i, j, k};
+        c[aax] += q % 3 - 1;
+        c[bax] += q / 3 - 1;
+        if (c[aax] < 0 || c[aax] >= n3[aax] || c[bax] < 0 || c[bax] >= n3[bax]) continue;
+        const int d3[3] = {c[0] - i + 1, c[1] - j + 1, c[2] - k + 1};
+        if (imax3(ex[d3[0]], ey[d3[1]], ez[d3[2]]) != e) continue;
+        const unsigned long long ku = ((unsigned long long)order_key(kt[q]) << 32) |
+                                      (unsigned)face_fill(fg, c[0], c[1], c[2]);
+        n += ku < kv ? 1 : 0;
+    }
+    return n;
+}
+
+// Collective: the largest (key << 32 | fill) below `bound` over the nodes of
+// event e (setters only if `setters`), or ~0ull if there is none.  Also
+// checks that every key it reads is final; if one is not, sets kWcScanBad.
+template <class X, class SH>
+FD_NOINLINE unsigned long long wave_scan(X &x, SH &sh, const TableJob &J, const Scratch &W, const WaveTabs &WT,
+                                       int e, unsigned long long bound, bool setters, unsigned base) {
+    const int T = x.nthr(), tid = x.tid();
+    FaceGeom fg;
+    wave_face_geom(WT, e, fg);
+    const int F = fg.face, nax = face_nax(F), nd = face_ndir(F);
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    if (tid == 0) { sh.wbest = 0ull; sh.wc[kWcScanBad] = 0; }
+    x.sync();
+    unsigned long long best = ~0ull;
+    bool bad = false;
+    for (int p = tid; p < fg.F; p += T) {
+        int i, j, k;
+        face_coords(fg, p, i, j, k);
+        int ii = i, jj = j, kk = k;
+        if (nax == 0) ii += nd; else if (nax == 1) jj += nd; else kk += nd;
+        const long long w = (long long)kk * nxy + (long long)jj * nx + ii;
+        const int ew = imax3(WT.mx[ii], WT.my[jj], WT.mz[kk]);
+        if (ew >= 0 && ew < kWaveNever && !wv_final_in(W.wst[w], base)) bad = true;
+        const unsigned long long key = ((unsigned long long)order_key(J.time[w]) << 32) | (unsigned)p;
+        if (key >= bound) continue;
+        if (setters) {
+            const unsigned s = W.wst[(long long)k * nxy + (long long)j * nx + i];
+            if (!(wv_final_in(s, base) && (s & 2u))) continue;
+        }
+        if (best == ~0ull || key > best) best = key;
+    }
+    if (best != ~0ull) x.atomic_max_u64(&sh.wbest, best + 1);   // +1: 0 stays "none"
+    if (bad) sh.wc[kWcScanBad] = 1;
+    x.sync();
+    const unsigned long long r = sh.wbest;
+    x.sync();
+    return (r == ~0ull || r == 0) ? ~0ull : r - 1;
+}
+
+// Evaluates ready node v of face type F as a LEVEL face would, then updates
+// the counters of its neighbours that follow it and queues those it leaves
+// ready.
+template <int F, class X, class SH>
+FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G, const WaveTabs &WT,
+                         long long v, unsigned base, unsigned R, int stamp0) {
+    const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
+    int i, j, k;
+    node_ijk(J, v, i, j, k);
+    int ex[3], ey[3], ez[3];
+    cube_events(J, WT, i, j, k, ex, ey, ez);
+    const int e = imax3(ex[1], ey[1], ez[1]);
+    WT.box(e, G.box);
+    FaceGeom fg{};
+    Nbhd N;
+    load_nbhd<F, false>(J, G, fg, nullptr, nullptr, 0, 0, 0, i, j, k, N);
+    const float told = N.t[4];
+    float tin0 = 0.0f, tin4 = 0.0f;
+    if (needs_carry<F>(N, G, i, j, k, told)) {
+        // its resolved request, if any
+        int slot = -1;
+        for (int c = 0; c < kWaveCarryCap; c++) {
+            const int *q = WT.car + kWaveCarryInts * c;
+            if (q[5] == 2 && q[0] == (int)v) { slot = c; break; }
+        }
+        if (slot >= 0) {
+            tin0 = W.wt0[v];
+            tin4 = W.wt4[v];
+            WT.car[kWaveCarryInts * slot + 5] = 0;   // free the request
+        } else {
+            // file a request (the node is queued again when it is resolved)
+            FaceGeom g2;
+            wave_face_geom(WT, e, g2);
+            const int nax = face_nax(F), nd = face_ndir(F);
+            int ii = i, jj = j, kk = k;
+            if (nax == 0) ii += nd; else if (nax == 1) jj += nd; else kk += nd;
+            const unsigned key = order_key(J.time[(long long)kk * nxy + (long long)jj * nx + ii]);
+            int got = -1;
+            for (int c = 0; c < kWaveCarryCap && got < 0; c++)
+                if (x.atomic_cas((unsigned *)&WT.car[kWaveCarryInts * c + 5], 0u, 3u) == 0u) got = c;
+            if (got < 0) {   // no free request: try again next round
+                wave_queue(x, sh, W, v, F);
+                return;
+            }
+            int *q = WT.car + kWaveCarryInts * got;
+            q[0] = (int)v;
+            q[1] = e;
+            q[2] = face_fill(g2, i, j, k);
+            q[3] = (int)key;
+            q[4] = -1;
+            q[6] = e;
+            x.atomic_add(&sh.st[kStatWaveCarry], 1);
+            x.atomic_exch(&q[5], 1);   // waiting
+            return;
+        }
+    }
+    NodeOut o;
+    eval_node<F>(N, G, i, j, k, told, tin0, tin4, o);
+    if (o.stored) {
+        J.time[v] = o.tnew;
+        if (int *ms = stamps_of(J, W)) ms[v] = stamp0 + e;
+        if (o.head) x.atomic_add(&sh.headw[face_slot(F)], 1);
+    }
+    W.wt0[v] = o.tout0;
+    W.wt4[v] = o.tout4;
+    W.wst[v] = (R << 2) | (o.sets ? 2u : 0u) | 1u;
+    x.atomic_add(&sh.wc[kWcDone], 1);
+    // keys this node provides: the face nodes whose inner neighbour it is
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        const int ax = d >> 1, dd = (d & 1) ? 2 : 0;   // the axis neighbour at offset dd - 1
+        const int eu = ax == 0 ? imax3(ex[dd], ey[1], ez[1]) : ax == 1 ? imax3(ex[1], ey[dd], ez[1])
+                                                                     : imax3(ex[1], ey[1], ez[dd]);
+        const bool in = ax == 0 ? ex[dd] != -2 : ax == 1 ? ey[dd] != -2 : ez[dd] != -2;
+        if (!in || eu < 0 || eu >= kWaveNever) continue;
+        const int Fu = WT.type(eu);
+        if (face_nax(Fu) == ax && face_ndir(Fu) == 1 - dd) x.atomic_add(WT.pk(eu), -1);
+    }
+    // the neighbours that follow it: those of later faces, and those of this
+    // face with a higher (key, fill) -- their keys are the inner-layer times
+    // already loaded (N.t[9..17])
+    FaceGeom fgv;
+    wave_face_geom(WT, e, fgv);
+    const int fv = face_fill(fgv, i, j, k);
+    const unsigned long long kv = ((unsigned long long)order_key(N.t[13]) << 32) | (unsigned)fv;
+    using S = FaceSpec<F>;
+    unsigned want = 0, same = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        if (q == 13) continue;
+        const int dx = q % 3, dy = (q / 3) % 3, dz = q / 9;
+        if (ex[dx] == -2 || ey[dy] == -2 || ez[dz] == -2) continue;
+        const int eu = imax3(ex[dx], ey[dy], ez[dz]);
+        if (eu < e || eu >= kWaveNever) continue;
+        if (eu > e) { want |= 1u << q; continue; }
+        // same face: an in-plane neighbour
+        const int d3[3] = {dx - 1, dy - 1, dz - 1};
+        const int da = d3[S::a_ax], db = d3[S::b_ax];
+        const unsigned long long ku = ((unsigned long long)order_key(N.t[9 + (db + 1) * 3 + (da + 1)]) << 32) |
+                                      (unsigned)(fv + db * fgv.wa + da);
+        if (ku > kv) { want |= 1u << q; same |= 1u << q; }
+    }
+    // counter word: bits 0-7 earlier-face predecessors still open, 8-15 in-face
+    // predecessors final, 16-23 the in-face predecessor count, bit 24 set once
+    // that count is known (by the update that closed the earlier-face ones)
+    int old[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        old[q] = 0;
+        if (!(want & (1u << q))) continue;
+        const int dx = q % 3, dy = (q / 3) % 3, dz = q / 9;
+        old[q] = x.atomic_add(&W.wcnt[(long long)(k + dz - 1) * nxy + (long long)(j + dy - 1) * nx + (i + dx - 1)],
+                              (same & (1u << q)) ? 256 : -1);
+    }
+    unsigned closed = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        if (!(want & (1u << q))) continue;
+        const int o = old[q];
+        const int dx = q % 3, dy = (q / 3) % 3, dz = q / 9;
+        if (same & (1u << q)) {
+            // ready when the count is known and this was the last in-face predecessor
+            if (((o >> 24) & 1) && ((o >> 8) & 255) + 1 == ((o >> 16) & 255))
+                wave_queue(x, sh, W, (long long)(k + dz - 1) * nxy + (long long)(j + dy - 1) * nx + (i + dx - 1),
+                           WT.type(e));
+        } else if ((o & 255) == 1) {
+            closed |= 1u << q;   // this update closed its earlier-face predecessors
+        }
+    }
+    // for those: count their in-face predecessors (keys final now), record the
+    // count, and queue the node if they are all final already
+    while (closed) {
+        const int q = __builtin_ctz(closed);
+        closed &= closed - 1;
+        const int ui = i + q % 3 - 1, uj = j + (q / 3) % 3 - 1, uk = k + q / 9 - 1;
+        int fx[3], fy[3], fz[3];
+        cube_events(J, WT, ui, uj, uk, fx, fy, fz);
+        const int eu = imax3(fx[1], fy[1], fz[1]);
+        const int p2 = inface_preds(J, WT, ui, uj, uk, eu, fx, fy, fz);
+        const long long u = (long long)uk * nxy + (long long)uj * nx + ui;
+        const int o2 = x.atomic_add(&W.wcnt[u], (p2 << 16) | (1 << 24));
+        if (((o2 >> 8) & 255) == p2) wave_queue(x, sh, W, u, WT.type(eu));
+    }
+}
+
+// Runs one pass from the controller state (box, growth flags, rad0, radius)
+// as a wavefront.  Returns false if the pass could not be completed (list
+// capacity, or a request that never resolves: the host then solves the table
+// face by face).
+template <class X, class SH>
+FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G, const int box0[3][2],
+                         const int flags0[6], int rad0, int radius) {
+    const int T = x.nthr(), tid = x.tid();
+    const int nx = J.nx, ny = J.ny, nz = J.nz;
+    const long long nxy = (long long)nx * ny, N = nxy * nz;
+    const long long t0 = x.clock();
+    WaveTabs WT;
+    WT.mx = sh.arena;
+    WT.my = WT.mx + nx;
+    WT.mz = WT.my + ny;
+    WT.ev = WT.mz + nz;
+    WT.car = WT.ev + (long long)kWaveEvInts * (nx + ny + nz);
+    // planes inside the starting box are interior, all others unswept so far
+    for (int q = tid; q < nx + ny + nz; q += T) {
+        int a, c;
+        if (q < nx) { a = 0; c = q; } else if (q < nx + ny) { a = 1; c = q - nx; } else { a = 2; c = q - nx - ny; }
+        WT.mx[q] = (c > box0[a][0] && c < box0[a][1]) ? -1 : kWaveNever;
+    }
+    for (int q = tid; q < kWaveCarryCap * kWaveCarryInts; q += T) WT.car[q] = 0;
+    for (int q = tid; q < 32; q += T) sh.wc[q] = 0;
+    x.sync();
+    if (tid == 0) {
+        // replay the controller (solve_table's first-pass loop) to list the events
+        int x1 = box0[0][0], x2 = box0[0][1], y1 = box0[1][0], y2 = box0[1][1], z1 = box0[2][0], z2 = box0[2][1];
+        int dx1 = flags0[0], dx2 = flags0[1], dy1 = flags0[2], dy2 = flags0[3], dz1 = flags0[4], dz2 = flags0[5];
+        int ne = 0, total = 0;
+        auto rec = [&](int F, int plane) {
+            int *E = WT.ev + kWaveEvInts * ne;
+            E[0] = (int)(((unsigned)(x1 + 1)) | ((unsigned)(x2 + 1) << 16));
+            E[1] = (int)(((unsigned)(y1 + 1)) | ((unsigned)(y2 + 1) << 16));
+            E[2] = (int)(((unsigned)(z1 + 1)) | ((unsigned)(z2 + 1) << 16));
+            E[3] = F;
+            const int wa = (F == EAST || F == WEST) ? (y2 - y1 - 1) : (x2 - x1 - 1);
+            const int wb = (F == TOP || F == BOTTOM) ? (y2 - y1 - 1) : (z2 - z1 - 1);
+            E[4] = (wa > 0 && wb > 0) ? wa * wb : 0;
+            E[5] = E[4];
+            total += E[4];
+            int *m = (F == TOP || F == BOTTOM) ? WT.mz : (F == EAST || F == WEST) ? WT.mx : WT.my;
+            m[plane] = ne;
+            ne++;
+        };
+        while (rad0 && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
+            for (int g = 1; g <= J.grow[0]; g++)
+                if (dz1) { rec(TOP, z1); if (z1 == 0) dz1 = 0; z1--; }
+            for (int g = 1; g <= J.grow[1]; g++)
+                if (dz2) { rec(BOTTOM, z2); if (z2 == nz - 1) dz2 = 0; z2++; }
+            for (int g = 1; g <= J.grow[2]; g++)
+                if (dx2) { rec(EAST, x2); if (x2 == nx - 1) dx2 = 0; x2++; }
+            for (int g = 1; g <= J.grow[3]; g++)
+                if (dx1) { rec(WEST, x1); if (x1 == 0) dx1 = 0; x1--; }
+            for (int g = 1; g <= J.grow[4]; g++)
+                if (dy1) { rec(NORTH, y1); if (y1 == 0) dy1 = 0; y1--; }
+            for (int g = 1; g <= J.grow[5]; g++)
+                if (dy2) { rec(SOUTH, y2); if (y2 == ny - 1) dy2 = 0; y2++; }
+            radius++;
+            if (radius == J.maxrad) rad0 = 0;
+        }
+        sh.wc[kWcNev] = ne;
+        sh.wc[kWcTotal] = total;
+        // the slot's round numbers: every status word of earlier passes and
+        // tables is below this pass's base
+        unsigned b = *W.wround;
+        if (b >= kWaveRoundLimit) b = 0;
+        sh.wc[kWcResolved] = (int)b;
+    }
+    x.sync();
+    unsigned base = (unsigned)sh.wc[kWcResolved];
+    if (base == 0) {
+        // the slot's round numbers are exhausted: clear every status word
+        for (long long q = tid; q < W.wn; q += T) W.wst[q] = 0u;
+        base = 1;
+    }
+    const int nev = sh.wc[kWcNev];
+    const int stamp0 = sh.face_ctr + 1;
+    const float cin0 = sh.cin0, cin4 = sh.cin4;
+    x.sync();
+    if (tid == 0) sh.wc[kWcResolved] = 0;
+    // counters: earlier-face predecessors of every node of the pass; nodes with
+    // none and no in-face predecessor are ready for the first round
+    for (long long v = tid; v < N; v += T) {
+        int i, j, k;
+        node_ijk(J, v, i, j, k);
+        int ex[3], ey[3], ez[3];
+        cube_events(J, WT, i, j, k, ex, ey, ez);
+        const int e = imax3(ex[1], ey[1], ez[1]);
+        if (e < 0 || e >= kWaveNever) continue;
+        const int p1 = earlier_preds(ex, ey, ez, e);
+        if (p1 > 0) {
+            W.wcnt[v] = p1;
+        } else {
+            const int p2 = inface_preds(J, WT, i, j, k, e, ex, ey, ez);
+            W.wcnt[v] = (p2 << 16) | (1 << 24);
+            if (p2 == 0) {
+                wave_queue(x, sh, W, v, WT.type(e));
+                x.atomic_add(&sh.st[kStatWaveSeeds], 1);
+            }
+        }
+    }
+    // the starting box's nodes are final keys of the faces beside it
+    {
+        const int lo[3] = {box0[0][0] + 1 > 0 ? box0[0][0] + 1 : 0, box0[1][0] + 1 > 0 ? box0[1][0] + 1 : 0,
+                           box0[2][0] + 1 > 0 ? box0[2][0] + 1 : 0};
+        const int hi[3] = {box0[0][1] - 1 < nx - 1 ? box0[0][1] - 1 : nx - 1,
+                           box0[1][1] - 1 < ny - 1 ? box0[1][1] - 1 : ny - 1,
+                           box0[2][1] - 1 < nz - 1 ? box0[2][1] - 1 : nz - 1};
+        const long long wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = hi[2] - lo[2] + 1;
+        const long long nin = (wx > 0 && wy > 0 && wz > 0) ? wx * wy * wz : 0;
+        for (long long q = tid; q < nin; q += T) {
+            const int i = lo[0] + (int)(q % wx), j = lo[1] + (int)((q / wx) % wy), k = lo[2] + (int)(q / (wx * wy));
+            for (int d = 0; d < 6; d++) {
+                const int ax = d >> 1, sg = (d & 1) ? 1 : -1;
+                int ui = i, uj = j, uk = k;
+                if (ax == 0) ui += sg; else if (ax == 1) uj += sg; else uk += sg;
+                if (ui < 0 || uj < 0 || uk < 0 || ui >= nx || uj >= ny || uk >= nz) continue;
+                const int eu = imax3(WT.mx[ui], WT.my[uj], WT.mz[uk]);
+                if (eu < 0 || eu >= kWaveNever) continue;
+                const int Fu = WT.type(eu);
+                if (face_nax(Fu) == ax && face_ndir(Fu) == -sg) x.atomic_add(WT.pk(eu), -1);
+            }
+        }
+    }
+    x.sync();
+    unsigned R = base;
+    bool ok = true;
+    int idle = 0;
+    for (;;) {
+        // round R evaluates the list filled in round R - 1 (wtmp), grouped by
+        // face type into wready so that waves of one type evaluate together
+        const long long tr0 = tid == 0 ? x.clock() : 0;
+        if (tid == 0) {
+            const int n = sh.wc[kWcNNext];
+            sh.wc[kWcNCur] = n;
+            sh.wc[kWcNNext] = 0;
+            int o = 0;
+            for (int f = 0; f < 6; f++) {
+                sh.wc[kWcTypeOff + f] = o;
+                o += sh.wc[kWcTypeCnt + f];
+                sh.wc[kWcTypeCnt + f] = 0;
+            }
+            int waiting = 0;
+            for (int c = 0; c < kWaveCarryCap; c++) waiting += WT.car[kWaveCarryInts * c + 5] != 0;
+            sh.wc[kWcWaiting] = waiting;
+        }
+        x.sync();
+        const int ncur = sh.wc[kWcNCur];
+        if (ncur > W.wcap) { ok = false; break; }
+        if (ncur == 0 && sh.wc[kWcWaiting] == 0) break;
+        if (ncur == 0) {
+            // only carried-register requests are open: one must resolve now
+            if (++idle > 2) { ok = false; break; }
+        } else {
+            idle = 0;
+        }
+        if (R - base > (unsigned)(2 * N + 64) || R >= (1u << 30) - 2) { ok = false; break; }
+        for (int it = tid; it < ncur; it += T) {
+            const int w = W.wtmp[it];
+            W.wready[x.atomic_add(&sh.wc[kWcTypeOff + (int)((unsigned)w >> 29)], 1)] = w;
+        }
+        if (tid == 0) sh.st[kStatRoundsWave] += 1;
+        x.sync();
+        // evaluate; each evaluation queues the neighbours it leaves ready
+        const long long tr1 = tid == 0 ? x.clock() : 0;
+        // one face type at a time: the six evaluation routines are large, and
+        // waves running different ones at once miss in the instruction cache
+        static_for<0, 6>([&](auto FT) {
+            constexpr int Fc = decltype(FT)::value;
+            const int lo = Fc == 0 ? 0 : sh.wc[kWcTypeOff + Fc - 1], hi = sh.wc[kWcTypeOff + Fc];
+            if (lo == hi) return;   // uniform: offsets read after the barrier
+            for (int it = lo + tid; it < hi; it += T) wave_eval<Fc>(x, sh, J, W, G, WT, W.wready[it] & 0x1fffffff, base, R, stamp0);
+            x.sync();
+        });
+        if (tid == 0) sh.st[kStatEvals] += ncur;
+        x.sync();
+        const long long tr2 = tid == 0 ? x.clock() : 0;
+        // carried-register requests: find each one's predecessor
+        for (int c = 0; c < kWaveCarryCap; c++) {
+            int *q = WT.car + kWaveCarryInts * c;
+            if (q[5] != 1) continue;   // uniform: read after the barrier
+            for (int guard = 0; guard < 4 * nev + 8; guard++) {
+                const int cand = q[4];
+                if (cand >= 0) {
+                    const unsigned s = W.wst[cand];
+                    if (!wv_final_in(s, base)) break;            // wait for it
+                    if (s & 2u) {                                // it set them: its t[0]/t[4]
+                        x.sync();
+                        if (tid == 0) {
+                            W.wt0[q[0]] = W.wt0[cand];
+                            W.wt4[q[0]] = W.wt4[cand];
+                            q[5] = 2;
+                            wave_queue(x, sh, W, q[0], WT.type(q[1]));
+                        }
+                        x.sync();
+                        break;
+                    }
+                    x.sync();
+                    if (tid == 0) q[4] = -1;   // it passed its own carry on: search below it
+                    x.sync();
+                    continue;
+                }
+                const int e = q[6];
+                if (e < 0) {   // before the first face: the pass's carry-in
+                    x.sync();
+                    if (tid == 0) {
+                        W.wt0[q[0]] = cin0;
+                        W.wt4[q[0]] = cin4;
+                        q[5] = 2;
+                        wave_queue(x, sh, W, q[0], WT.type(q[1]));
+                    }
+                    x.sync();
+                    break;
+                }
+                const unsigned long long bound = ((unsigned long long)(unsigned)q[3] << 32) | (unsigned)q[2];
+                if (WT.fsize(e) > 0 && *WT.pk(e) != 0) break;   // keys of that face not all final yet
+                unsigned long long best = ~0ull;
+                if (WT.fsize(e) > 0) {
+                    best = wave_scan(x, sh, J, W, WT, e, bound, false, base);
+                    if (tid == 0) sh.st[kStatWaveScans] += 1;
+                    if (sh.wc[kWcScanBad]) {   // a key not final although its face count says so
+                        if (tid == 0) sh.st[kStatError] |= 128;
+                        ok = false;
+                        break;
+                    }
+                }
+                x.sync();
+                if (tid == 0) {
+                    if (best == ~0ull) {   // continue at the last node of the previous face
+                        q[6] = e - 1;
+                        q[2] = -1;
+                        q[3] = -1;
+                    } else {
+                        FaceGeom fg;
+                        wave_face_geom(WT, e, fg);
+                        int ci, cj, ck;
+                        face_coords(fg, (int)(best & 0xffffffffu), ci, cj, ck);
+                        q[4] = (int)((long long)ck * nxy + (long long)cj * nx + ci);
+                        q[2] = (int)(unsigned)(best & 0xffffffffu);
+                        q[3] = (int)(unsigned)(best >> 32);
+                    }
+                }
+                x.sync();
+            }
+            if (!ok) break;
+        }
+        x.sync();
+        if (tid == 0) {
+            const long long tr3 = x.clock();
+            sh.st[kStatTimeWaveGroup] += (int)(tr1 - tr0);
+            sh.st[kStatTimeWaveEval] += (int)(tr2 - tr1);
+            sh.st[kStatTimeWaveCarry] += (int)(tr3 - tr2);
+        }
+        if (!ok) break;
+        R++;
+    }
+    // every node of the pass must have been evaluated exactly once
+    if (ok && sh.wc[kWcDone] != sh.wc[kWcTotal]) ok = false;
+    if (tid == 0 && !ok) sh.st[kStatError] |= 128;
+    x.sync();
+    if (ok) {
+        // t[0]/t[4] leaving the pass: those of the last node in sequential order
+        // that set them (unchanged if no node did)
+        for (int e = nev - 1; e >= 0; e--) {
+            if (WT.fsize(e) == 0) continue;
+            const unsigned long long best = wave_scan(x, sh, J, W, WT, e, ~0ull, true, base);
+            if (best == ~0ull) continue;
+            if (tid == 0) {
+                FaceGeom fg;
+                wave_face_geom(WT, e, fg);
+                int ci, cj, ck;
+                face_coords(fg, (int)(best & 0xffffffffu), ci, cj, ck);
+                const long long u = (long long)ck * nxy + (long long)cj * nx + ci;
+                sh.cin0 = W.wt0[u];
+                sh.cin4 = W.wt4[u];
+            }
+            break;
+        }
+    }
+    if (tid == 0) {
+        *W.wround = R + 1;
+        sh.face_ctr += nev;
+        sh.st[kStatWavePasses] += 1;
+        sh.st[kStatTimeWave] += (int)(x.clock() - t0);
+    }
+    x.sync();
+    return ok;
+}
+
+}  // namespace fdsweep

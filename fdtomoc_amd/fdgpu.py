@@ -18,7 +18,9 @@ EXE_PATH = os.path.join(PKG, "bin", "sphfd")
 
 STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level", "rounds_change",
               "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
-    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1"]
+    ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1",
+     "wave_passes", "rounds_wave", "wave_seeds", "wave_carry", "wave_scans", "t_wave",
+     "t_wave_group", "t_wave_eval", "t_wave_carry"]
 
 BATCHED, PLAIN = 0, 1
 
@@ -260,6 +262,8 @@ def lib():
         L.fdgpu_set_schedule.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
+        L.fdgpu_set_wave.argtypes = [C.c_void_p, C.c_int]
+        L.fdgpu_get_wave.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.fdgpu_serve_config_defaults.argtypes = [C.POINTER(CServeConfig)]
         L.fdgpu_serve_config_defaults.restype = None
         L.fdgpu_serve_start.argtypes = [C.c_void_p, C.POINTER(CServeConfig)]
@@ -466,6 +470,16 @@ class Context:
         """Workgroup shape of batched launches: 512 (one table per CU), 256 (two
         per CU) or 0 (automatic); results do not depend on it."""
         _check(lib().fdgpu_set_workgroup(self._h, int(lanes)), "fdgpu_set_workgroup")
+
+    def set_wave(self, flags: int):
+        """Wavefront passes (fdgpu_set_wave): bit0 the first pass, bit1 reverse
+        passes; results do not depend on it."""
+        _check(lib().fdgpu_set_wave(self._h, int(flags)), "fdgpu_set_wave")
+
+    def get_wave(self) -> int:
+        b = C.c_int()
+        _check(lib().fdgpu_get_wave(self._h, C.byref(b)), "fdgpu_get_wave")
+        return b.value
 
     def get_schedule(self) -> int:
         b = C.c_int()

@@ -234,6 +234,15 @@ int fdgpu_get_schedule(fdgpu_ctx *ctx, int *change_budget);
  * not depend on it): 512 lanes and one workgroup (table) per CU, or 256 lanes
  * and two per CU.  0 (default): 512 while the tables fit one per CU, else 256. */
 int fdgpu_set_workgroup(fdgpu_ctx *ctx, int lanes);
+/* Wavefront passes (no reference counterpart; results do not depend on it).
+ * A pass whose bit is set runs as one dataflow wavefront over the whole
+ * table: a node is evaluated as soon as every neighbour that precedes it in
+ * the reference's face-by-face, time-sorted order is final, so the faces of a
+ * pass overlap instead of running one after another.  Bit 0: the first pass
+ * (box growth); bit 1: reverse passes.  Default 1.  Tables of srctype 2/3 and
+ * tables whose wavefront cannot finish in the slot's state run face by face. */
+int fdgpu_set_wave(fdgpu_ctx *ctx, int flags);
+int fdgpu_get_wave(fdgpu_ctx *ctx, int *flags);
 
 /* Copies a resident table (nx*ny*nz floats) to host memory. */
 int fdgpu_table_read(fdgpu_ctx *ctx, int table, float *host, unsigned long long count);

@@ -28,6 +28,16 @@ struct HostExec {
     unsigned atomic_sub_u(unsigned *p, unsigned v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
     unsigned atomic_or(unsigned *p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
     unsigned atomic_and(unsigned *p, unsigned v) { return __atomic_fetch_and(p, v, __ATOMIC_SEQ_CST); }
+    unsigned atomic_cas(unsigned *p, unsigned cmp, unsigned v) {
+        __atomic_compare_exchange_n(p, &cmp, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+        return cmp;
+    }
+    unsigned long long atomic_max_u64(unsigned long long *p, unsigned long long v) {
+        unsigned long long o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+        while (o < v && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+        }
+        return o;
+    }
     long long clock() const {
         return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
                    .count() / 10;
@@ -69,10 +79,12 @@ static void run_threads(int nthreads, const fdsweep::TableJob &J, const fdsweep:
     for (auto &t : th) t.join();
 }
 
-// global_state: 0 per-node schedule state in the arena where a face fits
-// (as the device builds), 1 always in global scratch.
+// mode: bit0 per-node schedule state always in global scratch (else in the
+// arena where a face fits, as the device builds); bit1 the first pass as a
+// wavefront, bit2 reverse passes as wavefronts.
 extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const float *extra, int nthreads,
-                         int *stats, int change_budget, int global_state) {
+                         int *stats, int change_budget, int mode) {
+    const int global_state = mode & 1;
     fdprep::ParValues p;
     p.nx = pc->nx; p.ny = pc->ny; p.nz = pc->nz;
     p.h = pc->h; p.dq = pc->dq; p.df = pc->df; p.x0 = pc->x0; p.y0 = pc->y0; p.z0 = pc->z0;
@@ -101,6 +113,14 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     W.tout0 = tout0.data(); W.tout4 = tout4.data();
     W.list0 = l0.data(); W.list1 = l1.data(); W.list2 = l2.data(); W.stamp = stamp.data(); W.flag = flag.data();
     W.mstamp = nullptr; W.lastf = nullptr;
+    const long long nn = (long long)p.nx * p.ny * p.nz;
+    std::vector<unsigned> wst((size_t)nn, 0u);
+    std::vector<float> wt0((size_t)nn), wt4((size_t)nn);
+    std::vector<int> wl((size_t)nn), wtmp((size_t)nn), wrd((size_t)nn);
+    unsigned wround = 1;
+    W.wst = wst.data(); W.wt0 = wt0.data(); W.wt4 = wt4.data();
+    W.wcnt = wl.data(); W.wtmp = wtmp.data(); W.wready = wrd.data();
+    W.wcap = nn; W.wn = nn; W.wround = &wround;
     fdsweep::TableJob J;
     std::memset(&J, 0, sizeof J);
     J.nx = p.nx; J.ny = p.ny; J.nz = p.nz;
@@ -125,6 +145,7 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     }
     J.skip_ok = fdprep::skip_safe(p, pr, 1.0 / smax, 1.0 / smin) ? 1 : 0;
     J.change_budget = change_budget;
+    J.wave = (mode >> 1) & 3;
     std::vector<double> geo((size_t)fdsweep::geo_doubles(p.ny, p.nz));
     J.geo_fill = geo.data();
     if (global_state) run_threads<SharedGlobal>(nthreads, J, W);

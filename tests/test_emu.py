@@ -71,3 +71,39 @@ def test_schedules_exercised(emu):
     assert tot["defers"] > 0
     assert tot["rounds_change"] < tot["faces_change"] * 64
     assert tot1["fallbacks"] > tot["fallbacks"]
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_wavefront_passes_match_oracle(emu, seed):
+    """Passes run as one dataflow wavefront over the table (the first pass, or
+    every pass), with 1 to 24 host threads standing in for the lanes: the same
+    grid bit for bit, the same pass count, and the wavefront actually ran."""
+    par, v = make_case(seed + 500)
+    s = ob.slowness(v)
+    t_ref, st = ob.solve(par, s)
+    nthreads = [1, 2, 5, 8][seed % 4]
+    wave = [1, 3][seed % 2]
+    t, est = emu.solve(par, s, nthreads, change_budget=[0, 1, 8][seed % 3], wave=wave,
+                       global_state=bool(seed % 4 == 3))
+    assert est["error"] == 0
+    assert est["passes"] == st["passes"] and est["restarts"] == st["restarts"]
+    assert est["wave_passes"] == (st["passes"] if wave == 3 else 1)
+    assert int((t.view(np.uint32) != t_ref.view(np.uint32)).sum()) == 0
+
+
+def test_wavefront_carried_registers_exercised(emu):
+    """Across a handful of cases the wavefront meets nodes whose 2D trials read
+    the carried t[0]/t[4] of their sequential predecessor, resolves them by face
+    scans, and needs far fewer rounds than the face-by-face schedule has."""
+    tot = {}
+    for seed in range(700, 708):
+        par, v = make_case(seed)
+        s = ob.slowness(v)
+        _, est = emu.solve(par, s, 8, wave=3)
+        _, est0 = emu.solve(par, s, 8, wave=0, change_budget=0)
+        for k, x in est.items():
+            tot[k] = tot.get(k, 0) + x
+        tot["rounds_face"] = tot.get("rounds_face", 0) + est0["rounds_level"] + est0["rounds_change"]
+    assert tot["wave_carry"] > 0 and tot["wave_scans"] >= tot["wave_carry"]
+    assert tot["wave_seeds"] > 0
+    assert tot["rounds_wave"] < tot["rounds_face"]

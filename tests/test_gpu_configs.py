@@ -110,15 +110,23 @@ def test_c1_full_size(ctx):
 
 # ----------------------------------------------------------------------- C2
 
-def test_c2_full_launch(ctx, expected):
+@pytest.mark.parametrize("wave", [0, 3])
+def test_c2_full_launch(ctx, expected, wave):
+    """wave 0: every pass face by face (LEVEL/CHANGE); wave 3: every pass as a
+    wavefront (the other tests run the default, first passes as wavefronts)."""
     from fdtomoc_amd import fdgpu
     g, vp, vs, pars = cc.c2()
     mp, ms = ctx.model(vp), ctx.model(vs)
     specs = [fdgpu.TableSpec(p, mp if q % 2 == 0 else ms) for q, p in enumerate(pars)]
     assert len(specs) == 96
+    ctx.set_wave(wave)
     t0 = time.time()
-    ids = ctx.solve(specs, fdgpu.BATCHED)
-    print(f"C2: 96 tables in one launch, {time.time() - t0:.2f} s", flush=True)
+    try:
+        ids = ctx.solve(specs, fdgpu.BATCHED)
+    finally:
+        ctx.set_wave(1)
+    print(f"C2 (wave {wave}): 96 tables in one launch, {time.time() - t0:.2f} s", flush=True)
+    assert all(ctx.stats(i)["wave_passes"] == (ctx.stats(i)["passes"] if wave == 3 else 0) for i in ids)
     for q in cc.C2_CHECK:
         _check(expected, f"c2_t{q}", pars[q], vp if q % 2 == 0 else vs, ctx.read(ids[q], vp.shape),
                ctx.stats(ids[q]))
@@ -133,13 +141,21 @@ def test_c2_full_launch(ctx, expected):
 
 # ----------------------------------------------------------------------- C3
 
-def test_c3_all_windows(ctx, expected):
+@pytest.mark.parametrize("wave", [1, 3])
+def test_c3_all_windows(ctx, expected, wave):
+    """wave 1: first passes as wavefronts, reverse passes face by face; wave 3:
+    every pass as a wavefront."""
     from fdtomoc_amd import fdgpu
     G, v, wins, check = cc.c3()
     m = ctx.model(v)
+    ctx.set_wave(wave)
     t0 = time.time()
-    ids = ctx.solve([fdgpu.TableSpec(w.par, m, w.origin) for w in wins], fdgpu.BATCHED)
-    print(f"C3: 256 windows in one launch, {time.time() - t0:.2f} s", flush=True)
+    try:
+        ids = ctx.solve([fdgpu.TableSpec(w.par, m, w.origin) for w in wins], fdgpu.BATCHED)
+    finally:
+        ctx.set_wave(1)
+    print(f"C3 (wave {wave}): 256 windows in one launch, {time.time() - t0:.2f} s", flush=True)
+    assert all(ctx.stats(i)["wave_passes"] >= 1 for i in ids)
 
     def shape(w):
         return (w.par["nz"], w.par["ny"], w.par["nx"])

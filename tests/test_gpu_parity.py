@@ -39,7 +39,34 @@ def test_kernel_matches_oracle_batched(ctx, seed):
     print(f"seed {seed} shape {v.shape} oracle {st} gpu {gst}", flush=True)
     assert gst["error"] & ~1 == 0
     assert gst["passes"] == st["passes"]
+    assert gst["wave_passes"] == 1   # default: the first pass as a wavefront
     assert _bits_equal(t, t_ref) == 0
+
+
+@pytest.mark.parametrize("wave", [0, 3])
+def test_kernel_matches_oracle_wave_modes(ctx, wave):
+    """The 24 seeded option mixes again with every pass face by face (wave 0)
+    and with every pass as one dataflow wavefront (wave 3), all in one launch;
+    the default (first passes as wavefronts) runs in the test above."""
+    from fdtomoc_amd import fdgpu
+    cases = [make_case(seed) for seed in range(24)]
+    models = [ctx.model(v) for _, v in cases]
+    ctx.set_wave(wave)
+    try:
+        ids = ctx.solve([fdgpu.TableSpec(par_dict(par), m) for (par, _), m in zip(cases, models)], fdgpu.BATCHED)
+    finally:
+        ctx.set_wave(1)
+    for (par, v), tid in zip(cases, ids):
+        t_ref, st = ob.solve(par, ob.slowness(v))
+        gst = ctx.stats(tid)
+        t = ctx.read(tid, v.shape)
+        ctx.release(tid)
+        assert gst["error"] & ~1 == 0
+        assert gst["passes"] == st["passes"]
+        assert gst["wave_passes"] == (gst["passes"] if wave == 3 else 0)
+        assert _bits_equal(t, t_ref) == 0
+    for m in models:
+        ctx.release_model(m)
 
 
 def _golden_par(keys, vals):

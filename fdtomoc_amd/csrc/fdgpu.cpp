@@ -47,19 +47,20 @@ void dev_free(fdgpu_ctx *c, void *p, size_t bytes) {
 // reports it and the table is solved face by face instead.
 long long wave_list_cap(long long N) { return std::min(N, (long long)1 << 22); }
 
-size_t slot_bytes(long long F, long long N, int D) {
+size_t slot_bytes(long long F, long long N, int D, bool wave) {
     const size_t f = align256(sizeof(float) * (size_t)F);
     size_t b = 15 * f + align256((size_t)F);
     if (N > 0) {
         b += align256(sizeof(int) * (size_t)N) + align256(sizeof(int) * 6 * (size_t)D);
         // wavefront state: status, t[0], t[4], counter per node, two lists, the round word
-        b += 4 * align256(sizeof(int) * (size_t)N) + 2 * align256(sizeof(int) * (size_t)wave_list_cap(N)) + 256;
+        if (wave)
+            b += 4 * align256(sizeof(int) * (size_t)N) + 2 * align256(sizeof(int) * (size_t)wave_list_cap(N)) + 256;
     }
     return b;
 }
 
-int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long long N, int D) {
-    const size_t per = slot_bytes(F, N, D);
+int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long long N, int D, bool wave) {
+    const size_t per = slot_bytes(F, N, D, wave);
     const size_t f = align256(sizeof(float) * (size_t)F);
     std::vector<fdsweep::Scratch> h((size_t)ns);
     for (int s = 0; s < ns; s++) {
@@ -91,6 +92,8 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long 
         if (N > 0) {
             W.mstamp = (int *)take(align256(sizeof(int) * (size_t)N));
             W.lastf = (int *)take(align256(sizeof(int) * 6 * (size_t)D));
+        }
+        if (N > 0 && wave) {
             const size_t nb = align256(sizeof(int) * (size_t)N), lb = align256(sizeof(int) * (size_t)wave_list_cap(N));
             W.wst = (unsigned *)take(nb);
             W.wt0 = (float *)take(nb);
@@ -113,13 +116,21 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long 
 // change stamps of tables of up to maxN nodes (the stamps live only for the
 // duration of one table's solve, so they belong to the slot, not the table);
 // grown on demand, never shrunk.
-int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF, long long maxN, int maxD) {
-    if (nslots <= c->slots_cap && maxF <= c->slot_face_cap && maxN <= c->slot_node_cap && maxD <= c->slot_dim_cap)
-        return FDGPU_OK;
+int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF, long long maxN, int maxD, bool wave) {
+    const bool fits = nslots <= c->slots_cap && maxF <= c->slot_face_cap && maxN <= c->slot_node_cap &&
+                      maxD <= c->slot_dim_cap;
+    if (fits && (c->slot_wave || !wave)) return FDGPU_OK;
     const int ns = std::max(nslots, c->slots_cap);
     const long long F = std::max(maxF, c->slot_face_cap);
     const long long N = std::max(maxN, c->slot_node_cap);
     const int D = std::max(maxD, c->slot_dim_cap);
+    // wavefront state only where it fits the context's budget beside what is
+    // already allocated (the slots' own earlier allocation is released first)
+    if (wave) {
+        const size_t avail = c->budget - (c->used - c->scratch_bytes - sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
+        wave = slot_bytes(F, N, D, true) * (size_t)ns + sizeof(fdsweep::Scratch) * (size_t)ns <= avail;
+        if (!wave && fits) return FDGPU_OK;
+    }
     (void)hipStreamSynchronize(c->stream);
     clear_graphs(c);
     dev_free(c, c->d_scratch, c->scratch_bytes);
@@ -130,14 +141,15 @@ int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF, long long maxN, int
     c->slot_face_cap = 0;
     c->slot_node_cap = 0;
     c->slot_dim_cap = 0;
-    const size_t per = slot_bytes(F, N, D);
+    const size_t per = slot_bytes(F, N, D, wave);
     int rc = dev_alloc(c, (void **)&c->d_scratch, per * (size_t)ns);
     if (rc) return rc;
     c->scratch_bytes = per * (size_t)ns;
     rc = dev_alloc(c, (void **)&c->d_slots, sizeof(fdsweep::Scratch) * (size_t)ns);
     if (rc) return rc;
-    rc = fill_slots(c->d_scratch, ns, F, c->d_slots, N, D);
+    rc = fill_slots(c->d_scratch, ns, F, c->d_slots, N, D, wave);
     if (rc) return rc;
+    c->slot_wave = wave;
     c->slots_cap = ns;
     c->slot_face_cap = F;
     c->slot_node_cap = N;
@@ -325,7 +337,7 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
     }
     int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu[variant]) : 1;
     if (nslots < 1) nslots = 1;
-    int rc = ensure_scratch(c, nslots, maxF, maxN, maxD);
+    int rc = ensure_scratch(c, nslots, maxF, maxN, maxD, !no_wave && c->wave != 0);
     if (rc) return rc;
     rc = ensure_jobs(c, n);
     if (rc) return rc;
@@ -658,7 +670,7 @@ int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, 
     const size_t b_lf = align256(sizeof(int) * 6 * (size_t)maxdim);
     const size_t b_geo = align256(sizeof(double) * (size_t)fdsweep::geo_doubles(T.par.ny, T.par.nz));
     const size_t b_host = b_sin + b_idx + b_dist + b_val;
-    const size_t need = b_host + b_st + b_lf + b_geo + slot_bytes(maxF, 0, 0);
+    const size_t need = b_host + b_st + b_lf + b_geo + slot_bytes(maxF, 0, 0, false);
     if (need > FD_NOALLOC_SCRATCH_BYTES) return FDGPU_ESCRATCH;
     char *base = c->d_fixed;
     std::vector<char> aux(b_host, 0);

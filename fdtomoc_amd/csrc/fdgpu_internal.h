@@ -193,6 +193,7 @@ struct fdgpu_ctx {
     int slots_cap = 0;
     long long slot_face_cap = 0;
     long long slot_node_cap = 0;   // per-slot change stamps: nodes of the largest table
+    bool slot_wave = false;        // the slots hold wavefront state (sized by slot_node_cap)
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;
     fdsweep::TableJob *d_jobs = nullptr;   // descriptors of the current batch
@@ -229,12 +230,13 @@ void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
 // One slot's scratch: face arrays for faces of up to F nodes, plus change
 // stamps for tables of up to N nodes and last full-plane marks for dimensions
 // up to D (none when N == 0).
-size_t slot_bytes(long long F, long long N = 0, int D = 0);
+size_t slot_bytes(long long F, long long N = 0, int D = 0, bool wave = false);
 // Destroys the captured batched launches (their pointers are about to change).
 void clear_graphs(fdgpu_ctx *c);
 // Carves `ns` scratch slots for faces of up to F nodes out of `base` and
 // uploads their descriptors to `d_slots`.
-int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long long N = 0, int D = 0);
+int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long long N = 0, int D = 0,
+               bool wave = false);
 fdprep::ParValues to_values(const fdgpu_par &p);
 // Frees every cached geometry table (stream idle).
 void clear_geos(fdgpu_ctx *c);

@@ -3,10 +3,15 @@
 //   fdgpu_run_spec(spec)  = body of main()                src/sphfd.c:200-257
 // The per-table work of the reference (par intake, velocity read, solve,
 // table write) is kept; the OpenMP loop over tables becomes one batched launch
-// per distinct velocity input.
+// per distinct velocity input.  Concurrent sphfd() calls (the reference's
+// OpenMP threads, sphfd.c:247-255) are gathered into shared launches the same
+// way: the first caller to find no launch running leads, waits briefly for
+// the others, and solves every call queued by then.
 #include <stdio.h>
 #include <string.h>
 
+#include <chrono>
+#include <condition_variable>
 #include <map>
 #include <mutex>
 #include <string>
@@ -72,6 +77,88 @@ int load_table(int ac, char **av, const char *output_path, Loaded &L) {
 std::mutex g_mu;
 fdgpu_ctx *g_ctx = nullptr;
 
+int shared_ctx(fdgpu_ctx **c);
+
+// Gathering of concurrent sphfd() calls.
+struct Call {
+    Loaded *L;
+    int rc = 0;
+    bool done = false;
+};
+struct Gather {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Call *> queue;   // calls not yet taken by a leader
+    bool leading = false;        // a leader is gathering or solving
+    unsigned long long arrivals = 0;
+    unsigned long long calls = 0, launches = 0;
+};
+Gather g_gather;
+
+int solve_group(fdgpu_ctx *c, std::vector<Loaded *> &grp, int mode);
+
+// Solves the calls of one gathering, one batched launch per velocity input
+// (as fdgpu_run_spec groups parlist lines), and sets each call's result.
+void solve_calls(std::vector<Call *> &calls) {
+    fdgpu_ctx *c = nullptr;
+    int rc = shared_ctx(&c);
+    std::map<std::tuple<std::string, int, int, int, int>, std::vector<Call *>> groups;
+    for (Call *k : calls) {
+        const fdprep::ParValues &p = k->L->p;
+        groups[std::make_tuple(p.velfile, p.nx, p.ny, p.nz, p.swab)].push_back(k);
+    }
+    unsigned long long nl = 0;
+    for (auto &kv : groups) {
+        int grc = rc;
+        if (!grc) {
+            std::vector<Loaded *> grp;
+            for (Call *k : kv.second) grp.push_back(k->L);
+            grc = solve_group(c, grp, FDGPU_BATCHED);
+            nl++;
+        }
+        for (Call *k : kv.second) k->rc = grc;
+    }
+    std::lock_guard<std::mutex> lk(g_gather.mu);
+    g_gather.launches += nl;
+}
+
+// Queues one loaded call and returns its result once a leader solved it.
+int gathered_call(Loaded &L) {
+    Call me;
+    me.L = &L;
+    std::unique_lock<std::mutex> lk(g_gather.mu);
+    g_gather.queue.push_back(&me);
+    g_gather.arrivals++;
+    g_gather.calls++;
+    g_gather.cv.notify_all();
+    while (!me.done) {
+        if (!g_gather.leading) {
+            // lead: let callers that start together arrive (until none came
+            // for 1 ms, at most 5 ms), then take everything queued
+            g_gather.leading = true;
+            const auto t0 = std::chrono::steady_clock::now();
+            for (;;) {
+                const unsigned long long seen = g_gather.arrivals;
+                g_gather.cv.wait_for(lk, std::chrono::milliseconds(1));
+                if (g_gather.arrivals == seen ||
+                    std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(5))
+                    break;
+            }
+            std::vector<Call *> batch;
+            batch.swap(g_gather.queue);
+            lk.unlock();
+            solve_calls(batch);
+            lk.lock();
+            for (Call *k : batch) k->done = true;
+            g_gather.leading = false;
+            g_gather.cv.notify_all();
+        } else {
+            g_gather.cv.wait(lk);
+        }
+    }
+    return me.rc;
+}
+
 int shared_ctx(fdgpu_ctx **c) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_ctx) {
@@ -130,11 +217,15 @@ extern "C" int sphfd(int ac, char **av, char *output_path) {
         fprintf(stderr, "sphfd: %s\n", fdgpu_strerror(rc));
         return rc;
     }
-    fdgpu_ctx *c = nullptr;
-    rc = shared_ctx(&c);
-    if (rc) return rc;
-    std::vector<Loaded *> grp{&L};
-    return solve_group(c, grp, FDGPU_PLAIN);
+    return gathered_call(L);
+}
+
+extern "C" int fdgpu_sphfd_info(unsigned long long *info, int ninfo) {
+    if (!info || ninfo < 0) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(g_gather.mu);
+    const unsigned long long v[2] = {g_gather.calls, g_gather.launches};
+    for (int q = 0; q < ninfo && q < 2; q++) info[q] = v[q];
+    return FDGPU_OK;
 }
 
 extern "C" int fdgpu_run_spec(const char *spec_path) {

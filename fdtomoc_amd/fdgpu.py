@@ -277,6 +277,7 @@ def lib():
         L.fdgpu_serve_info.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.fdgpu_serve_stop.argtypes = [C.c_void_p]
         L.sphfd.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_char_p]
+        L.fdgpu_sphfd_info.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -587,3 +588,22 @@ class Context:
 def run_spec(spec_path: str) -> None:
     """The drop-in `sphfd` run of a spec file (src/sphfd.c:200-257), in process."""
     _check(lib().fdgpu_run_spec(spec_path.encode()), "fdgpu_run_spec")
+
+
+def sphfd(parline: str, output_path: str) -> int:
+    """The reference's in-process entry point `sphfd(2, {"sphfd", parline},
+    output_path)` (src/sphfd.c:260): solves one parlist line and writes its
+    table under output_path.  Returns the C return code (0 or FDGPU_E*).  The
+    ctypes call releases the GIL, so calls from several Python threads run
+    concurrently and are gathered into shared launches, as OpenMP threads'
+    calls are."""
+    av = (C.c_char_p * 2)(b"sphfd", parline.encode())
+    return int(lib().sphfd(2, av, output_path.encode()))
+
+
+def sphfd_info() -> dict:
+    """Counters of the in-process entry point: calls and the launches they
+    were solved in."""
+    info = (C.c_ulonglong * 2)()
+    _check(lib().fdgpu_sphfd_info(info, 2), "fdgpu_sphfd_info")
+    return {"calls": int(info[0]), "launches": int(info[1])}

@@ -345,8 +345,15 @@ int fdgpu_serve_stop(fdgpu_ctx *ctx);
 
 /* Drop-in of src/sphfd.c:260: av[1] is one parlist line ("par=STA.par").
  * Solves on device 0 and writes output_path/timefile.  Returns 0 or a
- * negative FDGPU_E* code (the reference aborts instead). */
+ * negative FDGPU_E* code (the reference aborts instead).  Thread-safe:
+ * calls made concurrently (as the reference's OpenMP loop makes them,
+ * sphfd.c:247-255) are gathered into one batched launch per velocity input
+ * -- the first caller to find no launch running waits until no other call
+ * arrived for 1 ms (at most 5 ms) and solves every call queued by then. */
 int sphfd(int ac, char **av, char *output_path);
+/* Counters of the in-process entry point (no reference counterpart):
+ * info[0] sphfd() calls, info[1] launches they were solved in. */
+int fdgpu_sphfd_info(unsigned long long *info, int ninfo);
 
 /* The `sphfd` executable's body (src/sphfd.c:200-257) with all tables of the
  * parlist solved in one batched launch per velocity file.  Reads the spec path

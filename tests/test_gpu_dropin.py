@@ -195,3 +195,49 @@ def test_location_over_reference_table_files(workdirs):
             ctx.release(i)
     finally:
         ctx.close()
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference executable not built")
+def test_inprocess_sphfd_concurrent_callers(workdirs):
+    """The in-process entry point `int sphfd(int, char**, char*)` called from
+    eight host threads at once, as the reference's OpenMP loop calls it
+    (sphfd.c:247-255): every table file equals the reference executable's
+    byte for byte, and the concurrent calls were gathered into fewer
+    launches than calls."""
+    import concurrent.futures as cf
+    ref_wd, gpu_wd = workdirs
+    g, vp = _model(25, 23, 15, 1.0, 11)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    head = tables.make_header(g.nx, g.ny, g.nz, g.x0, g.y0, g.z0, g.df_deg(), g.dq_deg(), g.h)
+    hbytes = np.asarray(head).tobytes()
+    files = {"VP.mod": hbytes + vp.astype("<f4").tobytes(), "VS.mod": hbytes + vs.astype("<f4").tobytes()}
+    rng = np.random.default_rng(12)
+    pars = []
+    for t in range(16):
+        st = synth.surface_node_station(g, rng, margin=2)
+        phase = "P" if t % 2 == 0 else "S"
+        pars.append(_par(g, st, f"timefile=s{t}.{phase.lower()}times\nvelfile=V{phase}.mod\n"))
+    _write_inputs(ref_wd, files, pars)
+    _write_inputs(gpu_wd, files, pars)
+    r = _run(REF, ref_wd)
+    assert r.returncode == 0, r.stderr[-2000:]
+    before = fdgpu.sphfd_info()
+    cwd = os.getcwd()
+    os.chdir(gpu_wd)   # par files and velocity files are named relative to the run directory
+    try:
+        with cf.ThreadPoolExecutor(max_workers=8) as ex:
+            rcs = list(ex.map(lambda t: fdgpu.sphfd(f"par=t{t}.par", "out"), range(16)))
+    finally:
+        os.chdir(cwd)
+    after = fdgpu.sphfd_info()
+    assert rcs == [0] * 16
+    calls, launches = after["calls"] - before["calls"], after["launches"] - before["launches"]
+    print(f"16 concurrent sphfd() calls from 8 threads: {launches} launches", flush=True)
+    assert calls == 16 and launches < calls
+    for t in range(16):
+        name = f"s{t}.{'p' if t % 2 == 0 else 's'}times"
+        with open(os.path.join(ref_wd, "out", name), "rb") as f:
+            a = f.read()
+        with open(os.path.join(gpu_wd, "out", name), "rb") as f:
+            b = f.read()
+        assert a == b, name

@@ -628,7 +628,16 @@ def run_c5(args):
             kern = 0.0
             for c0 in range(0, ntab, chunk):
                 if it == 0:
-                    ids += ctx.solve(specs[c0:c0 + chunk], fdgpu.BATCHED)
+                    try:
+                        ids += ctx.solve(specs[c0:c0 + chunk], fdgpu.BATCHED)
+                    except fdgpu.FdgpuError as e:
+                        if e.ids:
+                            bad = [(q, ctx.stats(q)) for q in e.ids if ctx.stats(q)["error"] & ~1]
+                            print(f"# {len(bad)} tables with error bits: " + "; ".join(
+                                f"table {q}: error {st['error']} passes {st['passes']} wave {st['wave_passes']} "
+                                f"level {st['faces_level']} change {st['faces_change']}" for q, st in bad[:8]),
+                                file=sys.stderr, flush=True)
+                        raise
                 else:
                     if any(c0 <= k < c0 + chunk for k in pending):
                         ctx.write_wait()   # the files of these tables are written first

@@ -42,6 +42,35 @@ void dev_free(fdgpu_ctx *c, void *p, size_t bytes) {
     c->used -= bytes;
 }
 
+// Releases the launch slots (regrown by the next launch).
+void release_scratch(fdgpu_ctx *c) {
+    if (!c->d_scratch && !c->d_slots) return;
+    (void)hipStreamSynchronize(c->stream);
+    clear_graphs(c);
+    dev_free(c, c->d_scratch, c->scratch_bytes);
+    dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
+    c->d_scratch = nullptr;
+    c->d_slots = nullptr;
+    c->scratch_bytes = 0;
+    c->slots_cap = 0;
+    c->slot_face_cap = 0;
+    c->slot_node_cap = 0;
+    c->slot_dim_cap = 0;
+    c->slot_wave = false;
+}
+
+// Memory that outlives a launch (tables, models): when the budget is short,
+// the launch slots are released first; the next launch regrows them within
+// what is left.
+int dev_alloc_resident(fdgpu_ctx *c, void **p, size_t bytes) {
+    int rc = dev_alloc(c, p, bytes);
+    if (rc == FDGPU_ENOMEM && c->d_scratch) {
+        release_scratch(c);
+        rc = dev_alloc(c, p, bytes);
+    }
+    return rc;
+}
+
 // Capacity of each per-round list of the wavefront passes: a round queues a
 // surface of the table, far below this; a pass that would overflow it
 // reports it and the table is solved face by face instead.
@@ -124,23 +153,16 @@ int ensure_scratch(fdgpu_ctx *c, int nslots, long long maxF, long long maxN, int
     const long long F = std::max(maxF, c->slot_face_cap);
     const long long N = std::max(maxN, c->slot_node_cap);
     const int D = std::max(maxD, c->slot_dim_cap);
-    // wavefront state only where it fits the context's budget beside what is
-    // already allocated (the slots' own earlier allocation is released first)
+    // wavefront state only where it takes at most half of what the budget
+    // has left beside everything else (the slots' own earlier allocation is
+    // released first), so that it does not crowd out the tables a caller
+    // keeps resident
     if (wave) {
         const size_t avail = c->budget - (c->used - c->scratch_bytes - sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
-        wave = slot_bytes(F, N, D, true) * (size_t)ns + sizeof(fdsweep::Scratch) * (size_t)ns <= avail;
+        wave = slot_bytes(F, N, D, true) * (size_t)ns + sizeof(fdsweep::Scratch) * (size_t)ns <= avail / 2;
         if (!wave && fits) return FDGPU_OK;
     }
-    (void)hipStreamSynchronize(c->stream);
-    clear_graphs(c);
-    dev_free(c, c->d_scratch, c->scratch_bytes);
-    dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
-    c->d_scratch = nullptr;
-    c->d_slots = nullptr;
-    c->slots_cap = 0;
-    c->slot_face_cap = 0;
-    c->slot_node_cap = 0;
-    c->slot_dim_cap = 0;
+    release_scratch(c);
     const size_t per = slot_bytes(F, N, D, wave);
     int rc = dev_alloc(c, (void **)&c->d_scratch, per * (size_t)ns);
     if (rc) return rc;
@@ -489,7 +511,7 @@ int fdgpu_model_create(fdgpu_ctx *c, int nx, int ny, int nz, const float *vel, i
     m.nx = nx; m.ny = ny; m.nz = nz;
     const long long n = (long long)nx * ny * nz;
     m.bytes = sizeof(float) * (size_t)n;
-    int rc = dev_alloc(c, (void **)&m.d_slow, m.bytes);
+    int rc = dev_alloc_resident(c, (void **)&m.d_slow, m.bytes);
     if (rc) return rc;
     const int id = c->next_id++;
     c->models[id] = m;
@@ -554,9 +576,9 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         const size_t b_val = align256(sizeof(float) * T.prep.init_val.size());
         const size_t b_st = align256(sizeof(int) * fdsweep::kStatCount);
         T.bytes = b_sin + b_idx + b_dist + b_val + b_st;
-        rc = dev_alloc(c, (void **)&T.d_time, sizeof(float) * (size_t)T.n);
+        rc = dev_alloc_resident(c, (void **)&T.d_time, sizeof(float) * (size_t)T.n);
         if (rc) break;
-        rc = dev_alloc(c, (void **)&T.d_aux, T.bytes);
+        rc = dev_alloc_resident(c, (void **)&T.d_aux, T.bytes);
         if (rc) { dev_free(c, T.d_time, sizeof(float) * (size_t)T.n); T.d_time = nullptr; break; }
         made++;
         std::vector<char> aux(T.bytes, 0);

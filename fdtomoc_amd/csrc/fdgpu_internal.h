@@ -227,6 +227,8 @@ bool writer_busy(fdgpu_ctx *c, int table);
 void writer_shutdown(fdgpu_ctx *c);
 int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes);
 void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
+int dev_alloc_resident(fdgpu_ctx *c, void **p, size_t bytes);
+void release_scratch(fdgpu_ctx *c);
 // One slot's scratch: face arrays for faces of up to F nodes, plus change
 // stamps for tables of up to N nodes and last full-plane marks for dimensions
 // up to D (none when N == 0).

@@ -55,7 +55,7 @@ extern "C" int fdgpu_table_load(fdgpu_ctx *c, const char *path, int nx, int ny, 
     T.par.nz = nz;
     T.n = n;
     T.from_file = true;
-    int rc = dev_alloc(c, (void **)&T.d_time, sizeof(float) * (size_t)n);
+    int rc = dev_alloc_resident(c, (void **)&T.d_time, sizeof(float) * (size_t)n);
     if (rc) return rc;
     if (hipMemcpyAsync(T.d_time, host.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream) !=
             hipSuccess ||

@@ -205,9 +205,10 @@ class FdgpuError(RuntimeError):
     def __init__(self, msg: str, code: int = 0):
         super().__init__(msg)
         self.code = code
+        self.ids = None   # EKERNEL from a solve: the tables stay registered (their stats name the error)
 
 
-EBUSY, ETIMEOUT, ESCRATCH = -10, -11, -12
+EKERNEL, EBUSY, ETIMEOUT, ESCRATCH = -9, -10, -11, -12
 SERVE_INFO = ["on", "outstanding", "arena_used", "arena_bytes", "launches", "workgroups"]
 
 
@@ -442,7 +443,12 @@ class Context:
         for t, spec in enumerate(tables):
             self._fill(arr[t], spec, keep)
         ids = (C.c_int * max(n, 1))()
-        _check(lib().fdgpu_solve(self._h, arr, n, mode, ids), "fdgpu_solve")
+        rc = lib().fdgpu_solve(self._h, arr, n, mode, ids)
+        if rc == EKERNEL:
+            e = FdgpuError(f"fdgpu_solve: {lib().fdgpu_strerror(rc).decode()} ({rc})", rc)
+            e.ids = list(ids[:n])
+            raise e
+        _check(rc, "fdgpu_solve")
         return list(ids[:n])
 
     def resolve(self, tables: list[int], mode: int = BATCHED):

@@ -147,6 +147,9 @@ def lib():
         L.oracle_loc_find_time.argtypes = [C.POINTER(LocGrid), C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.oracle_locate.argtypes = [C.POINTER(LocGrid), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_char_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(LocOut)]
+        L.oracle_ray_trace_many.argtypes = [C.POINTER(RayGridC), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.oracle_ray_trace_many.restype = C.c_int
         _lib = L
     return _lib
 
@@ -301,3 +304,58 @@ def locate(g: LocGrid, tables, obstime, pwt, phase: str, isgood) -> dict:
     r["node"] = tuple(out.node)
     r.update(isgood=good, res=res, tps=tps)
     return r
+
+
+# ------------------------------------------------------------------ rays
+
+class RayGridC(C.Structure):
+    """oracle_ray_grid (oracle/ray_oracle.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("x0", C.c_double), ("y0", C.c_double),
+                ("z0", C.c_double), ("df", C.c_double), ("dq", C.c_double), ("h", C.c_double), ("nxc", C.c_int),
+                ("nyc", C.c_int), ("nzc", C.c_int), ("gx", C.c_void_p), ("gy", C.c_void_p), ("gz", C.c_void_p),
+                ("ido1d", C.c_int), ("max_segments", C.c_int)]
+
+
+class RayInC(C.Structure):
+    _fields_ = [("ex", C.c_double), ("ey", C.c_double), ("ez", C.c_double), ("xs", C.c_double),
+                ("ys", C.c_double), ("zs", C.c_double)]
+
+
+class RayOutC(C.Structure):
+    _fields_ = [("status", C.c_int), ("nseg", C.c_int), ("ndu", C.c_int), ("first_in_cube", C.c_int),
+                ("tev", C.c_double), ("dtdx", C.c_float), ("dtdy", C.c_float), ("dtdz", C.c_float),
+                ("ai", C.c_float), ("az", C.c_float), ("rdev", C.c_float)]
+
+
+RAY_FIELDS = [name for name, _ in RayOutC._fields_]
+
+
+def ray_trace(grid, tables, rays, cap: int = 4096, nthreads: int = 8) -> list:
+    """The oracle's rays (oracle/ray_oracle.c).  grid: a
+    fdtomoc_amd.rayderv.RayGrid; tables[r]: ray r's table (nz, ny, nx);
+    rays: (ex, ey, ez, xs, ys, zs) tuples.  Returns one dict per ray with the
+    output fields and its derivatives (idx, val) in index order."""
+    gx, gy, gz = (np.ascontiguousarray(a, dtype=np.float32) for a in (grid.gx, grid.gy, grid.gz))
+    g = RayGridC(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.df, grid.dq, grid.h, len(gx), len(gy),
+                 len(gz), gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, grid.ido1d, grid.segment_limit())
+    ts = [np.ascontiguousarray(t, dtype=np.float32) for t in tables]
+    n = len(rays)
+    ptrs = (C.c_void_p * max(n, 1))(*[t.ctypes.data for t in ts])
+    ins = (RayInC * max(n, 1))(*[RayInC(*map(float, r)) for r in rays])
+    outs = (RayOutC * max(n, 1))()
+    while True:
+        idx = np.zeros(max(n, 1) * cap, dtype=np.int32)
+        val = np.zeros(max(n, 1) * cap, dtype=np.float32)
+        over = lib().oracle_ray_trace_many(C.byref(g), ptrs, ins, n, outs, idx.ctypes.data, val.ctypes.data, cap,
+                                           nthreads)
+        if over == 0:
+            break
+        cap = max(o.ndu for o in outs[:n])
+    res = []
+    for r in range(n):
+        d = {f: getattr(outs[r], f) for f in RAY_FIELDS}
+        k = outs[r].ndu if outs[r].status == 1 else 0
+        d["idx"] = idx[r * cap:r * cap + k].copy()
+        d["val"] = val[r * cap:r * cap + k].copy()
+        res.append(d)
+    return res

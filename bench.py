@@ -43,6 +43,11 @@ Other workloads (one JSON line each, same keys where they apply):
                   events of 24-40 picks over the 96 resident tables of the
                   default workload, all events per step, the oracle's location
                   timed on one core on a sample of the events
+  --workload rays sphrayderv's ray tracing (the row after loc): 1000 events x
+                  48 stations x (P, S) = 96,000 rays through the 96 resident
+                  tables of 161x161x81, derivatives on a 33x33x17 coarse grid,
+                  all rays per step; the ray oracle timed on the host's cores
+                  on a sample of the rays
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
     python bench.py --calibrate-copy MIB     # one known-size copy, for HBM counter scaling
@@ -684,6 +689,7 @@ def run_c5(args):
         out["iterations"] = iters
         out["run_s"] = round(total, 1)
         out["resident_table_bytes"] = 4 * g.n * ntab
+        out["tables_solved_again"] = ctx.resolved_again()
         out["sample_files_written_ok"] = bool(files_ok)
         last = iters[-1]
         out["roofline"] = roofline(int(ntab * g.n * (1 + last["reverse_passes_mean"])), last["kernel_s"])
@@ -914,6 +920,139 @@ def run_loc(args):
     return out
 
 
+# ------------------------------------------------------------------ rays
+
+# Per segment, the eight corner times of the station's table a step reads
+# (the algorithmic bytes of the ray kernel; derivatives add 8 bytes each).
+RAY_BYTES_PER_SEGMENT = 32
+
+
+def ray_workload(events: int, seed: int = 9):
+    """The 96 tables of 161 x 161 x 81 (48 stations, P and S, the c2 models),
+    sphrayderv's grid over them with a 33 x 33 x 17 coarse grid (every fifth
+    node), and `events` events at random positions 2 cells inside the sides
+    and below 2 km, each with P and S picks at every station."""
+    from fdtomoc_amd import rayderv
+    g, vp, vs, stations = workloads.c2()
+    ig = lambda n: np.full(n - 1, 5, dtype=np.int64)  # noqa: E731
+    grid = rayderv.grid_from_spec(33, 33, 17, ig(33), ig(33), ig(17), g.x0, g.y0, g.z0, g.h)
+    assert (grid.nx, grid.ny, grid.nz) == (g.nx, g.ny, g.nz)
+    sta = [rayderv.station(st.name, st.lat, st.lon, int(round(-st.depth * 1000))) for st in stations]
+    rng = np.random.default_rng(seed)
+    evs = []
+    for _ in range(events):
+        fi, fj, fk = rng.uniform(2, g.nx - 3), rng.uniform(2, g.ny - 3), rng.uniform(2, g.nz - 3)
+        evs.append((grid.x0 + fi * grid.df, grid.y0 + fj * grid.dq, grid.z0 + fk * grid.h))
+    return g, vp, vs, stations, grid, sta, evs
+
+
+def run_rays(args):
+    """sphrayderv's rays (src/sphrayderv.c:1306-2250) through resident tables:
+    the 96 tables solved once, untimed; a step traces every pick of every
+    event (fdgpu_rays_trace: host set-up, the launches, the derivatives back
+    in host memory)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    g, vp, vs, stations, grid, sta, evs = ray_workload(args.ray_events)
+    ctx = fdgpu.Context(0)
+    mp, ms = ctx.model(vp), ctx.model(vs)
+    specs = []
+    for st in stations:
+        for m in (mp, ms):
+            specs.append(fdgpu.TableSpec(workloads.table_par(g, st), m))
+    ids = ctx.solve(specs, fdgpu.BATCHED)
+    keys, coords = [], []
+    for e in evs:
+        for q, s in enumerate(sta):
+            for ph in range(2):
+                coords.append(e + (s.lon, s.colat, s.z))
+                keys.append(2 * q + ph)
+    keys = np.array(keys)
+    # the rays as fdgpu_ray records, built once (a C caller hands its own)
+    rays = fdgpu.ray_array(np.asarray(ids)[keys], coords)
+    for _ in range(args.warmup):
+        res, idx, val = ctx.trace_rays(grid, rays)
+    walls, kms = [], []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        res, idx, val = ctx.trace_rays(grid, rays)
+        walls.append(time.perf_counter() - t0)
+        kms.append(ctx.last_kernel_ms())
+        print(f"# step {s}: {walls[-1] * 1e3:.1f} ms wall, {kms[-1]:.1f} ms kernels", file=sys.stderr, flush=True)
+    wall, kern = float(np.mean(walls)), float(np.mean(kms)) * 1e-3
+    nok = int((res["status"] == 1).sum())
+    nseg = int(res["nseg"].sum())
+    alg = RAY_BYTES_PER_SEGMENT * nseg + 8 * len(idx)
+    ach = alg / kern / 1e9
+    out = {
+        "metric": METRIC + " | sphrayderv rays/s",
+        "value": round(len(rays) / wall, 1),
+        "unit": "rays traced/s (tables resident, derivatives in host memory)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64/f32",
+        "data": "synthetic (events at random positions in the c2 models' 96 tables, P and S at 48 stations)",
+        "config": {"workload": f"rays: {len(evs)} events x 48 stations x (P, S) = {len(rays)} rays through 96 "
+                               f"resident tables of 161x161x81, derivatives on a 33x33x17 coarse grid (18,513 "
+                               f"variables); timed: the fdgpu_rays_trace call",
+                   "rays": len(rays), "segments": nseg, "derivatives": int(len(idx))},
+        "kernel_ms_per_step": round(kern * 1e3, 3),
+        "rays_reaching_station": nok,
+        "status_counts": {int(k): int(v) for k, v in zip(*np.unique(res["status"], return_counts=True))},
+        "segments_per_ray": {"mean": round(float(res["nseg"].mean()), 1), "p99": int(np.percentile(res["nseg"], 99)),
+                             "max": int(res["nseg"].max())},
+        "segments_per_s": round(nseg / kern, 1),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "ray_kernel",
+                     "kernel_ms": round(kern * 1e3, 3), "algorithmic_bytes": alg,
+                     "what": "per step: 32 bytes of table corners per segment + 8 bytes per derivative; the walk "
+                             "is a chain of dependent FP64 steps per lane, so latency, not bandwidth, bounds it"},
+    }
+    out["cpu_baseline"] = None
+    if not args.no_cpu:
+        import oracle_bind as ob
+        cores = host_cores(args.cpu_threads)
+        host = [ctx.read(i, (g.nz, g.ny, g.nx)) for i in ids]
+        n = max(1, min(args.ray_cpu_rays, len(rays)))
+        pick = np.linspace(0, len(rays) - 1, n).astype(int)
+        t0 = time.perf_counter()
+        ref = ob.ray_trace(grid, [host[keys[r]] for r in pick], [coords[r] for r in pick], nthreads=cores)
+        rate = n / (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": round(rate, 2), "unit": "rays traced/s", "cores": cores, "kind": "port",
+                               "host": host_info(),
+                               "sample": f"{n} of the {len(rays)} rays, evenly spaced: the ray oracle "
+                                         f"(oracle/ray_oracle.c, the reference's loop in C, equal to the "
+                                         f"reference program's outputs) on {cores} host threads"}
+        out["gpu_over_cpu"] = round(out["value"] / rate, 1)
+        same = exact = 0
+        worst = 0.0
+        for r, o in zip(pick, ref):
+            d = res[r]
+            if d["status"] != o["status"] or d["nseg"] != o["nseg"]:
+                continue
+            same += 1
+            if d["status"] == 1:
+                gv = val[d["offset"]:d["offset"] + d["ndu"]]
+                gi = idx[d["offset"]:d["offset"] + d["ndu"]]
+                if np.array_equal(gi, o["idx"]):
+                    exact += bool(np.array_equal(gv.view(np.uint32), o["val"].view(np.uint32)))
+                    worst = max(worst, float(np.max(np.abs(gv - o["val"]) / np.abs(o["val"]))) if len(gv) else 0.0)
+                else:
+                    worst = float("inf")
+            else:
+                exact += 1
+        out["sample_vs_oracle"] = {"rays": n, "same_status_and_segments": same, "bit_identical": exact,
+                                   "max_rel_derivative_diff": worst,
+                                   "what": "same status and segment count, and derivatives bit for bit; the "
+                                           "device's sines, cosines and arctangents are correctly rounded, the "
+                                           "host C library's not in about one call in a thousand"}
+    for i in ids:
+        ctx.release(i)
+    ctx.release_model(mp)
+    ctx.release_model(ms)
+    ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ calibration
 
 def calibrate_copy(mib: int) -> None:
@@ -935,7 +1074,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2w", "c3", "c4", "c5", "c2f", "loc"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c2w", "c3", "c4", "c5", "c2f", "loc", "rays"])
     ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="tables for the CPU baseline (-1: one per core)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -950,6 +1089,8 @@ def main():
     ap.add_argument("--iterations", type=int, default=5, help="c5: inversion iterations (the first solves)")
     ap.add_argument("--events", type=int, default=256, help="loc: events per step")
     ap.add_argument("--loc-cpu-events", type=int, default=8, help="loc: events in the CPU baseline sample")
+    ap.add_argument("--ray-events", type=int, default=1000, help="rays: events per step (96 rays each)")
+    ap.add_argument("--ray-cpu-rays", type=int, default=4000, help="rays: rays in the CPU baseline sample")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -959,7 +1100,8 @@ def main():
         print("bench.py: measures one GPU; the station tables shard across GPUs with no "
               "communication on the hot path, so multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
-    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f, "loc": run_loc, "c2w": run_c2w}[args.workload]
+    run = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5, "c2f": run_c2f, "loc": run_loc, "c2w": run_c2w,
+           "rays": run_rays}[args.workload]
     print(json.dumps(run(args)), flush=True)
 
 

@@ -2,8 +2,9 @@
 
     python -m fdtomoc_amd.build
 
-produces ``fdtomoc_amd/libfdtomoc_amd.so`` (gfx950 kernels -- the sphfd sweep
-and c2f's coarse-to-fine interpolation -- + C ABI of include/fdtomoc_amd.h) and ``fdtomoc_amd/bin/sphfd`` (the drop-in
+produces ``fdtomoc_amd/libfdtomoc_amd.so`` (gfx950 kernels -- the sphfd sweep,
+c2f's coarse-to-fine interpolation, sphfdloc's location and sphrayderv's ray
+tracing -- + C ABI of include/fdtomoc_amd.h) and ``fdtomoc_amd/bin/sphfd`` (the drop-in
 executable).  All code is compiled without floating-point contraction: the
 FP64 stencils must round like the reference's x86-64 build.
 """
@@ -31,7 +32,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [("sweep_kernel.hip", True), ("coarse_kernel.hip", True), ("fdgpu.cpp", False), ("sphfd_io.cpp", False),
            ("table_prep.cpp", False), ("sphfd_drop.cpp", False), ("serve.cpp", False), ("coarse.cpp", False),
            ("loc_kernel.hip", True), ("loc.cpp", False), ("table_io.cpp", False),
-           ("table_writer.cpp", False), ("loc_geo.cpp", False)]
+           ("table_writer.cpp", False), ("loc_geo.cpp", False), ("ray_kernel.hip", True), ("ray.cpp", False)]
 
 
 def _run(cmd, verbose):

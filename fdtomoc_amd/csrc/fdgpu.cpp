@@ -390,9 +390,18 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
         T.stats.assign(fdsweep::kStatCount, 0);
         FD_HIP(hipMemcpy(T.stats.data(), T.d_stats, sizeof(int) * fdsweep::kStatCount, hipMemcpyDeviceToHost));
         // bit 7: a wavefront pass could not finish (list capacity, no progress);
-        // the table is solved again face by face
-        if (!no_wave && (T.stats[fdsweep::kStatError] & 128)) again.push_back(&T);
-        else if (T.stats[fdsweep::kStatError] & ~1) kerr = 1;   // bit0 (NaN keys) is reported, not fatal
+        // bits 2-5: one of the kernel's consistency checks failed (a LEVEL
+        // face left nodes unprocessed or ran out of rounds, a CHANGE face ran
+        // out of rounds, a skipped node did not reproduce its value).  The
+        // table is solved again face by face, once; a second failure is an
+        // error.  bit0 (NaN keys) is reported, not fatal.  Tables whose
+        // initial grid the host supplied (srctype 2/3) cannot run again.
+        if (!no_wave && (T.stats[fdsweep::kStatError] & ~1) && T.job.init_mode != 1) {
+            again.push_back(&T);
+            if (T.stats[fdsweep::kStatError] & ~(1 | 128)) c->resolved_again++;
+        } else if (T.stats[fdsweep::kStatError] & ~1) {
+            kerr = 1;
+        }
     }
     if (!again.empty()) {
         const float ms0 = c->last_ms;
@@ -438,6 +447,7 @@ const char *fdgpu_strerror(int code) {
     case FDGPU_EBUSY: return "busy: serving arena, staging or queue full, or call not allowed while serving";
     case FDGPU_ETIMEOUT: return "timed out waiting for a request";
     case FDGPU_ESCRATCH: return "table too large for the fixed device scratch of fdgpu_solve_noalloc";
+    case FDGPU_ESPACE: return "derivative arrays too small for the traced rays";
     }
     return "unknown error";
 }
@@ -648,6 +658,13 @@ int fdgpu_solve(fdgpu_ctx *c, const fdgpu_table_desc *desc, int n, int mode, int
         c->tables[id] = std::move(tabs[(size_t)t]);
     }
     return rc;
+}
+
+int fdgpu_context_counters(fdgpu_ctx *c, long long *out, int n) {
+    if (!c || n < 0 || (n > 0 && !out)) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n > 0) out[0] = c->resolved_again;
+    return FDGPU_OK;
 }
 
 int fdgpu_resolve(fdgpu_ctx *c, const int *ids, int n, int mode) {

@@ -193,6 +193,7 @@ struct fdgpu_ctx {
     int slots_cap = 0;
     long long slot_face_cap = 0;
     long long slot_node_cap = 0;   // per-slot change stamps: nodes of the largest table
+    long long resolved_again = 0;   // tables solved again after a failed consistency check
     bool slot_wave = false;        // the slots hold wavefront state (sized by slot_node_cap)
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;

@@ -1,0 +1,235 @@
+// ray.cpp -- host side of sphrayderv's ray tracing over resident tables
+// (include/fdtomoc_amd.h, fdgpu_rays_trace): the per-pick ray loop of
+// src/sphrayderv.c:1069-2252 for a whole batch of picks in one launch per
+// chunk of rays (ray_kernel.hip).
+//
+// The host orders the rays by table (lanes of a wave then walk the same
+// table), computes the row and column sines and cosines of the grid's cell
+// faces and each station's Cartesian point with its C library (the values
+// the reference computes), and collects each ray's derivatives from the
+// packed arrays the kernel appends to.  A ray whose derivatives did not fit
+// the launch's packed arrays is traced again in a later launch (tracing is
+// deterministic); every launch finishes at least one such ray, because the
+// packed arrays hold any single ray's derivatives.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "fdgpu_internal.h"
+#include "ray_kernels.h"
+#include "ray_prep.h"
+
+using namespace fdgpu_detail;
+
+namespace {
+
+// Device memory for the lane columns of derivatives: at most this, or a
+// quarter of what the budget has left (a lane's column is 4 bytes per
+// coarse variable; every ray of a batch in one launch keeps the CUs busy).
+constexpr size_t kColumnBytes = size_t(32) << 30;
+
+bool grid_ok(const fdgpu_ray_grid &g) {
+    if (g.nx < 2 || g.ny < 2 || g.nz < 2 || g.nxc < 2 || g.nyc < 2 || g.nzc < 2) return false;
+    if (!g.gx || !g.gy || !g.gz) return false;
+    if (!(g.df > 0.0) || !(g.dq > 0.0) || !(g.h > 0.0)) return false;
+    if ((long long)g.nx * g.ny * g.nz >= (1LL << 31)) return false;
+    if ((long long)g.nxc * g.nyc * g.nzc >= (1LL << 30)) return false;
+    return g.max_segments >= 0;
+}
+
+}  // namespace
+
+int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *rays, int n,
+                     fdgpu_ray_result *results, int *du_index, float *du_value, long long du_cap) {
+    if (!c || !gp || n < 0 || (n > 0 && (!rays || !results)) || du_cap < 0) return FDGPU_EINPUT;
+    const fdgpu_ray_grid &G = *gp;
+    if (!grid_ok(G)) return FDGPU_EINPUT;
+    if (n == 0) return FDGPU_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->serve.on) return FDGPU_EBUSY;   // no allocation while the request loop runs
+    if (hipSetDevice(c->device) != hipSuccess) return FDGPU_EHIP;
+
+    // tables: each distinct one once, of the grid's dimensions
+    std::map<int, int> slot;
+    std::vector<const float *> tabs;
+    for (int r = 0; r < n; r++) {
+        if (slot.count(rays[r].table)) continue;
+        auto it = c->tables.find(rays[r].table);
+        if (it == c->tables.end() || !it->second.d_time) return FDGPU_EINPUT;
+        const fdgpu_par &tp = it->second.par;
+        if (tp.nx != G.nx || tp.ny != G.ny || tp.nz != G.nz) return FDGPU_EINPUT;
+        slot[rays[r].table] = (int)tabs.size();
+        tabs.push_back(it->second.d_time);
+    }
+    // rays grouped by table (a wave's lanes then read one table), and within
+    // a table by the event-station distance in cells, longest first (a wave
+    // runs as long as its longest ray, so its lanes should walk alike)
+    std::vector<int> order((size_t)n);
+    std::vector<double> reach((size_t)n);
+    for (int r = 0; r < n; r++) {
+        const double di = (rays[r].ex - rays[r].xs) / G.df, dj = (rays[r].ey - rays[r].ys) / G.dq,
+                     dk = (rays[r].ez - rays[r].zs) / G.h;
+        reach[(size_t)r] = fabs(di) + fabs(dj) + fabs(dk);   // cells a walk crosses, about
+    }
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (rays[a].table != rays[b].table) return rays[a].table < rays[b].table;
+        return reach[(size_t)a] > reach[(size_t)b];
+    });
+    std::vector<fdray::RayIn> in((size_t)n);
+    for (int r = 0; r < n; r++) {
+        const fdgpu_ray &R = rays[r];
+        fdray::RayIn &q = in[(size_t)r];
+        q.ex = R.ex; q.ey = R.ey; q.ez = R.ez;
+        q.xs = R.xs; q.ys = R.ys; q.zs = R.zs;
+        fdray::station_point(R.xs, R.ys, R.zs, q.x2, q.y2, q.z2);
+        q.table = slot[R.table];
+    }
+    const std::vector<double> trig = fdray::face_trig(G.nx, G.ny, G.x0, G.y0, G.df, G.dq);
+    const int nvar = G.ido1d ? G.nzc : G.nxc * G.nyc * G.nzc;
+
+    // lanes per launch: the columns within their share of the budget
+    const size_t left = c->budget > c->used ? (size_t)(c->budget - c->used) : 0;
+    const size_t col_budget = std::min(kColumnBytes, left / 4);
+    long long lanes = (long long)(col_budget / (sizeof(float) * (size_t)nvar + sizeof(unsigned) * (size_t)((nvar + 31) / 32)));
+    lanes = std::min<long long>(lanes, n);
+    if (lanes < 1) return FDGPU_ENOMEM;
+    if (lanes > 256) lanes = lanes / 256 * 256;
+    const long long stride = (lanes + 63) / 64 * 64;
+    // packed derivatives of a launch: about 512 per ray, never fewer than one ray's worst case
+    const long long capacity = std::max<long long>(nvar, std::min<long long>(lanes * 512, 1LL << 26));
+
+    const size_t b_g = align256(sizeof(float) * (size_t)(G.nxc + G.nyc + G.nzc));
+    const size_t b_t = align256(sizeof(double) * trig.size());
+    const size_t b_p = align256(sizeof(const float *) * tabs.size());
+    const size_t b_r = align256(sizeof(fdray::RayIn) * (size_t)lanes);
+    const size_t b_o = align256(sizeof(RayResult) * (size_t)lanes);
+    const size_t b_d = align256(sizeof(float) * (size_t)nvar * (size_t)stride);
+    const size_t b_b = align256(sizeof(unsigned) * (size_t)((nvar + 31) / 32) * (size_t)stride);
+    const size_t b_c = 256;
+    const size_t b_i = align256(sizeof(int) * (size_t)capacity);
+    const size_t b_v = align256(sizeof(float) * (size_t)capacity);
+    char *blk = nullptr;
+    int rc = dev_alloc(c, (void **)&blk, b_g + b_t + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
+    if (rc) return rc;
+    size_t off = 0;
+    auto carve = [&](size_t b) {
+        char *p = blk + off;
+        off += b;
+        return p;
+    };
+    float *d_g = (float *)carve(b_g);
+    double *d_trig = (double *)carve(b_t);
+    const float **d_tabs = (const float **)carve(b_p);
+    fdray::RayIn *d_in = (fdray::RayIn *)carve(b_r);
+    RayResult *d_res = (RayResult *)carve(b_o);
+    float *d_du = (float *)carve(b_d);
+    unsigned *d_bits = (unsigned *)carve(b_b);
+    unsigned long long *d_cursor = (unsigned long long *)carve(b_c);
+    int *d_idx = (int *)carve(b_i);
+    float *d_val = (float *)carve(b_v);
+
+    std::vector<float> gxyz;
+    gxyz.insert(gxyz.end(), G.gx, G.gx + G.nxc);
+    gxyz.insert(gxyz.end(), G.gy, G.gy + G.nyc);
+    gxyz.insert(gxyz.end(), G.gz, G.gz + G.nzc);
+    fdray::RayGrid dg{G.nx, G.ny, G.nz, G.nxc, G.nyc, G.nzc, G.ido1d ? 1 : 0,
+                      G.max_segments > 0 ? G.max_segments : 8 * (G.nx + G.ny + G.nz),
+                      G.x0, G.y0, G.z0, G.df, G.dq, G.h, d_g, d_g + G.nxc, d_g + G.nxc + G.nyc, d_trig};
+    bool ok = hipMemcpyAsync(d_g, gxyz.data(), sizeof(float) * gxyz.size(), hipMemcpyHostToDevice, c->stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(d_trig, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, c->stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(d_tabs, tabs.data(), sizeof(const float *) * tabs.size(), hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemsetAsync(d_du, 0, b_d + b_b, c->stream) == hipSuccess;
+
+    // each ray's result and derivatives, as the launches deliver them
+    std::vector<fdray::RayOut> outs((size_t)n);
+    std::vector<long long> start((size_t)n, 0);
+    std::vector<int> h_idx;
+    std::vector<float> h_val;
+    std::vector<fdray::RayIn> chunk;
+    std::vector<int> chunk_ray;
+    std::vector<RayResult> got;
+    std::vector<int> pending = order, retry;
+    double ms_total = 0.0;
+    while (ok && !pending.empty()) {
+        retry.clear();
+        for (size_t p0 = 0; ok && p0 < pending.size(); p0 += (size_t)lanes) {
+            const int m = (int)std::min<size_t>((size_t)lanes, pending.size() - p0);
+            chunk.assign((size_t)m, fdray::RayIn{});
+            chunk_ray.assign(pending.begin() + (long)p0, pending.begin() + (long)p0 + m);
+            for (int k = 0; k < m; k++) chunk[(size_t)k] = in[(size_t)chunk_ray[(size_t)k]];
+            ok = hipMemcpyAsync(d_in, chunk.data(), sizeof(fdray::RayIn) * (size_t)m, hipMemcpyHostToDevice,
+                                c->stream) == hipSuccess &&
+                 hipMemsetAsync(d_cursor, 0, sizeof(unsigned long long), c->stream) == hipSuccess &&
+                 hipEventRecord(c->ev0, c->stream) == hipSuccess &&
+                 fd_launch_rays(&dg, d_in, d_tabs, m, d_du, d_bits, nvar, stride, d_res, d_cursor, d_idx, d_val,
+                                capacity, c->stream) == hipSuccess &&
+                 hipEventRecord(c->ev1, c->stream) == hipSuccess;
+            got.resize((size_t)m);
+            unsigned long long used = 0;
+            ok = ok &&
+                 hipMemcpyAsync(got.data(), d_res, sizeof(RayResult) * (size_t)m, hipMemcpyDeviceToHost, c->stream) ==
+                     hipSuccess &&
+                 hipMemcpyAsync(&used, d_cursor, sizeof(used), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+                 hipStreamSynchronize(c->stream) == hipSuccess;
+            if (!ok) break;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) ms_total += ms;
+            const long long filled = std::min<long long>((long long)used, capacity);
+            const size_t base = h_idx.size();
+            h_idx.resize(base + (size_t)filled);
+            h_val.resize(base + (size_t)filled);
+            if (filled > 0)
+                ok = hipMemcpy(h_idx.data() + base, d_idx, sizeof(int) * (size_t)filled, hipMemcpyDeviceToHost) ==
+                         hipSuccess &&
+                     hipMemcpy(h_val.data() + base, d_val, sizeof(float) * (size_t)filled, hipMemcpyDeviceToHost) ==
+                         hipSuccess;
+            for (int k = 0; k < m; k++) {
+                const RayResult &R = got[(size_t)k];
+                const int r = chunk_ray[(size_t)k];
+                if (R.o.ndu > 0 && R.offset < 0) {
+                    retry.push_back(r);   // its derivatives did not fit: trace it again
+                    continue;
+                }
+                outs[(size_t)r] = R.o;
+                start[(size_t)r] = (long long)base + (R.o.ndu > 0 ? R.offset : 0);
+            }
+        }
+        pending.swap(retry);
+    }
+    dev_free(c, blk, b_g + b_t + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
+    if (!ok) return FDGPU_EHIP;
+    c->last_ms = ms_total;
+
+    // results in input order; the derivatives packed in the same order
+    long long total = 0;
+    for (int r = 0; r < n; r++) {
+        const fdray::RayOut &o = outs[(size_t)r];
+        fdgpu_ray_result &q = results[r];
+        q.status = o.status;
+        q.nseg = o.nseg;
+        q.ndu = o.ndu;
+        q.first_in_cube = o.first_in_cube;
+        q.tev = o.tev;
+        q.dtdx = o.dtdx; q.dtdy = o.dtdy; q.dtdz = o.dtdz;
+        q.ai = o.ai; q.az = o.az; q.rdev = o.rdev;
+        q.offset = total;
+        total += o.ndu;
+    }
+    if (total > du_cap || (total > 0 && (!du_index || !du_value))) return FDGPU_ESPACE;
+    for (int r = 0; r < n; r++) {
+        const int k = outs[(size_t)r].ndu;
+        if (k <= 0) continue;
+        memcpy(du_index + results[r].offset, h_idx.data() + start[(size_t)r], sizeof(int) * (size_t)k);
+        memcpy(du_value + results[r].offset, h_val.data() + start[(size_t)r], sizeof(float) * (size_t)k);
+    }
+    return FDGPU_OK;
+}

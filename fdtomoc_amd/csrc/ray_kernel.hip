@@ -1,0 +1,108 @@
+// ray_kernel.hip -- gfx950 kernel of sphrayderv's ray tracing
+// (src/sphrayderv.c:1306-2250) over resident travel-time tables.
+//
+// One lane per ray: a ray is a serial walk from cell to cell (each step
+// depends on the last), so the parallelism is across rays.  The host orders
+// a launch's rays by table, so the lanes of a wave walk one station's table
+// and share its cache lines.  Each lane accumulates its ray's coarse-grid
+// derivatives in its own column of a lane-interleaved array (row v holds
+// variable v of every lane: the scan that collects them in index order is
+// coalesced across the wave), exactly as the reference's dense per-ray
+// array, in segment order.  Only the touched index range is scanned, and
+// the scan leaves the column zero for the next launch.  The position's
+// sines, cosines and arctangents are correctly rounded (cr_math.h).
+//
+// Compiled with -ffp-contract=off: every multiply and add rounds separately,
+// as in the reference's x86-64 build.
+#include <hip/hip_runtime.h>
+
+#include "ray_kernels.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// du[v] = (float)(du[v] + c) in this lane's column, and v's bit in the
+// lane's column of touched-variable bits (row w holds variables 32w..32w+31).
+struct ColumnAcc {
+    float *du;
+    unsigned *bits;
+    long long stride;
+    int nvar, lo, hi;
+    __device__ void add(int v, double c) {
+        if (v < 0 || v >= nvar) return;
+        float *p = du + (long long)v * stride;
+        *p = (float)((double)*p + c);
+        unsigned *b = bits + (long long)(v >> 5) * stride;
+        *b |= 1u << (v & 31);
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+};
+
+__global__ __launch_bounds__(kThreads) void ray_kernel(fdray::RayGrid g, const fdray::RayIn *__restrict__ rays,
+                                                       const float *const *__restrict__ tables, int n,
+                                                       float *__restrict__ du, unsigned *__restrict__ bits, int nvar,
+                                                       long long stride, RayResult *__restrict__ results,
+                                                       unsigned long long *__restrict__ cursor,
+                                                       int *__restrict__ idx, float *__restrict__ val,
+                                                       long long capacity) {
+    const int r = blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n) return;
+    const fdray::RayIn in = rays[r];
+    ColumnAcc acc{du + r, bits + r, stride, nvar, nvar, -1};
+    RayResult res;
+    fdray::ray_trace<fdray::CrMath>(g, tables[in.table], in, res.o, acc);
+    const bool ok = res.o.status == fdray::kRayOk;
+    // count the nonzero touched variables, claim space, then write them in
+    // index order and clear the column and its bits
+    int k = 0;
+    if (ok)
+        for (int w = acc.lo >> 5; w <= (acc.hi >> 5); w++) {
+            unsigned m = acc.bits[(long long)w * stride];
+            while (m) {
+                const int v = (w << 5) + __builtin_ctz(m);
+                m &= m - 1;
+                k += acc.du[(long long)v * stride] != 0.0f;
+            }
+        }
+    long long off = -1;
+    if (k > 0) {
+        const unsigned long long o = atomicAdd(cursor, (unsigned long long)k);
+        if ((long long)o + k <= capacity) off = (long long)o;
+    }
+    int wr = 0;
+    for (int w = acc.lo >> 5; w <= (acc.hi >> 5); w++) {
+        unsigned *bw = acc.bits + (long long)w * stride;
+        unsigned m = *bw;
+        if (!m) continue;
+        *bw = 0u;
+        while (m) {
+            const int v = (w << 5) + __builtin_ctz(m);
+            m &= m - 1;
+            float *p = acc.du + (long long)v * stride;
+            const float d = *p;
+            *p = 0.0f;
+            if (d != 0.0f && off >= 0) {
+                idx[off + wr] = v;
+                val[off + wr] = d;
+                wr++;
+            }
+        }
+    }
+    res.o.ndu = ok ? k : 0;
+    res.offset = k > 0 ? off : 0;
+    results[r] = res;
+}
+
+}  // namespace
+
+extern "C" hipError_t fd_launch_rays(const fdray::RayGrid *grid, const fdray::RayIn *rays, const float *const *tables,
+                                     int n, float *du, unsigned *bits, int nvar, long long stride, RayResult *results,
+                                     unsigned long long *cursor, int *idx, float *val, long long capacity,
+                                     hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ray_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, stream, *grid,
+                       rays, tables, n, du, bits, nvar, stride, results, cursor, idx, val, capacity);
+    return hipGetLastError();
+}

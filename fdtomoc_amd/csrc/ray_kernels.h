@@ -1,0 +1,27 @@
+// ray_kernels.h -- launch wrapper of the ray-tracing kernel of FDtomoC's
+// sphrayderv (ray_core.h) over resident travel-time tables, shared by the
+// kernel (ray_kernel.hip) and the host side (ray.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ray_core.h"
+
+// Device view of one ray's result: RayOut plus where its derivatives went in
+// the launch's packed arrays (-1: they did not fit; the host traces the ray
+// again in a later launch).
+struct RayResult {
+    fdray::RayOut o;
+    long long offset;
+};
+
+// Traces rays [0, n): lane r traces ray r through tables[rays[r].table],
+// accumulating its derivatives in column r of du (nvar rows of `stride`
+// floats) with a bit per touched variable in column r of bits ((nvar + 31)
+// / 32 rows of `stride` words), both all zero on entry and on return, then
+// appends its nonzero derivatives in index order to (idx, val) at an offset
+// taken from *cursor, up to `capacity` entries in all.
+extern "C" hipError_t fd_launch_rays(const fdray::RayGrid *grid, const fdray::RayIn *rays, const float *const *tables,
+                                     int n, float *du, unsigned *bits, int nvar, long long stride, RayResult *results,
+                                     unsigned long long *cursor, int *idx, float *val, long long capacity,
+                                     hipStream_t stream);

@@ -78,6 +78,51 @@ class CLocEvent(C.Structure):
                 ("ezm", C.c_double)]
 
 
+class CRayGrid(C.Structure):
+    """fdgpu_ray_grid: the fine and coarse grids of sphrayderv's rays."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("x0", C.c_double), ("y0", C.c_double),
+                ("z0", C.c_double), ("df", C.c_double), ("dq", C.c_double), ("h", C.c_double), ("nxc", C.c_int),
+                ("nyc", C.c_int), ("nzc", C.c_int), ("gx", C.c_void_p), ("gy", C.c_void_p), ("gz", C.c_void_p),
+                ("ido1d", C.c_int), ("max_segments", C.c_int)]
+
+
+class CRay(C.Structure):
+    _fields_ = [("table", C.c_int), ("ex", C.c_double), ("ey", C.c_double), ("ez", C.c_double),
+                ("xs", C.c_double), ("ys", C.c_double), ("zs", C.c_double)]
+
+
+class CRayResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("nseg", C.c_int), ("ndu", C.c_int), ("first_in_cube", C.c_int),
+                ("tev", C.c_double), ("dtdx", C.c_float), ("dtdy", C.c_float), ("dtdz", C.c_float),
+                ("ai", C.c_float), ("az", C.c_float), ("rdev", C.c_float), ("offset", C.c_longlong)]
+
+
+RAY_FIELDS = [name for name, _ in CRayResult._fields_ if name != "offset"]
+
+
+def _np_dtype(ct) -> np.dtype:
+    """numpy view of a ctypes structure's layout."""
+    names = [n for n, _ in ct._fields_]
+    fmt = {C.c_int: "<i4", C.c_double: "<f8", C.c_float: "<f4", C.c_longlong: "<i8"}
+    return np.dtype({"names": names, "formats": [fmt[t] for _, t in ct._fields_],
+                     "offsets": [getattr(ct, n).offset for n in names], "itemsize": C.sizeof(ct)})
+
+
+RAY_DTYPE = _np_dtype(CRay)                 # one fdgpu_ray
+RAY_RESULT_DTYPE = _np_dtype(CRayResult)    # one fdgpu_ray_result
+
+
+def ray_array(tables, coords) -> np.ndarray:
+    """fdgpu_ray records: tables (n ids) and coords (n x 6: ex, ey, ez, xs,
+    ys, zs in the grid's units)."""
+    coords = np.asarray(coords, dtype=np.float64).reshape(-1, 6)
+    a = np.zeros(len(coords), dtype=RAY_DTYPE)
+    a["table"] = np.asarray(tables, dtype=np.int32)
+    for k, f in enumerate(("ex", "ey", "ez", "xs", "ys", "zs")):
+        a[f] = coords[:, k]
+    return a
+
+
 def loc_params(**kw) -> CLocParams:
     """fdgpu_loc_params with sphfdloc's defaults (src/sphfdloc.c:111-128), then kw."""
     p = CLocParams()
@@ -208,7 +253,7 @@ class FdgpuError(RuntimeError):
         self.ids = None   # EKERNEL from a solve: the tables stay registered (their stats name the error)
 
 
-EKERNEL, EBUSY, ETIMEOUT, ESCRATCH = -9, -10, -11, -12
+EKERNEL, EBUSY, ETIMEOUT, ESCRATCH, ESPACE = -9, -10, -11, -12, -13
 SERVE_INFO = ["on", "outstanding", "arena_used", "arena_bytes", "launches", "workgroups"]
 
 
@@ -277,6 +322,9 @@ def lib():
         L.fdgpu_serve_release.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_serve_info.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.fdgpu_serve_stop.argtypes = [C.c_void_p]
+        L.fdgpu_context_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.fdgpu_rays_trace.argtypes = [C.c_void_p, C.POINTER(CRayGrid), C.POINTER(CRay), C.c_int,
+                                       C.POINTER(CRayResult), C.c_void_p, C.c_void_p, C.c_longlong]
         L.sphfd.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_char_p]
         L.fdgpu_sphfd_info.argtypes = [C.c_void_p, C.c_int]
         _lib = L
@@ -362,6 +410,37 @@ class Context:
         """fdgpu_locate on a prepared batch (its events' C records built once):
         the results are in the batch afterwards (LocBatch.results)."""
         _check(lib().fdgpu_locate(self._h, C.byref(params), batch.arr, batch.n), "fdgpu_locate")
+
+    def trace_rays(self, grid, rays, du_cap: int | None = None):
+        """sphrayderv's rays through resident tables (fdgpu_rays_trace).
+        grid: a fdtomoc_amd.rayderv.RayGrid; rays: a RAY_DTYPE array
+        (ray_array) or (table, ex, ey, ez, xs, ys, zs) tuples.  Returns
+        (results, du_index, du_value): a RAY_RESULT_DTYPE array (status,
+        segments, derivative count and offset, event time and derivatives,
+        angles) and the packed derivatives, each ray's in index order, rays
+        in input order.  du_cap: size the derivative arrays to it (an
+        FDGPU_ESPACE error when they are too small) instead of growing them."""
+        if not isinstance(rays, np.ndarray):
+            rays = ray_array([r[0] for r in rays], [r[1:7] for r in rays])
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        gx, gy, gz = (np.ascontiguousarray(a, dtype=np.float32) for a in (grid.gx, grid.gy, grid.gz))
+        g = CRayGrid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.df, grid.dq, grid.h, len(gx), len(gy),
+                     len(gz), gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, grid.ido1d, grid.max_segments)
+        n = len(rays)
+        res = np.zeros(max(n, 1), dtype=RAY_RESULT_DTYPE)
+        cap = du_cap if du_cap is not None else 128 * max(n, 1)
+        while True:
+            idx = np.empty(max(cap, 1), dtype=np.int32)
+            val = np.empty(max(cap, 1), dtype=np.float32)
+            rc = lib().fdgpu_rays_trace(self._h, C.byref(g), rays.ctypes.data_as(C.POINTER(CRay)), n,
+                                        res.ctypes.data_as(C.POINTER(CRayResult)), idx.ctypes.data,
+                                        val.ctypes.data, cap)
+            if rc != ESPACE or du_cap is not None:
+                break
+            cap = int(res["ndu"][:n].sum())
+        _check(rc, "fdgpu_rays_trace")
+        total = int(res["ndu"][:n].sum())
+        return res[:n], idx[:total], val[:total]
 
     def write_async(self, tables: list, paths: list, vel_headers: list | None = None):
         """Background writes of resident tables (fdgpu_tables_write_async);
@@ -492,6 +571,12 @@ class Context:
         b = C.c_int()
         _check(lib().fdgpu_get_schedule(self._h, C.byref(b)), "fdgpu_get_schedule")
         return b.value
+
+    def resolved_again(self) -> int:
+        """Tables solved a second time after a failed kernel consistency check."""
+        c = (C.c_longlong * 1)()
+        _check(lib().fdgpu_context_counters(self._h, c, 1), "fdgpu_context_counters")
+        return int(c[0])
 
     def last_kernel_ms(self) -> float:
         return float(lib().fdgpu_last_kernel_ms(self._h))

@@ -40,8 +40,10 @@ enum {
                                finished requests), or a call that allocates or frees
                                device memory made while the request loop runs */
     FDGPU_ETIMEOUT = -11,   /* serving: request not finished within the wait's timeout */
-    FDGPU_ESCRATCH = -12    /* fdgpu_solve_noalloc: the table's device state exceeds the
+    FDGPU_ESCRATCH = -12,   /* fdgpu_solve_noalloc: the table's device state exceeds the
                                library's fixed 8 MiB array */
+    FDGPU_ESPACE = -13      /* fdgpu_rays_trace: the derivative arrays are too small (the
+                               results state how many entries each ray needs) */
 };
 
 /* Par-file values of one table (keys of src/sphfd.c:383-530), before the
@@ -171,6 +173,78 @@ int fdgpu_locate(fdgpu_ctx *ctx, const fdgpu_loc_params *p, fdgpu_loc_event *eve
  * a usable pick names, per searched node, per pass); n <= 4 values. */
 int fdgpu_locate_timing(fdgpu_ctx *ctx, double *t, int n);
 
+/* ------------------------------------------------ ray tracing (sphrayderv)
+ *
+ * The rays of sphrayderv (src/sphrayderv.c:1306-2250, lib/sphrayderv
+ * dfind.c, pmin.c) through resident travel-time tables: from each event
+ * back to its station, cell by cell down the gradient of the station's
+ * table, with each segment's length spread trilinearly onto the coarse
+ * inversion grid at its midpoint.  Replaces the per-pick loop of
+ * sphrayderv.c:1069-2252 and its table reads (1163-1217); the caller keeps
+ * the bookkeeping around it (weights, residual statistics, S-variable
+ * offsets, the matrix files).  Results equal the reference's except where
+ * the device's sine, cosine and arctangent of a ray position differ from the
+ * host C library's in the last bit (derivatives and residuals are float
+ * results of double steps; such a difference rarely reaches them).
+ *
+ * The grid in sphrayderv's units after its set-up (sphrayderv.c:527-545,
+ * 814-826): x0 longitude and y0 geocentric colatitude of the origin in
+ * radians, z0 depth in km, df/dq radians, h km; gx/gy/gz the coarse node
+ * coordinates as the program accumulates them (floats). */
+typedef struct {
+    int nx, ny, nz;                 /* fine grid: every table's dimensions */
+    double x0, y0, z0, df, dq, h;
+    int nxc, nyc, nzc;              /* coarse nodes per axis, >= 2 */
+    const float *gx, *gy, *gz;      /* nxc, nyc, nzc coarse coordinates (host arrays) */
+    int ido1d;                      /* 1: derivatives per coarse layer (sphrayderv.c:1567-1571) */
+    int max_segments;               /* segments before a ray is given up (0: 8*(nx+ny+nz));
+                                       the reference has no limit */
+} fdgpu_ray_grid;
+
+/* One pick's ray: its station-phase table and the event and station in the
+ * grid's units -- the event's ex, ey, ez (sphrayderv.c:949-951) and the
+ * station's stlon, stlat (float values) and stz (sphrayderv.c:874-876). */
+typedef struct {
+    int table;
+    double ex, ey, ez;
+    double xs, ys, zs;
+} fdgpu_ray;
+
+enum {
+    FDGPU_RAY_OK = 1,          /* reached the station's cell: derivatives valid */
+    FDGPU_RAY_UNREACHED = 0,   /* a corner of the event's cell holds no time (> 1e9; sphrayderv.c:1340-1350) */
+    FDGPU_RAY_LEFT = -1,       /* left the medium (sphrayderv.c:2248-2251) */
+    FDGPU_RAY_EVENT_OUT = -2,  /* event outside the grid (sphrayderv.c:958-962) */
+    FDGPU_RAY_LIMIT = -3,      /* more than max_segments segments */
+    FDGPU_RAY_COARSE = -4,     /* a segment midpoint beyond the coarse grid (the reference asserts) */
+    FDGPU_RAY_EDGE = -5        /* a cell whose corners lie outside the table (the reference reads
+                                  memory beyond the table there) */
+};
+
+typedef struct {
+    int status;                 /* FDGPU_RAY_* */
+    int nseg;                   /* segments traced */
+    int ndu;                    /* nonzero derivatives (FDGPU_RAY_OK only) */
+    int first_in_cube;          /* 1: the first segment ran inside the station cube; ai/az are NaN
+                                   (the reference keeps its previous ray's values there) */
+    double tev;                 /* trilinear table time at the event (dt before corrections, 1385-1390) */
+    float dtdx, dtdy, dtdz;     /* its derivatives (1404-1445) */
+    float ai, az;               /* incidence and azimuth at the source, degrees (1865-1873) */
+    float rdev;                 /* mean deviation from the source-station plane (1592) */
+    long long offset;           /* first derivative of this ray in du_index/du_value */
+} fdgpu_ray_result;
+
+/* Traces n rays.  The derivatives of the rays that reached their station go
+ * to du_index (coarse variable, P numbering: the caller adds nxc*nyc*nzc
+ * for S picks as sphrayderv.c:1653-1656 does) and du_value (path length in
+ * km, before weighting), each ray's in index order, rays in input order,
+ * at results[r].offset.  When the total exceeds du_cap the call returns
+ * FDGPU_ESPACE with every result filled and nothing written to du_*: call
+ * again with room for the sum of ndu.  FDGPU_EINPUT for an unknown table or
+ * one whose dimensions are not the grid's, or bad grid values. */
+int fdgpu_rays_trace(fdgpu_ctx *ctx, const fdgpu_ray_grid *grid, const fdgpu_ray *rays, int n,
+                     fdgpu_ray_result *results, int *du_index, float *du_value, long long du_cap);
+
 /* One table request.  Table node (0,0,0) sits at model node (ox, oy, oz); the
  * table spans par.nx*par.ny*par.nz nodes inside the model. */
 typedef struct {
@@ -243,6 +317,12 @@ int fdgpu_set_workgroup(fdgpu_ctx *ctx, int lanes);
  * tables whose wavefront cannot finish in the slot's state run face by face. */
 int fdgpu_set_wave(fdgpu_ctx *ctx, int flags);
 int fdgpu_get_wave(fdgpu_ctx *ctx, int *flags);
+
+/* Counters of the context (no reference counterpart): c[0] tables solved a
+ * second time, face by face, because one of the sweep kernel's own
+ * consistency checks failed on the first solve (the second solve's results
+ * are kept; a second failure is FDGPU_EKERNEL); n <= 1 values. */
+int fdgpu_context_counters(fdgpu_ctx *ctx, long long *c, int n);
 
 /* Copies a resident table (nx*ny*nz floats) to host memory. */
 int fdgpu_table_read(fdgpu_ctx *ctx, int table, float *host, unsigned long long count);

@@ -330,11 +330,29 @@ class RayOutC(C.Structure):
 RAY_FIELDS = [name for name, _ in RayOutC._fields_]
 
 
-def ray_trace(grid, tables, rays, cap: int = 4096, nthreads: int = 8) -> list:
-    """The oracle's rays (oracle/ray_oracle.c).  grid: a
-    fdtomoc_amd.rayderv.RayGrid; tables[r]: ray r's table (nz, ny, nx);
-    rays: (ex, ey, ez, xs, ys, zs) tuples.  Returns one dict per ray with the
-    output fields and its derivatives (idx, val) in index order."""
+EMU_RAY_SO = os.path.join(REPO, "tests", "native", "libemu_ray.so")
+_emu_ray = None
+
+
+def emu_ray_lib():
+    """The device ray tracer's host build (tests/native/emu_ray.cpp), with
+    the oracle's entry-point signature."""
+    global _emu_ray
+    if _emu_ray is None:
+        L = C.CDLL(EMU_RAY_SO)
+        L.emu_ray_trace_many.argtypes = lib().oracle_ray_trace_many.argtypes
+        L.emu_ray_trace_many.restype = C.c_int
+        L.oracle_ray_trace_many = L.emu_ray_trace_many
+        _emu_ray = L
+    return _emu_ray
+
+
+def ray_trace(grid, tables, rays, cap: int = 4096, nthreads: int = 8, impl=None) -> list:
+    """The oracle's rays (oracle/ray_oracle.c; impl=emu_ray_lib() for the
+    device code's host build).  grid: a fdtomoc_amd.rayderv.RayGrid;
+    tables[r]: ray r's table (nz, ny, nx); rays: (ex, ey, ez, xs, ys, zs)
+    tuples.  Returns one dict per ray with the output fields and its
+    derivatives (idx, val) in index order."""
     gx, gy, gz = (np.ascontiguousarray(a, dtype=np.float32) for a in (grid.gx, grid.gy, grid.gz))
     g = RayGridC(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0, grid.df, grid.dq, grid.h, len(gx), len(gy),
                  len(gz), gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, grid.ido1d, grid.segment_limit())
@@ -346,8 +364,8 @@ def ray_trace(grid, tables, rays, cap: int = 4096, nthreads: int = 8) -> list:
     while True:
         idx = np.zeros(max(n, 1) * cap, dtype=np.int32)
         val = np.zeros(max(n, 1) * cap, dtype=np.float32)
-        over = lib().oracle_ray_trace_many(C.byref(g), ptrs, ins, n, outs, idx.ctypes.data, val.ctypes.data, cap,
-                                           nthreads)
+        over = (impl or lib()).oracle_ray_trace_many(C.byref(g), ptrs, ins, n, outs, idx.ctypes.data,
+                                                     val.ctypes.data, cap, nthreads)
         if over == 0:
             break
         cap = max(o.ndu for o in outs[:n])

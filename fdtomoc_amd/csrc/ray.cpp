@@ -91,6 +91,11 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         q.table = slot[R.table];
     }
     const std::vector<double> trig = fdray::face_trig(G.nx, G.ny, G.x0, G.y0, G.df, G.dq);
+    std::vector<int> guess = fdray::coarse_guess(G.gx, G.nxc, G.x0, G.df, G.nx);
+    const std::vector<int> gyv = fdray::coarse_guess(G.gy, G.nyc, G.y0, G.dq, G.ny),
+                           gzv = fdray::coarse_guess(G.gz, G.nzc, G.z0, G.h, G.nz);
+    guess.insert(guess.end(), gyv.begin(), gyv.end());
+    guess.insert(guess.end(), gzv.begin(), gzv.end());
     const int nvar = G.ido1d ? G.nzc : G.nxc * G.nyc * G.nzc;
 
     // lanes per launch: the columns within their share of the budget
@@ -106,6 +111,7 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
 
     const size_t b_g = align256(sizeof(float) * (size_t)(G.nxc + G.nyc + G.nzc));
     const size_t b_t = align256(sizeof(double) * trig.size());
+    const size_t b_m = align256(sizeof(int) * guess.size());
     const size_t b_p = align256(sizeof(const float *) * tabs.size());
     const size_t b_r = align256(sizeof(fdray::RayIn) * (size_t)lanes);
     const size_t b_o = align256(sizeof(RayResult) * (size_t)lanes);
@@ -115,7 +121,7 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     const size_t b_i = align256(sizeof(int) * (size_t)capacity);
     const size_t b_v = align256(sizeof(float) * (size_t)capacity);
     char *blk = nullptr;
-    int rc = dev_alloc(c, (void **)&blk, b_g + b_t + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
+    int rc = dev_alloc(c, (void **)&blk, b_g + b_t + b_m + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
     if (rc) return rc;
     size_t off = 0;
     auto carve = [&](size_t b) {
@@ -125,6 +131,7 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     };
     float *d_g = (float *)carve(b_g);
     double *d_trig = (double *)carve(b_t);
+    int *d_guess = (int *)carve(b_m);
     const float **d_tabs = (const float **)carve(b_p);
     fdray::RayIn *d_in = (fdray::RayIn *)carve(b_r);
     RayResult *d_res = (RayResult *)carve(b_o);
@@ -140,10 +147,13 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     gxyz.insert(gxyz.end(), G.gz, G.gz + G.nzc);
     fdray::RayGrid dg{G.nx, G.ny, G.nz, G.nxc, G.nyc, G.nzc, G.ido1d ? 1 : 0,
                       G.max_segments > 0 ? G.max_segments : 8 * (G.nx + G.ny + G.nz),
-                      G.x0, G.y0, G.z0, G.df, G.dq, G.h, d_g, d_g + G.nxc, d_g + G.nxc + G.nyc, d_trig};
+                      G.x0, G.y0, G.z0, G.df, G.dq, G.h, d_g, d_g + G.nxc, d_g + G.nxc + G.nyc, d_trig,
+                      d_guess, d_guess + G.nx, d_guess + G.nx + G.ny};
     bool ok = hipMemcpyAsync(d_g, gxyz.data(), sizeof(float) * gxyz.size(), hipMemcpyHostToDevice, c->stream) ==
                   hipSuccess &&
               hipMemcpyAsync(d_trig, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, c->stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(d_guess, guess.data(), sizeof(int) * guess.size(), hipMemcpyHostToDevice, c->stream) ==
                   hipSuccess &&
               hipMemcpyAsync(d_tabs, tabs.data(), sizeof(const float *) * tabs.size(), hipMemcpyHostToDevice,
                              c->stream) == hipSuccess &&
@@ -205,7 +215,7 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         }
         pending.swap(retry);
     }
-    dev_free(c, blk, b_g + b_t + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
+    dev_free(c, blk, b_g + b_t + b_m + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
     if (!ok) return FDGPU_EHIP;
     c->last_ms = ms_total;
 

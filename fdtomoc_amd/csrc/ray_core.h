@@ -54,12 +54,15 @@ enum : int {
 // trig: per row j in [-1, ny] four doubles sin(yj), cos(yj), sin(yj + dq),
 // cos(yj + dq) with yj = dq*j + y0, then per column i in [-1, nx]
 // sin(xi), cos(xi), sin(xi + df), cos(xi + df) with xi = df*i + x0.
+// mx/my/mz: per fine node index, a first guess of the coarse search's
+// result near it (any guess gives the same result; a close one saves steps).
 struct RayGrid {
     int nx, ny, nz, nxc, nyc, nzc;
     int ido1d, max_segments;
     double x0, y0, z0, df, dq, h;
     const float *gx, *gy, *gz;
     const double *trig;
+    const int *mx, *my, *mz;
 };
 
 // One ray: event and station (sphrayderv.c:949-951, 874-876), and the
@@ -136,10 +139,16 @@ RAY_INLINE void ray_pick(const double dd[3], double tolmin, double tolmax, doubl
     }
 }
 
-RAY_INLINE int ray_coarse_upper(const float *g, int n, float v) {
-    for (int i = 1; i < n; i++)
-        if (v < g[i]) return i;
-    return -1;
+// The first coarse node i >= 1 with v < g[i], or -1 (the reference's linear
+// searches): from a guess taken at v's fine node, down while the node below
+// also qualifies, then up to the first that does (g is increasing).
+RAY_INLINE int ray_coarse_upper(const float *g, int n, const int *map, int nf, double g0, double step, float v) {
+    const double f = ((double)v - g0) / step;
+    int i = (f >= 0.0 && f < (double)nf) ? map[(int)f] : 1;
+    i = i < 1 ? 1 : (i > n ? n : i);
+    while (i > 1 && v < g[i - 1]) i--;
+    while (i < n && !(v < g[i])) i++;
+    return i < n ? i : -1;
 }
 
 // One segment's length at its midpoint onto the coarse derivatives: the
@@ -149,9 +158,9 @@ RAY_INLINE int ray_coarse_upper(const float *g, int n, float v) {
 // kept (the reference never reads them).
 template <class Acc>
 RAY_INLINE bool ray_add_segment(const RayGrid &g, Acc &acc, float xm, float ym, float zm, double dlen) {
-    const int i1 = ray_coarse_upper(g.gx, g.nxc, xm);
-    const int j1 = ray_coarse_upper(g.gy, g.nyc, ym);
-    const int k1 = ray_coarse_upper(g.gz, g.nzc, zm);
+    const int i1 = ray_coarse_upper(g.gx, g.nxc, g.mx, g.nx, g.x0, g.df, xm);
+    const int j1 = ray_coarse_upper(g.gy, g.nyc, g.my, g.ny, g.y0, g.dq, ym);
+    const int k1 = ray_coarse_upper(g.gz, g.nzc, g.mz, g.nz, g.z0, g.h, zm);
     if (i1 < 0 || j1 < 0 || k1 < 0) return false;
     const float hx = g.gx[i1] - g.gx[i1 - 1];
     const float hy = g.gy[j1] - g.gy[j1 - 1];

@@ -32,9 +32,11 @@ struct ColumnAcc {
     __device__ void add(int v, double c) {
         if (v < 0 || v >= nvar) return;
         float *p = du + (long long)v * stride;
-        *p = (float)((double)*p + c);
-        unsigned *b = bits + (long long)(v >> 5) * stride;
-        *b |= 1u << (v & 31);
+        const float old = *p;
+        *p = (float)((double)old + c);
+        // mark the variable on its first touch (a zero sum marks it again,
+        // harmlessly); the OR is not waited for
+        if (old == 0.0f) atomicOr(bits + (long long)(v >> 5) * stride, 1u << (v & 31));
         lo = v < lo ? v : lo;
         hi = v > hi ? v : hi;
     }

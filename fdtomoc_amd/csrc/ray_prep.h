@@ -44,4 +44,17 @@ inline void station_point(double xs, double ys, double zs, float &x2, float &y2,
     z2 = (float)(r2 * cos(ys));
 }
 
+// Guesses for the coarse searches (RayGrid::mx/my/mz): per fine node f of
+// an axis, the search's result at the middle of the fine cell f.
+inline std::vector<int> coarse_guess(const float *g, int n, double g0, double step, int nf) {
+    std::vector<int> m((size_t)nf);
+    for (int f = 0; f < nf; f++) {
+        const float v = (float)(g0 + (f + 0.5) * step);
+        int i = 1;
+        while (i < n && !(v < g[i])) i++;
+        m[(size_t)f] = i;
+    }
+    return m;
+}
+
 }  // namespace fdray

@@ -39,8 +39,12 @@ extern "C" int emu_ray_trace_many(const oracle_ray_grid *g, const float *const *
                                   int n, oracle_ray_out *out, int *idx, float *val, int cap, int nthreads) {
     (void)nthreads;
     std::vector<double> trig = fdray::face_trig(g->nx, g->ny, g->x0, g->y0, g->df, g->dq);
+    const std::vector<int> mx = fdray::coarse_guess(g->gx, g->nxc, g->x0, g->df, g->nx),
+                           my = fdray::coarse_guess(g->gy, g->nyc, g->y0, g->dq, g->ny),
+                           mz = fdray::coarse_guess(g->gz, g->nzc, g->z0, g->h, g->nz);
     fdray::RayGrid G{g->nx, g->ny, g->nz, g->nxc, g->nyc, g->nzc, g->ido1d, g->max_segments,
-                     g->x0, g->y0, g->z0, g->df, g->dq, g->h, g->gx, g->gy, g->gz, trig.data()};
+                     g->x0, g->y0, g->z0, g->df, g->dq, g->h, g->gx, g->gy, g->gz, trig.data(),
+                     mx.data(), my.data(), mz.data()};
     const int nvar = g->ido1d ? g->nzc : g->nxc * g->nyc * g->nzc;
     std::vector<float> du((size_t)nvar);
     int over = 0;

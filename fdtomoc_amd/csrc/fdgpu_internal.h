@@ -210,7 +210,7 @@ struct fdgpu_ctx {
     // the last fdgpu_locate: passes, pass ms, node-search ms, node-search table bytes
     double loc_timing[4] = {0.0, 0.0, 0.0, 0.0};
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
-    int wave = 1;              // passes run as wavefronts (fdgpu_set_wave): bit0 first, bit1 reverse
+    int wave = 0;              // passes run as wavefronts (fdgpu_set_wave): bit0 first, bit1 reverse
     // this device's copy of the fixed array behind fdgpu_solve_noalloc
     // (sweep_kernel.hip `scratch`), looked up once at fdgpu_open
     char *d_fixed = nullptr;

@@ -313,8 +313,10 @@ int fdgpu_set_workgroup(fdgpu_ctx *ctx, int lanes);
  * table: a node is evaluated as soon as every neighbour that precedes it in
  * the reference's face-by-face, time-sorted order is final, so the faces of a
  * pass overlap instead of running one after another.  Bit 0: the first pass
- * (box growth); bit 1: reverse passes.  Default 1.  Tables of srctype 2/3 and
- * tables whose wavefront cannot finish in the slot's state run face by face. */
+ * (box growth); bit 1: reverse passes.  Default 0: face by face, the
+ * faster on most measured workloads (fdtomoc_amd/autotune.py times both on
+ * the caller's tables).  Tables of srctype 2/3 and tables whose wavefront
+ * cannot finish in the slot's state run face by face. */
 int fdgpu_set_wave(fdgpu_ctx *ctx, int flags);
 int fdgpu_get_wave(fdgpu_ctx *ctx, int *flags);
 

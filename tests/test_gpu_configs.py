@@ -112,8 +112,8 @@ def test_c1_full_size(ctx):
 
 @pytest.mark.parametrize("wave", [0, 3])
 def test_c2_full_launch(ctx, expected, wave):
-    """wave 0: every pass face by face (LEVEL/CHANGE); wave 3: every pass as a
-    wavefront (the other tests run the default, first passes as wavefronts)."""
+    """wave 0 (the default): every pass face by face (LEVEL/CHANGE); wave 3:
+    every pass as a wavefront."""
     from fdtomoc_amd import fdgpu
     g, vp, vs, pars = cc.c2()
     mp, ms = ctx.model(vp), ctx.model(vs)
@@ -124,7 +124,7 @@ def test_c2_full_launch(ctx, expected, wave):
     try:
         ids = ctx.solve(specs, fdgpu.BATCHED)
     finally:
-        ctx.set_wave(1)
+        ctx.set_wave(0)
     print(f"C2 (wave {wave}): 96 tables in one launch, {time.time() - t0:.2f} s", flush=True)
     assert all(ctx.stats(i)["wave_passes"] == (ctx.stats(i)["passes"] if wave == 3 else 0) for i in ids)
     for q in cc.C2_CHECK:
@@ -153,7 +153,7 @@ def test_c3_all_windows(ctx, expected, wave):
     try:
         ids = ctx.solve([fdgpu.TableSpec(w.par, m, w.origin) for w in wins], fdgpu.BATCHED)
     finally:
-        ctx.set_wave(1)
+        ctx.set_wave(0)
     print(f"C3 (wave {wave}): 256 windows in one launch, {time.time() - t0:.2f} s", flush=True)
     assert all(ctx.stats(i)["wave_passes"] >= 1 for i in ids)
 

@@ -21,6 +21,7 @@ def test_resident_tables_reclaim_launch_scratch():
     ctx = fdgpu.Context(0, BUDGET)
     try:
         m = ctx.model(v)
+        ctx.set_wave(1)   # first passes as wavefronts: the slots hold wavefront state
         ids = ctx.solve([fdgpu.TableSpec(par, m)] * 8, fdgpu.BATCHED)
         first_wave = ctx.stats(ids[0])["wave_passes"]
         batch = 8

@@ -39,15 +39,15 @@ def test_kernel_matches_oracle_batched(ctx, seed):
     print(f"seed {seed} shape {v.shape} oracle {st} gpu {gst}", flush=True)
     assert gst["error"] & ~1 == 0
     assert gst["passes"] == st["passes"]
-    assert gst["wave_passes"] == 1   # default: the first pass as a wavefront
+    assert gst["wave_passes"] == 0   # default: every pass face by face
     assert _bits_equal(t, t_ref) == 0
 
 
-@pytest.mark.parametrize("wave", [0, 3])
+@pytest.mark.parametrize("wave", [1, 3])
 def test_kernel_matches_oracle_wave_modes(ctx, wave):
-    """The 24 seeded option mixes again with every pass face by face (wave 0)
-    and with every pass as one dataflow wavefront (wave 3), all in one launch;
-    the default (first passes as wavefronts) runs in the test above."""
+    """The 24 seeded option mixes again with the first pass as one dataflow
+    wavefront (wave 1) and with every pass as one (wave 3), all in one launch;
+    the default (every pass face by face) runs in the test above."""
     from fdtomoc_amd import fdgpu
     cases = [make_case(seed) for seed in range(24)]
     models = [ctx.model(v) for _, v in cases]
@@ -55,7 +55,7 @@ def test_kernel_matches_oracle_wave_modes(ctx, wave):
     try:
         ids = ctx.solve([fdgpu.TableSpec(par_dict(par), m) for (par, _), m in zip(cases, models)], fdgpu.BATCHED)
     finally:
-        ctx.set_wave(1)
+        ctx.set_wave(0)
     for (par, v), tid in zip(cases, ids):
         t_ref, st = ob.solve(par, ob.slowness(v))
         gst = ctx.stats(tid)
@@ -63,7 +63,7 @@ def test_kernel_matches_oracle_wave_modes(ctx, wave):
         ctx.release(tid)
         assert gst["error"] & ~1 == 0
         assert gst["passes"] == st["passes"]
-        assert gst["wave_passes"] == (gst["passes"] if wave == 3 else 0)
+        assert gst["wave_passes"] == (gst["passes"] if wave == 3 else 1)
         assert _bits_equal(t, t_ref) == 0
     for m in models:
         ctx.release_model(m)

@@ -186,20 +186,25 @@ def base_line(args, value, unit, ms_per_step, config, dtype="f64"):
 
 
 # HBM bytes per launch (FETCH_SIZE x calibration + WRITE_SIZE x calibration)
-# counted by rocprofv3 for one launch of a workload at the per-face schedule it
-# was profiled with; attached to a bench line only when the run uses that schedule.
+# counted by rocprofv3 for one launch of a workload at the schedule (wavefront
+# flags, CHANGE budget) it was profiled with; attached to a bench line only
+# when the run uses that schedule.
 PROFILED_TRAFFIC = {
-    # FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0 at change budget 0
-    "c2": {"change_budget": 0, "bytes": 323.92e9, "source": "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md"},
-    "c3": {"change_budget": 2, "bytes": 398.23e9, "source": "profiles/sphfd_sweep_kernel_c3_256windows.md"},
+    # FETCH_SIZE x 1.99992 + WRITE_SIZE x 1.0
+    "c2": [{"wave": 0, "change_budget": 0, "bytes": 323.92e9,
+            "source": "profiles/sphfd_sweep_kernel_c2_96tables_hbm_traffic.md"}],
+    "c3": [{"wave": 0, "change_budget": 2, "bytes": 398.23e9,
+            "source": "profiles/sphfd_sweep_kernel_c3_256windows.md"},
+           {"wave": 3, "change_budget": 2, "bytes": 1484.32e9,
+            "source": "profiles/sphfd_sweep_kernel_c3_256windows_wavefront.md"}],
 }
 
 
 def profiled_traffic(workload: str, sched: dict):
-    p = PROFILED_TRAFFIC.get(workload)
-    if p is None or p["change_budget"] != sched.get("change_budget"):
-        return None, None
-    return p["bytes"], p["source"]
+    for p in PROFILED_TRAFFIC.get(workload, []):
+        if p["wave"] == sched.get("wave") and p["change_budget"] == sched.get("change_budget"):
+            return p["bytes"], p["source"]
+    return None, None
 
 
 def roofline(visits, kern_s, traffic=None, source=None):

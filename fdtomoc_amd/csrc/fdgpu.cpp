@@ -59,13 +59,23 @@ void release_scratch(fdgpu_ctx *c) {
     c->slot_wave = false;
 }
 
+// The ray tracer's cached lane columns (regrown by its next call).
+void release_ray_columns(fdgpu_ctx *c) {
+    if (!c->ray_cols) return;
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(c, c->ray_cols, c->ray_cols_bytes);
+    c->ray_cols = nullptr;
+    c->ray_cols_bytes = 0;
+}
+
 // Memory that outlives a launch (tables, models): when the budget is short,
-// the launch slots are released first; the next launch regrows them within
-// what is left.
+// the launch slots and the ray tracer's columns are released first; the
+// next launch or trace regrows them within what is left.
 int dev_alloc_resident(fdgpu_ctx *c, void **p, size_t bytes) {
     int rc = dev_alloc(c, p, bytes);
-    if (rc == FDGPU_ENOMEM && c->d_scratch) {
+    if (rc == FDGPU_ENOMEM && (c->d_scratch || c->ray_cols)) {
         release_scratch(c);
+        release_ray_columns(c);
         rc = dev_alloc(c, p, bytes);
     }
     return rc;
@@ -502,6 +512,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     dev_free(c, c->d_jobs, sizeof(fdsweep::TableJob) * (size_t)c->jobs_cap);
     dev_free(c, c->d_scratch, c->scratch_bytes);
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
+    dev_free(c, c->ray_cols, c->ray_cols_bytes);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -194,6 +194,8 @@ struct fdgpu_ctx {
     long long slot_face_cap = 0;
     long long slot_node_cap = 0;   // per-slot change stamps: nodes of the largest table
     long long resolved_again = 0;   // tables solved again after a failed consistency check
+    char *ray_cols = nullptr;       // fdgpu_rays_trace's lane columns (kept zero between calls)
+    size_t ray_cols_bytes = 0;
     bool slot_wave = false;        // the slots hold wavefront state (sized by slot_node_cap)
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;
@@ -229,6 +231,7 @@ void writer_shutdown(fdgpu_ctx *c);
 int dev_alloc(fdgpu_ctx *c, void **p, size_t bytes);
 void dev_free(fdgpu_ctx *c, void *p, size_t bytes);
 int dev_alloc_resident(fdgpu_ctx *c, void **p, size_t bytes);
+void release_ray_columns(fdgpu_ctx *c);
 void release_scratch(fdgpu_ctx *c);
 // One slot's scratch: face arrays for faces of up to F nodes, plus change
 // stamps for tables of up to N nodes and last full-plane marks for dimensions

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <map>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "fdgpu_internal.h"
@@ -81,13 +82,22 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         if (rays[a].table != rays[b].table) return rays[a].table < rays[b].table;
         return reach[(size_t)a] > reach[(size_t)b];
     });
+    // each station's Cartesian point once (a batch names few stations)
+    std::map<std::tuple<double, double, double>, std::tuple<float, float, float>> points;
     std::vector<fdray::RayIn> in((size_t)n);
     for (int r = 0; r < n; r++) {
         const fdgpu_ray &R = rays[r];
         fdray::RayIn &q = in[(size_t)r];
         q.ex = R.ex; q.ey = R.ey; q.ez = R.ez;
         q.xs = R.xs; q.ys = R.ys; q.zs = R.zs;
-        fdray::station_point(R.xs, R.ys, R.zs, q.x2, q.y2, q.z2);
+        const auto key = std::make_tuple(R.xs, R.ys, R.zs);
+        auto pt = points.find(key);
+        if (pt == points.end()) {
+            float x2, y2, z2;
+            fdray::station_point(R.xs, R.ys, R.zs, x2, y2, z2);
+            pt = points.emplace(key, std::make_tuple(x2, y2, z2)).first;
+        }
+        std::tie(q.x2, q.y2, q.z2) = pt->second;
         q.table = slot[R.table];
     }
     const std::vector<double> trig = fdray::face_trig(G.nx, G.ny, G.x0, G.y0, G.df, G.dq);
@@ -99,13 +109,16 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     const int nvar = G.ido1d ? G.nzc : G.nxc * G.nyc * G.nzc;
 
     // lanes per launch: the columns within their share of the budget
-    const size_t left = c->budget > c->used ? (size_t)(c->budget - c->used) : 0;
+    // (the cached columns of an earlier call count as available)
+    const size_t left = (c->budget > c->used ? (size_t)(c->budget - c->used) : 0) + c->ray_cols_bytes;
     const size_t col_budget = std::min(kColumnBytes, left / 4);
     long long lanes = (long long)(col_budget / (sizeof(float) * (size_t)nvar + sizeof(unsigned) * (size_t)((nvar + 31) / 32)));
     lanes = std::min<long long>(lanes, n);
     if (lanes < 1) return FDGPU_ENOMEM;
     if (lanes > 256) lanes = lanes / 256 * 256;
-    const long long stride = (lanes + 63) / 64 * 64;
+    // each lane's row of derivatives and of touched bits, 128-byte aligned
+    const long long dstride = ((long long)nvar + 31) / 32 * 32;
+    const long long bstride = (((long long)nvar + 31) / 32 + 31) / 32 * 32;
     // packed derivatives of a launch: about 512 per ray, never fewer than one ray's worst case
     const long long capacity = std::max<long long>(nvar, std::min<long long>(lanes * 512, 1LL << 26));
 
@@ -115,13 +128,23 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     const size_t b_p = align256(sizeof(const float *) * tabs.size());
     const size_t b_r = align256(sizeof(fdray::RayIn) * (size_t)lanes);
     const size_t b_o = align256(sizeof(RayResult) * (size_t)lanes);
-    const size_t b_d = align256(sizeof(float) * (size_t)nvar * (size_t)stride);
-    const size_t b_b = align256(sizeof(unsigned) * (size_t)((nvar + 31) / 32) * (size_t)stride);
+    const size_t b_d = align256(sizeof(float) * (size_t)dstride * (size_t)lanes);
+    const size_t b_b = align256(sizeof(unsigned) * (size_t)bstride * (size_t)lanes);
     const size_t b_c = 256;
     const size_t b_i = align256(sizeof(int) * (size_t)capacity);
     const size_t b_v = align256(sizeof(float) * (size_t)capacity);
+    // the lane columns: cached in the context between calls (every launch
+    // leaves them zero), grown when a call needs more
+    bool ok = true;
+    if (c->ray_cols_bytes < b_d + b_b) {
+        release_ray_columns(c);
+        int rc = dev_alloc(c, (void **)&c->ray_cols, b_d + b_b);
+        if (rc) return rc;
+        c->ray_cols_bytes = b_d + b_b;
+        ok = hipMemsetAsync(c->ray_cols, 0, c->ray_cols_bytes, c->stream) == hipSuccess;
+    }
     char *blk = nullptr;
-    int rc = dev_alloc(c, (void **)&blk, b_g + b_t + b_m + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
+    int rc = dev_alloc(c, (void **)&blk, b_g + b_t + b_m + b_p + b_r + b_o + b_c + b_i + b_v);
     if (rc) return rc;
     size_t off = 0;
     auto carve = [&](size_t b) {
@@ -135,8 +158,8 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     const float **d_tabs = (const float **)carve(b_p);
     fdray::RayIn *d_in = (fdray::RayIn *)carve(b_r);
     RayResult *d_res = (RayResult *)carve(b_o);
-    float *d_du = (float *)carve(b_d);
-    unsigned *d_bits = (unsigned *)carve(b_b);
+    float *d_du = (float *)c->ray_cols;
+    unsigned *d_bits = (unsigned *)(c->ray_cols + b_d);
     unsigned long long *d_cursor = (unsigned long long *)carve(b_c);
     int *d_idx = (int *)carve(b_i);
     float *d_val = (float *)carve(b_v);
@@ -149,15 +172,15 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
                       G.max_segments > 0 ? G.max_segments : 8 * (G.nx + G.ny + G.nz),
                       G.x0, G.y0, G.z0, G.df, G.dq, G.h, d_g, d_g + G.nxc, d_g + G.nxc + G.nyc, d_trig,
                       d_guess, d_guess + G.nx, d_guess + G.nx + G.ny};
-    bool ok = hipMemcpyAsync(d_g, gxyz.data(), sizeof(float) * gxyz.size(), hipMemcpyHostToDevice, c->stream) ==
+    ok = ok &&
+         hipMemcpyAsync(d_g, gxyz.data(), sizeof(float) * gxyz.size(), hipMemcpyHostToDevice, c->stream) ==
                   hipSuccess &&
               hipMemcpyAsync(d_trig, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, c->stream) ==
                   hipSuccess &&
               hipMemcpyAsync(d_guess, guess.data(), sizeof(int) * guess.size(), hipMemcpyHostToDevice, c->stream) ==
                   hipSuccess &&
               hipMemcpyAsync(d_tabs, tabs.data(), sizeof(const float *) * tabs.size(), hipMemcpyHostToDevice,
-                             c->stream) == hipSuccess &&
-              hipMemsetAsync(d_du, 0, b_d + b_b, c->stream) == hipSuccess;
+                             c->stream) == hipSuccess;
 
     // each ray's result and derivatives, as the launches deliver them
     std::vector<fdray::RayOut> outs((size_t)n);
@@ -180,8 +203,8 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
                                 c->stream) == hipSuccess &&
                  hipMemsetAsync(d_cursor, 0, sizeof(unsigned long long), c->stream) == hipSuccess &&
                  hipEventRecord(c->ev0, c->stream) == hipSuccess &&
-                 fd_launch_rays(&dg, d_in, d_tabs, m, d_du, d_bits, nvar, stride, d_res, d_cursor, d_idx, d_val,
-                                capacity, c->stream) == hipSuccess &&
+                 fd_launch_rays(&dg, d_in, d_tabs, m, d_du, d_bits, nvar, dstride, bstride, d_res, d_cursor, d_idx,
+                                d_val, capacity, c->stream) == hipSuccess &&
                  hipEventRecord(c->ev1, c->stream) == hipSuccess;
             got.resize((size_t)m);
             unsigned long long used = 0;
@@ -215,8 +238,11 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         }
         pending.swap(retry);
     }
-    dev_free(c, blk, b_g + b_t + b_m + b_p + b_r + b_o + b_d + b_b + b_c + b_i + b_v);
-    if (!ok) return FDGPU_EHIP;
+    dev_free(c, blk, b_g + b_t + b_m + b_p + b_r + b_o + b_c + b_i + b_v);
+    if (!ok) {
+        release_ray_columns(c);   // a failed launch may have left them dirty
+        return FDGPU_EHIP;
+    }
     c->last_ms = ms_total;
 
     // results in input order; the derivatives packed in the same order
@@ -230,7 +256,8 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         q.first_in_cube = o.first_in_cube;
         q.tev = o.tev;
         q.dtdx = o.dtdx; q.dtdy = o.dtdy; q.dtdz = o.dtdz;
-        q.ai = o.ai; q.az = o.az; q.rdev = o.rdev;
+        fdray::take_off(o, q.ai, q.az);
+        q.rdev = o.rdev;
         q.offset = total;
         total += o.ndu;
     }

@@ -74,10 +74,15 @@ struct RayIn {
     int table;   // index into the launch's table pointers
 };
 
+// sf0/sq0/sr0: the first step's unit gradient direction (longitude,
+// colatitude, radius), from which the host computes the take-off angles with
+// its own atan2f (sphrayderv.c:1865-1873; the device's atan2f rounds
+// differently); not set when first_in_cube.
 struct RayOut {
     int status, nseg, ndu, first_in_cube;
     double tev;
-    float dtdx, dtdy, dtdz, ai, az, rdev;
+    float dtdx, dtdy, dtdz, rdev;
+    double sf0, sq0, sr0;
 };
 
 // The position's sine, cosine and arctangent: correctly rounded.
@@ -193,7 +198,7 @@ RAY_HD void ray_trace(const RayGrid &g, const float *__restrict__ t, const RayIn
     out.first_in_cube = 0;
     out.tev = 0;
     out.dtdx = out.dtdy = out.dtdz = 0.0f;
-    out.ai = out.az = __builtin_nanf("");
+    out.sf0 = out.sq0 = out.sr0 = 0.0;
     out.rdev = 0.0f;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long nxy = (long long)nx * ny, nxyz = nxy * nz;
@@ -331,9 +336,9 @@ RAY_HD void ray_trace(const RayGrid &g, const float *__restrict__ t, const RayIn
             if (in_cube) {
                 out.first_in_cube = 1;
             } else {
-                const double sfq = sqrt(sf * sf + sq * sq);
-                out.az = (float)(atan2f((float)sf, (float)-sq) / kDegRad);
-                out.ai = (float)(atan2f((float)sfq, (float)-sr) / kDegRad);
+                out.sf0 = sf;
+                out.sq0 = sq;
+                out.sr0 = sr;
             }
         }
         // distances to the cell's surfaces of constant radius, colatitude,

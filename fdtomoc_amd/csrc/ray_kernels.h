@@ -16,12 +16,13 @@ struct RayResult {
 };
 
 // Traces rays [0, n): lane r traces ray r through tables[rays[r].table],
-// accumulating its derivatives in column r of du (nvar rows of `stride`
-// floats) with a bit per touched variable in column r of bits ((nvar + 31)
-// / 32 rows of `stride` words), both all zero on entry and on return, then
+// accumulating its derivatives in row r of du (rows of dstride >= nvar
+// floats) with a bit per touched variable in row r of bits (rows of bstride
+// >= (nvar + 31) / 32 words), both all zero on entry and on return, then
 // appends its nonzero derivatives in index order to (idx, val) at an offset
 // taken from *cursor, up to `capacity` entries in all.
 extern "C" hipError_t fd_launch_rays(const fdray::RayGrid *grid, const fdray::RayIn *rays, const float *const *tables,
-                                     int n, float *du, unsigned *bits, int nvar, long long stride, RayResult *results,
+                                     int n, float *du, unsigned *bits, int nvar, long long dstride, long long bstride,
+                                     RayResult *results,
                                      unsigned long long *cursor, int *idx, float *val, long long capacity,
                                      hipStream_t stream);

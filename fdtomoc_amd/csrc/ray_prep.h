@@ -57,4 +57,18 @@ inline std::vector<int> coarse_guess(const float *g, int n, double g0, double st
     return m;
 }
 
+// The take-off angles of a ray (sphrayderv.c:1865-1873): azimuth and
+// incidence in degrees from its first step's direction, with the host's
+// atan2f as the reference computes them; NaN when the first step ran in the
+// station cube (the reference then reports its previous ray's values).
+inline void take_off(const RayOut &o, float &ai, float &az) {
+    if (o.first_in_cube || o.status == kRayEventOut || o.status == kRayUnreached || o.nseg == 0) {
+        ai = az = __builtin_nanf("");
+        return;
+    }
+    const double sfq = sqrt(o.sf0 * o.sf0 + o.sq0 * o.sq0);
+    az = (float)(atan2f((float)o.sf0, (float)-o.sq0) / kDegRad);
+    ai = (float)(atan2f((float)sfq, (float)-o.sr0) / kDegRad);
+}
+
 }  // namespace fdray

@@ -143,3 +143,258 @@ def epoch(year: int, day: int, hour: int, minute: int, sec: float) -> float:
             days -= 365 + leap(y)
     days += day - 1
     return float(days * 86400) + 3600.0 * hour + 60.0 * minute + sec
+
+
+def date_of(epoch_s: float) -> tuple:
+    """etoh (lib/common/time_process.c:11-84): (year, day of year, hour,
+    minute, seconds) of an epoch time."""
+    doy = int(epoch_s / 86400.0)
+    secleft = math.fmod(epoch_s, 86400.0)
+    hour = minute = 0
+    second = 0.0
+    if secleft != 0.0:
+        if secleft < 0:
+            doy -= 1
+            secleft += 86400
+        hour = int(secleft / 3600)
+        secleft = math.fmod(secleft, 3600.0)
+        minute = int(secleft / 60)
+        second = math.fmod(secleft, 60.0)
+
+    def leap(y):
+        return (y % 4 == 0 and y % 100 != 0) or y % 400 == 0
+    if doy >= 0:
+        year = 1970
+        while doy >= 365 + leap(year):
+            doy -= 365 + leap(year)
+            year += 1
+    else:
+        year = 1969
+        while doy < 0:
+            doy += 365 + leap(year)
+            year -= 1
+    return year, doy + 1, hour, minute, second
+
+
+# --------------------------------------------------- sphrayderv's input files
+
+def read_spec(path: str) -> dict:
+    """The spec keys sphrayderv reads (sphrayderv.c:198-480): the first
+    token of a line is the key, the next its value; igridx/igridy/igridz
+    carry one value per coarse node."""
+    vals, rows = {}, {}
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if len(tok) < 2:
+                continue
+            if tok[0] in ("igridx", "igridy", "igridz"):
+                rows.setdefault(tok[0], [int(t) for t in tok[1:]])
+            else:
+                vals.setdefault(tok[0], tok[1])
+    vals.update(rows)
+    return vals
+
+
+def read_station_list(path: str) -> list:
+    """Station-list lines `sty stx ielev name slat slon tcorP tcorS`
+    (sphrayderv.c:853-880): Station records (slat/slon are the positions
+    used) in file order."""
+    out = []
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if len(tok) < 8:
+                continue
+            out.append(station(tok[3], float(tok[4]), float(tok[5]), int(tok[2]), (float(tok[6]), float(tok[7]))))
+    return out
+
+
+def read_events(path: str) -> list:
+    """Event blocks (sphrayderv.c:914-1044): a header `year day hour minute
+    sec lat lon depth evid`, pick lines `sta year day hour minute sec phase
+    rwt` (a leading '*' before the phase is dropped), a blank line."""
+    events, cur = [], None
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                if cur is not None:
+                    events.append(cur)
+                    cur = None
+                continue
+            if cur is None:
+                cur = dict(time=(int(tok[0]), int(tok[1]), int(tok[2]), int(tok[3]), float(np.float32(tok[4]))),
+                           lat=float(tok[5]), lon=float(tok[6]), depth=float(np.float32(tok[7])), evid=tok[8],
+                           picks=[])
+                continue
+            rest = line.split(None, 6)[6] if len(line.split(None, 6)) > 6 else ""
+            rest = rest.strip()
+            if rest.startswith("*"):
+                rest = rest[1:]
+            rt = rest.split()
+            cur["picks"].append(dict(sta=tok[0], time=(int(tok[1]), int(tok[2]), int(tok[3]), int(tok[4]),
+                                                       float(np.float32(tok[5]))),
+                                     phase=rt[0][0] if rt else " ", rwt=float(np.float32(rt[1])) if len(rt) > 1 else 0.0))
+    if cur is not None:
+        events.append(cur)
+    return events
+
+
+# ------------------------------------------------------- the program's files
+
+def run(spec_path: str, trace, workdir: str = ".") -> dict:
+    """sphrayderv's local-earthquake run (src/sphrayderv.c main with its
+    defaults: local events, no shots or teleseisms, ivpvs 0, matrices and
+    data files written) with the rays traced by `trace` and the reference's
+    bookkeeping and files: dtdsfil (binary rows of weighted derivatives),
+    resfile (weighted residuals), dtdhfil (hypocentre partials), bookfil,
+    hitfile, sclefil, dotfile and headfil.  The error and log text files
+    (statistics and warnings) are not written.
+
+    trace(grid, keys, rays) -> list of dicts (status, tev, dtdx, dtdy, dtdz,
+    ai, az, idx, val): keys[r] = (station, phase letter of its table file),
+    rays[r] = (ex, ey, ez, xs, ys, zs).  Returns counters."""
+    import os
+    import struct
+    sp = read_spec(os.path.join(workdir, spec_path))
+    for key, default in (("dotel", 0), ("doshot", 0), ("ivpvs", 0), ("nomat", 0), ("iray", 0), ("iraystat", 0),
+                         ("istacor", 0)):
+        if int(sp.get(key, default)) not in (0,):
+            raise NotImplementedError(f"{key} = 1 (only local earthquakes with the default outputs)")
+    idatout = int(sp.get("idatout", 1)) == 1
+    ivs = 0 if int(sp.get("ivs", 1)) != 1 else 1
+    resflag = float(np.float32(sp.get("resflag", 1.0)))
+    nxc, nyc, nzc = int(sp["nxc"]), int(sp["nyc"]), int(sp["nzc"])
+    grid = grid_from_spec(nxc, nyc, nzc, sp["igridx"], sp["igridy"], sp["igridz"], float(sp.get("x0", 0.0)),
+                          float(sp.get("y0", 0.0)), float(sp.get("z0", 0.0)), float(sp["h"]),
+                          float(sp.get("dq", 0.0)), float(sp.get("df", 0.0)))
+    maxvarc = nxc * nyc * nzc
+    sta = read_station_list(os.path.join(workdir, sp["stafile"]))
+    by_name = {}
+    for kn, s in enumerate(sta):
+        by_name.setdefault(s.name, kn)
+    events = read_events(os.path.join(workdir, sp["locdfil"]))
+    f32 = np.float32
+    # trace every pick of every in-grid event at once
+    keys, rays, owner = [], [], []
+    for e, ev in enumerate(events):
+        ex, ey, ez = event_position(ev["lat"], ev["lon"], ev["depth"])
+        ies, jes, kes = int((ex - grid.x0) / grid.df), int((ey - grid.y0) / grid.dq), int((ez - grid.z0) / grid.h)
+        ev["pos"] = (ex, ey, ez)
+        ev["inside"] = not (ex < grid.x0 or ies >= grid.nx - 1 or ey < grid.y0 or jes >= grid.ny - 1 or ez < grid.z0
+                            or kes >= grid.nz - 1)
+        if not ev["inside"]:
+            continue
+        for p, pk in enumerate(ev["picks"]):
+            kn = by_name.get(pk["sta"])
+            if kn is None:
+                continue
+            s = sta[kn]
+            keys.append((pk["sta"], "S" if (ivs and pk["phase"] == "S") else "P"))
+            rays.append((ex, ey, ez, s.lon, s.colat, s.z))
+            owner.append((e, p))
+    traced = dict(zip(owner, trace(grid, keys, rays))) if rays else {}
+
+    nhit, indx, jndx = {}, {}, []
+    kbl, mhit, mndex, mndm = 0, set(), {}, []
+    files = {k: open(os.path.join(workdir, sp[k]), "wb") for k in ("dtdsfil", "resfile", "dtdhfil", "bookfil")}
+    dot = open(os.path.join(workdir, sp["dotfile"]), "w") if idatout else None
+    hed = open(os.path.join(workdir, sp["headfil"]), "w") if idatout else None
+    nev = nwritten = 0
+    for e, ev in enumerate(events):
+        nev += 1
+        if not ev["inside"]:
+            continue
+        ex, ey, ez = ev["pos"]
+        dpot = epoch(*ev["time"])
+        rows, avresev, facsev = [], f32(0), f32(0)
+        for p, pk in enumerate(ev["picks"]):
+            pwt = f32(f32(1.0) / f32(f32(pk["rwt"]) * f32(pk["rwt"])))
+            obstime = f32(epoch(*pk["time"]) - dpot)
+            row = dict(pick=pk, good=False, obstime=obstime, pwt=pwt)
+            rows.append(row)
+            kn = by_name.get(pk["sta"])
+            r = traced.get((e, p))
+            if kn is None or r is None or r["status"] == 0:
+                continue
+            iph = 1 if pk["phase"] == "S" else 0
+            tc = f32(sta[kn].tcor[iph])
+            dt = float(obstime) - ((r["tev"] + 0.0) + float(tc))
+            wet = np.sqrt(pwt, dtype=np.float32)
+            res1 = f32(dt * float(pwt))
+            avresev = f32(avresev + res1)
+            facsev = f32(facsev + pwt)
+            row.update(resmin=f32(dt), dat=f32(dt * float(wet)),
+                       am=[wet, f32(f32(r["dtdx"]) * wet), f32(f32(r["dtdy"]) * wet), f32(f32(r["dtdz"]) * wet)],
+                       az=r["az"], ai=r["ai"])
+            if r["status"] != 1:
+                continue
+            row["good"] = True
+            if p == 0:
+                kbl, mhit, mndex, mndm = 0, set(), {}, []
+            vm = []
+            for jb, du in zip(r["idx"], r["val"]):
+                j = int(jb) + (maxvarc if iph == 1 else 0)
+                vm.append((j, f32(f32(du) * wet)))
+                if j not in mhit:
+                    mhit.add(j)
+                    mndex[j] = kbl
+                    mndm.append(j)
+                    kbl += 1
+                if nhit.get(j, 0) == 0:
+                    indx[j] = len(jndx)
+                    jndx.append(j)
+                nhit[j] = nhit.get(j, 0) + 1
+            row["vm"] = vm
+        avresev = f32(avresev / facsev) if facsev != 0 else f32(np.nan)
+        nwrt = 0
+        for row in rows:
+            if not row["good"]:
+                continue
+            vmp = {}
+            for j, v in row["vm"]:
+                vmp[mndex[j]] = f32(vmp.get(mndex[j], f32(0)) + v)
+            cols = [i1 for i1 in range(kbl) if vmp.get(i1, 0) != 0]
+            if not cols:
+                continue
+            out = [struct.pack("<i", len(cols))]
+            for i1 in cols:
+                out.append(struct.pack("<i", indx[mndm[i1]] + 1) + np.float32(vmp[i1]).tobytes())
+            files["dtdsfil"].write(b"".join(out))
+            files["resfile"].write(np.float32(row["dat"]).tobytes())
+            files["dtdhfil"].write("".join("%f\n" % float(a) for a in row["am"]).encode())
+            nwrt += 1
+        files["bookfil"].write(f"{nev} {nwrt}\n".encode())
+        nwritten += nwrt
+        if idatout:
+            iyr, jday, ihr, imn, dsec = date_of(dpot)
+            sec = float(f32(dsec))
+            xlon = ex / DEGRAD
+            xlat = glatinv(HPI - ey) / DEGRAD
+            head = "%4d %3d %2d %2d %8.4f %9.5f %10.5f %8.4f %12s %13.3f\n" % (
+                iyr, jday, ihr, imn, sec, xlat, xlon, ez, ev["evid"], float(avresev))
+            hed.write(head)
+            dot.write(head)
+            for row in rows:
+                if not row["good"]:
+                    continue
+                pk = row["pick"]
+                iyr, jday, ihr, imn, dsec = date_of(float(row["obstime"]) + dpot)
+                mark = "*" if abs(float(row["resmin"])) > resflag else " "
+                dot.write("%s  %4d %3d %2d %2d %7.3f %c %15.3f %10.3f %c %f %f\n" % (
+                    pk["sta"], iyr, jday, ihr, imn, dsec, pk["phase"], pk["rwt"], float(row["resmin"]), mark,
+                    float(row["az"]), float(row["ai"])))
+            dot.write("\n")
+    files["dtdsfil"].write(struct.pack("<ii", -1, len(jndx)) + np.asarray(jndx, dtype="<i4").tobytes())
+    files["bookfil"].write(b"-2 -2\n")
+    for f in files.values():
+        f.close()
+    with open(os.path.join(workdir, sp["sclefil"]), "w") as f:
+        f.write(f"{len(jndx)}\n" + "".join(f"{j}\n" for j in jndx))
+    with open(os.path.join(workdir, sp["hitfile"]), "w") as f:
+        f.write(f"{len(jndx)}" + "".join(f"{j} {nhit[j]}\n" for j in jndx))
+    if idatout:
+        dot.close()
+        hed.close()
+    return dict(events=nev, rays=len(rays), written=nwritten, variables=len(jndx))

@@ -10,7 +10,9 @@ so that a test can regenerate and check them), and writes its files; stored
 here per written ray: the weighted derivatives with their variables
 (dtdsfil, float bits), the weighted residual (resfile, float bits), the four
 hypocentre derivatives (dtdhfil, six decimals), and the residual, azimuth and
-incidence the data file prints.  Only data is stored.
+incidence the data file prints; and the program's output files themselves
+(FILES, byte for byte), for the file-level comparison of
+fdtomoc_amd.rayderv.run.  Only data is stored.
 """
 from __future__ import annotations
 
@@ -29,6 +31,7 @@ import ray_cases as rc  # noqa: E402
 
 REF_RAY = os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle", "_ref", "sphrayderv_ref")
 SEEDS = (11, 12, 13)
+FILES = ("dtds.bin", "res.bin", "dtdh.txt", "book.txt", "hit.txt", "scle.txt", "dot.txt", "head.txt")
 
 
 def run_reference(case, tabs, wd):
@@ -36,7 +39,9 @@ def run_reference(case, tabs, wd):
     p = subprocess.run([REF_RAY], input=spec + "\n", cwd=wd, capture_output=True, text=True, timeout=300)
     if p.returncode != 0:
         raise RuntimeError(f"reference failed ({p.returncode}): {p.stdout[-2000:]} {p.stderr[-2000:]}")
-    return rc.read_outputs(wd)
+    out = rc.read_outputs(wd)
+    out["files"] = {f: open(os.path.join(wd, f), "rb").read() for f in FILES}
+    return out
 
 
 def main():
@@ -62,6 +67,8 @@ def main():
         out[pre + "du_idx"] = np.concatenate([w[0] for w in r["written"]]).astype(np.int32)
         out[pre + "du_val"] = np.concatenate([w[1] for w in r["written"]]).astype(np.float32)
         out[pre + "book"] = r["book"]
+        for f in FILES:
+            out[pre + "file_" + f] = np.frombuffer(r["files"][f], dtype=np.uint8)
         print(f"seed {seed}: {len(case.events)} events, {sum(len(e[9]) for e in case.events)} picks, "
               f"{len(good)} good, {len(r['written'])} written, {ptr[-1]} derivatives", flush=True)
     np.savez_compressed(os.path.join(HERE, "ray_cases.npz"), **out)

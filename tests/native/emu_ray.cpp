@@ -64,7 +64,8 @@ extern "C" int emu_ray_trace_many(const oracle_ray_grid *g, const float *const *
         q.first_in_cube = o.first_in_cube;
         q.tev = o.tev;
         q.dtdx = o.dtdx; q.dtdy = o.dtdy; q.dtdz = o.dtdz;
-        q.ai = o.ai; q.az = o.az; q.rdev = o.rdev;
+        fdray::take_off(o, q.ai, q.az);
+        q.rdev = o.rdev;
         int k = 0;
         if (o.status == fdray::kRayOk)
             for (int v = 0; v < nvar; v++)

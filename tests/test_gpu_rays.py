@@ -16,6 +16,8 @@ float derived from such a position can then differ by a unit
 arctangents of the device library)."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -133,3 +135,41 @@ def test_small_output_arrays_report_sizes(ctx):
     assert np.array_equal(idx, idx2) and np.array_equal(val.view(np.uint32), val2.view(np.uint32))
     for i in ids.values():
         ctx.release(i)
+
+
+def _device_trace(ctx, workdir, timedir="ttimes"):
+    """A trace function for fdtomoc_amd.rayderv.run: the run's table files
+    loaded as resident tables (fdgpu_table_load, as the reference reads
+    them), the rays traced on the device."""
+    from fdtomoc_amd import fdgpu
+    ids = {}
+
+    def trace(grid, keys, rays):
+        for k in set(keys):
+            path = os.path.join(workdir, timedir, f"{k[0]}.{'p' if k[1] == 'P' else 's'}times")
+            ids[k] = ctx.load_table(path, (grid.nz, grid.ny, grid.nx))
+        arr = fdgpu.ray_array([ids[k] for k in keys], rays)
+        res, idx, val = ctx.trace_rays(grid, arr)
+        return [dict(status=int(d["status"]), tev=float(d["tev"]), dtdx=float(d["dtdx"]), dtdy=float(d["dtdy"]),
+                     dtdz=float(d["dtdz"]), ai=float(d["ai"]), az=float(d["az"]),
+                     idx=idx[d["offset"]:d["offset"] + d["ndu"]], val=val[d["offset"]:d["offset"] + d["ndu"]])
+                for d in res]
+    return trace, ids
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_program_files_with_device_rays(ctx, tmp_path, seed):
+    """sphrayderv's program (fdtomoc_amd.rayderv.run) with the rays traced on
+    the device through tables loaded from the run's files: its eight output
+    files against the reference program's for the same inputs
+    (tests/golden/ray_cases.npz), byte for byte."""
+    from fdtomoc_amd import rayderv
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ray_cases.npz"))
+    case = rc.ray_case(seed)
+    spec = rc.write_inputs(case, str(tmp_path), rc.case_tables(case))
+    trace, ids = _device_trace(ctx, str(tmp_path))
+    rayderv.run(spec, trace, workdir=str(tmp_path))
+    for i in ids.values():
+        ctx.release(i)
+    for f in ("dtds.bin", "res.bin", "dtdh.txt", "book.txt", "hit.txt", "scle.txt", "dot.txt", "head.txt"):
+        assert open(tmp_path / f, "rb").read() == golden[f"c{seed}_file_{f}"].tobytes(), f

@@ -930,6 +930,9 @@ def run_loc(args):
 # Per segment, the eight corner times of the station's table a step reads
 # (the algorithmic bytes of the ray kernel; derivatives add 8 bytes each).
 RAY_BYTES_PER_SEGMENT = 32
+# HBM bytes of one ray launch of the rays workload (FETCH_SIZE x 1.99992 +
+# WRITE_SIZE, copy-calibrated) (profiles/ray_kernel_rays_96000.md)
+TRAFFIC_BYTES_RAYS = 18.06e9
 
 
 def ray_workload(events: int, seed: int = 9):
@@ -1007,7 +1010,9 @@ def run_rays(args):
                              "max": int(res["nseg"].max())},
         "segments_per_s": round(nseg / kern, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "ray_kernel",
+                     "frac": round(ach / HBM_PEAK_GBS, 6),
+                     "traffic": TRAFFIC_BYTES_RAYS if len(evs) == 1000 else None,
+                     "traffic_source": "profiles/ray_kernel_rays_96000.md", "kernel": "ray_kernel",
                      "kernel_ms": round(kern * 1e3, 3), "algorithmic_bytes": alg,
                      "what": "per step: 32 bytes of table corners per segment + 8 bytes per derivative; the walk "
                              "is a chain of dependent FP64 steps per lane, so latency, not bandwidth, bounds it"},

@@ -398,3 +398,51 @@ def run(spec_path: str, trace, workdir: str = ".") -> dict:
         dot.close()
         hed.close()
     return dict(events=nev, rays=len(rays), written=nwritten, variables=len(jndx))
+
+
+def device_trace(ctx, workdir: str, timedir: str):
+    """A trace function for `run` on the device: each station-phase table
+    file of the run loaded once as a resident table (fdgpu_table_load, read
+    as the reference reads it, sphrayderv.c:1163-1217), every ray traced in
+    one fdgpu_rays_trace call; returns the table ids it loaded too."""
+    import os
+    from . import fdgpu
+    ids = {}
+
+    def trace(grid, keys, rays):
+        for k in sorted(set(keys)):
+            if k not in ids:
+                path = os.path.join(workdir, timedir, f"{k[0]}.{'p' if k[1] == 'P' else 's'}times")
+                ids[k] = ctx.load_table(path, (grid.nz, grid.ny, grid.nx))
+        res, idx, val = ctx.trace_rays(grid, fdgpu.ray_array([ids[k] for k in keys], rays))
+        return [dict(status=int(d["status"]), tev=float(d["tev"]), dtdx=float(d["dtdx"]), dtdy=float(d["dtdy"]),
+                     dtdz=float(d["dtdz"]), ai=float(d["ai"]), az=float(d["az"]),
+                     idx=idx[d["offset"]:d["offset"] + d["ndu"]], val=val[d["offset"]:d["offset"] + d["ndu"]])
+                for d in res]
+    return trace, ids
+
+
+def main() -> int:
+    """`python -m fdtomoc_amd.rayderv` reads the spec file's name on stdin, as
+    the reference program does (sphrayderv.c:191-196), and runs it with the
+    rays on the device."""
+    import sys
+    from . import fdgpu
+    spec = sys.stdin.readline().strip()
+    if not spec:
+        print("sphrayderv: no spec file named on stdin", file=sys.stderr)
+        return 1
+    timedir = read_spec(spec).get("timedir", "./ttimes")
+    ctx = fdgpu.Context(0)
+    try:
+        trace, ids = device_trace(ctx, ".", timedir)
+        info = run(spec, trace, ".")
+    finally:
+        ctx.close()
+    print(f"sphrayderv: {info['events']} events, {info['rays']} rays, {info['written']} rows, "
+          f"{info['variables']} variables")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

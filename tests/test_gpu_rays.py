@@ -137,26 +137,6 @@ def test_small_output_arrays_report_sizes(ctx):
         ctx.release(i)
 
 
-def _device_trace(ctx, workdir, timedir="ttimes"):
-    """A trace function for fdtomoc_amd.rayderv.run: the run's table files
-    loaded as resident tables (fdgpu_table_load, as the reference reads
-    them), the rays traced on the device."""
-    from fdtomoc_amd import fdgpu
-    ids = {}
-
-    def trace(grid, keys, rays):
-        for k in set(keys):
-            path = os.path.join(workdir, timedir, f"{k[0]}.{'p' if k[1] == 'P' else 's'}times")
-            ids[k] = ctx.load_table(path, (grid.nz, grid.ny, grid.nx))
-        arr = fdgpu.ray_array([ids[k] for k in keys], rays)
-        res, idx, val = ctx.trace_rays(grid, arr)
-        return [dict(status=int(d["status"]), tev=float(d["tev"]), dtdx=float(d["dtdx"]), dtdy=float(d["dtdy"]),
-                     dtdz=float(d["dtdz"]), ai=float(d["ai"]), az=float(d["az"]),
-                     idx=idx[d["offset"]:d["offset"] + d["ndu"]], val=val[d["offset"]:d["offset"] + d["ndu"]])
-                for d in res]
-    return trace, ids
-
-
 @pytest.mark.parametrize("seed", [11, 12, 13])
 def test_program_files_with_device_rays(ctx, tmp_path, seed):
     """sphrayderv's program (fdtomoc_amd.rayderv.run) with the rays traced on
@@ -167,9 +147,25 @@ def test_program_files_with_device_rays(ctx, tmp_path, seed):
     golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ray_cases.npz"))
     case = rc.ray_case(seed)
     spec = rc.write_inputs(case, str(tmp_path), rc.case_tables(case))
-    trace, ids = _device_trace(ctx, str(tmp_path))
+    trace, ids = rayderv.device_trace(ctx, str(tmp_path), "ttimes")
     rayderv.run(spec, trace, workdir=str(tmp_path))
     for i in ids.values():
         ctx.release(i)
     for f in ("dtds.bin", "res.bin", "dtdh.txt", "book.txt", "hit.txt", "scle.txt", "dot.txt", "head.txt"):
         assert open(tmp_path / f, "rb").read() == golden[f"c{seed}_file_{f}"].tobytes(), f
+
+
+def test_program_entry_point(tmp_path):
+    """`python -m fdtomoc_amd.rayderv <<< spec` in a run directory, as run.sh
+    calls the reference program: the same eight files as the reference's."""
+    import subprocess
+    import sys
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ray_cases.npz"))
+    case = rc.ray_case(12)
+    spec = rc.write_inputs(case, str(tmp_path), rc.case_tables(case))
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, "-m", "fdtomoc_amd.rayderv"], input=spec + "\n", cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300, env=dict(os.environ, PYTHONPATH=repo))
+    assert p.returncode == 0, p.stderr[-2000:]
+    for f in ("dtds.bin", "res.bin", "dtdh.txt", "book.txt", "hit.txt", "scle.txt", "dot.txt", "head.txt"):
+        assert open(tmp_path / f, "rb").read() == golden[f"c12_file_{f}"].tobytes(), f

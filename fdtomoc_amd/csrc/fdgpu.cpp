@@ -59,13 +59,16 @@ void release_scratch(fdgpu_ctx *c) {
     c->slot_wave = false;
 }
 
-// The ray tracer's cached lane columns (regrown by its next call).
+// The ray tracer's cached device arrays (regrown by its next call).
 void release_ray_columns(fdgpu_ctx *c) {
-    if (!c->ray_cols) return;
+    if (!c->ray_cols && !c->ray_blk) return;
     (void)hipStreamSynchronize(c->stream);
     dev_free(c, c->ray_cols, c->ray_cols_bytes);
+    dev_free(c, c->ray_blk, c->ray_blk_bytes);
     c->ray_cols = nullptr;
     c->ray_cols_bytes = 0;
+    c->ray_blk = nullptr;
+    c->ray_blk_bytes = 0;
 }
 
 // Memory that outlives a launch (tables, models): when the budget is short,
@@ -73,7 +76,7 @@ void release_ray_columns(fdgpu_ctx *c) {
 // next launch or trace regrows them within what is left.
 int dev_alloc_resident(fdgpu_ctx *c, void **p, size_t bytes) {
     int rc = dev_alloc(c, p, bytes);
-    if (rc == FDGPU_ENOMEM && (c->d_scratch || c->ray_cols)) {
+    if (rc == FDGPU_ENOMEM && (c->d_scratch || c->ray_cols || c->ray_blk)) {
         release_scratch(c);
         release_ray_columns(c);
         rc = dev_alloc(c, p, bytes);
@@ -513,6 +516,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     dev_free(c, c->d_scratch, c->scratch_bytes);
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
     dev_free(c, c->ray_cols, c->ray_cols_bytes);
+    dev_free(c, c->ray_blk, c->ray_blk_bytes);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -194,8 +194,10 @@ struct fdgpu_ctx {
     long long slot_face_cap = 0;
     long long slot_node_cap = 0;   // per-slot change stamps: nodes of the largest table
     long long resolved_again = 0;   // tables solved again after a failed consistency check
-    char *ray_cols = nullptr;       // fdgpu_rays_trace's lane columns (kept zero between calls)
+    char *ray_cols = nullptr;       // fdgpu_rays_trace's lane rows (kept zero between calls)
     size_t ray_cols_bytes = 0;
+    char *ray_blk = nullptr;        // ... and its other device arrays, kept between calls
+    size_t ray_blk_bytes = 0;
     bool slot_wave = false;        // the slots hold wavefront state (sized by slot_node_cap)
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;

@@ -143,9 +143,17 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         c->ray_cols_bytes = b_d + b_b;
         ok = hipMemsetAsync(c->ray_cols, 0, c->ray_cols_bytes, c->stream) == hipSuccess;
     }
-    char *blk = nullptr;
-    int rc = dev_alloc(c, (void **)&blk, b_g + b_t + b_m + b_p + b_r + b_o + b_c + b_i + b_v);
-    if (rc) return rc;
+    const size_t blk_bytes = b_g + b_t + b_m + b_p + b_r + b_o + b_c + b_i + b_v;
+    if (c->ray_blk_bytes < blk_bytes) {
+        (void)hipStreamSynchronize(c->stream);
+        dev_free(c, c->ray_blk, c->ray_blk_bytes);
+        c->ray_blk = nullptr;
+        c->ray_blk_bytes = 0;
+        int rc = dev_alloc(c, (void **)&c->ray_blk, blk_bytes);
+        if (rc) return rc;
+        c->ray_blk_bytes = blk_bytes;
+    }
+    char *blk = c->ray_blk;
     size_t off = 0;
     auto carve = [&](size_t b) {
         char *p = blk + off;
@@ -238,7 +246,6 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         }
         pending.swap(retry);
     }
-    dev_free(c, blk, b_g + b_t + b_m + b_p + b_r + b_o + b_c + b_i + b_v);
     if (!ok) {
         release_ray_columns(c);   // a failed launch may have left them dirty
         return FDGPU_EHIP;

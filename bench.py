@@ -221,13 +221,17 @@ def roofline(visits, kern_s, traffic=None, source=None):
 def tune(ctx, specs, args, sample=None):
     """Per-face schedule picked by the autotuner before timing (untimed, like
     the warmup); the tables are identical for every choice."""
+    if args.team >= 0 or args.team_share >= 0:
+        size, share = ctx.get_team()
+        ctx.set_team(args.team if args.team >= 0 else size, args.team_share if args.team_share >= 0 else share)
     if args.wave >= 0:
         ctx.set_wave(args.wave)
     if args.schedule >= 0:
         ctx.set_schedule(args.schedule)
-        return {"wave": ctx.get_wave(), "change_budget": args.schedule, "tuned": False}
+        return {"wave": ctx.get_wave(), "change_budget": args.schedule, "team": list(ctx.get_team()), "tuned": False}
     if args.no_autotune:
-        return {"wave": ctx.get_wave(), "change_budget": ctx.get_schedule(), "tuned": False}
+        return {"wave": ctx.get_wave(), "change_budget": ctx.get_schedule(), "team": list(ctx.get_team()),
+                "tuned": False}
     if args.wave >= 0:
         best, timings = autotune.tune_change_budget(ctx, specs, sample=sample)
         cand = {str(k): round(v, 3) for k, v in timings.items()}
@@ -235,7 +239,8 @@ def tune(ctx, specs, args, sample=None):
         (_, best), timings = autotune.tune_schedule(ctx, specs, sample=sample)
         cand = {k: round(v, 3) for k, v in timings.items()}
     print(f"# autotune wave {ctx.get_wave()} change_budget {best}: {cand}", file=sys.stderr, flush=True)
-    return {"wave": ctx.get_wave(), "change_budget": best, "tuned": True, "candidates_s": cand}
+    return {"wave": ctx.get_wave(), "change_budget": best, "team": list(ctx.get_team()), "tuned": True,
+            "candidates_s": cand}
 
 
 def timed_steps(ctx, specs, args, mode=fdgpu.BATCHED):
@@ -466,6 +471,7 @@ def run_c3(args):
     sched = tune(ctx, specs, args)
     wall, kern, last = timed_steps(ctx, specs, args)
     restarts = np.array([ctx.stats(i)["restarts"] for i in last])
+    team_sizes = np.array([ctx.stats(i)["team"] for i in last])
     visits = int(np.sum(sizes * (1 + restarts)))
     stations_s = len(specs) / float(wall.mean())
     rng = np.random.default_rng(33)
@@ -480,6 +486,7 @@ def run_c3(args):
                      "reverse_passes_mean": float(restarts.mean())})
     out["nodes_per_s"] = round(float(sizes.sum()) / float(wall.mean()), 1)
     out["schedule"] = sched
+    out["team_tables"] = int((team_sizes > 1).sum())
     out["plain_station_solves_per_s"] = None if plain_adj is None else round(plain_adj, 4)
     out["batched_over_plain"] = None if plain_adj is None else round(stations_s / plain_adj, 2)
     out["plain_sample"] = f"{len(psel)} random tables, one launch each, rate scaled by mean size"
@@ -1093,6 +1100,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-face schedule")
     ap.add_argument("--schedule", type=int, default=-1, help="set the per-face schedule (CHANGE budget) instead of tuning")
     ap.add_argument("--wave", type=int, default=-1, help="wavefront flags (1 first pass, 3 all passes) instead of tuning")
+    ap.add_argument("--team", type=int, default=-1, help="workgroups per large table's wavefront passes (0 auto, 1 never)")
+    ap.add_argument("--team-share", type=int, default=-1, help="teams for tables of at least this %% of the mean share")
     ap.add_argument("--requests", type=int, default=96, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=3.0, help="c4: mean arrivals per second")
     ap.add_argument("--tables", type=int, default=2000, help="c5: resident tables (1000 stations x P, S)")

@@ -36,12 +36,17 @@ extern "C" hipError_t fd_noalloc_scratch(void **ptr);
 // One table by one wide workgroup, descriptor and scratch layout by value.
 extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::Scratch *W, hipStream_t stream);
 
+// The persistent sweep over njobs tables on nslots workgroups; the first
+// nteam tables are each solved by a team of team_size workgroups (nteam 0:
+// none).
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
-                                      const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream);
+                                      const fdsweep::Scratch *d_slots, int nslots, int *d_queue,
+                                      fdsweep::TeamCtl *d_teams, int nteam, int team_size, hipStream_t stream);
 // Kernel address of a sweep build, for occupancy queries.
 extern "C" const void *fd_sweep_kernel_ptr(int variant);
-// Sets the table queue counter to 0 (a kernel, so that it can sit in a graph).
-extern "C" hipError_t fd_launch_reset(int *d_queue, hipStream_t stream);
+// Sets the table queue counter to 0 and clears the control words of nteam
+// team blocks (a kernel, so that it can sit in a graph).
+extern "C" hipError_t fd_launch_reset(int *d_queue, fdsweep::TeamCtl *d_teams, int nteam, hipStream_t stream);
 // Fills the geometry table of one grid (p->sinq is a device pointer).
 extern "C" hipError_t fd_launch_geo(const fdsweep::GeoParams *p, double *d_out, hipStream_t stream);
 extern "C" hipError_t fd_launch_slowness(const float *d_vel, float *d_slow, long long n, hipStream_t stream);

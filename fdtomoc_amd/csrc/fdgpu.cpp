@@ -94,9 +94,9 @@ size_t slot_bytes(long long F, long long N, int D, bool wave) {
     size_t b = 15 * f + align256((size_t)F);
     if (N > 0) {
         b += align256(sizeof(int) * (size_t)N) + align256(sizeof(int) * 6 * (size_t)D);
-        // wavefront state: status, t[0], t[4], counter per node, two lists, the round word
+        // wavefront state: status, t[0], t[4], counter per node, three lists, the round word
         if (wave)
-            b += 4 * align256(sizeof(int) * (size_t)N) + 2 * align256(sizeof(int) * (size_t)wave_list_cap(N)) + 256;
+            b += 4 * align256(sizeof(int) * (size_t)N) + 3 * align256(sizeof(int) * (size_t)wave_list_cap(N)) + 256;
     }
     return b;
 }
@@ -129,7 +129,7 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long 
         W.mstamp = nullptr;
         W.lastf = nullptr;
         W.wst = nullptr; W.wt0 = nullptr; W.wt4 = nullptr;
-        W.wcnt = nullptr; W.wtmp = nullptr; W.wready = nullptr;
+        W.wcnt = nullptr; W.wtmp = nullptr; W.wready = nullptr; W.wclose = nullptr;
         W.wcap = 0; W.wn = 0; W.wround = nullptr;
         if (N > 0) {
             W.mstamp = (int *)take(align256(sizeof(int) * (size_t)N));
@@ -143,6 +143,7 @@ int fill_slots(char *base, int ns, long long F, fdsweep::Scratch *d_slots, long 
             W.wcnt = (int *)take(nb);
             W.wtmp = (int *)take(lb);
             W.wready = (int *)take(lb);
+            W.wclose = (int *)take(lb);
             W.wcap = wave_list_cap(N);
             W.wn = N;
             W.wround = (unsigned *)take(256);
@@ -277,17 +278,39 @@ int ensure_jobs(fdgpu_ctx *c, int n) {
     return FDGPU_OK;
 }
 
+// Workgroups per team when the context leaves the choice automatic.
+constexpr int kTeamAuto = 4;
+
+// Team blocks for nteam teams (the captured launches keep their address, so
+// growing them drops the graphs).
+int ensure_teams(fdgpu_ctx *c, int nteam) {
+    if (nteam <= c->teams_cap) return FDGPU_OK;
+    (void)hipStreamSynchronize(c->stream);
+    clear_graphs(c);
+    dev_free(c, c->d_teams, sizeof(fdsweep::TeamCtl) * (size_t)c->teams_cap);
+    c->d_teams = nullptr;
+    c->teams_cap = 0;
+    const int cap = std::max(nteam, 64);
+    int rc = dev_alloc(c, (void **)&c->d_teams, sizeof(fdsweep::TeamCtl) * (size_t)cap);
+    if (rc) return rc;
+    c->teams_cap = cap;
+    return FDGPU_OK;
+}
+
 // The batched launch as a graph (a queue-reset kernel, then the persistent
 // sweep kernel), captured once per (build, table count, workgroups) and
 // replayed; descriptors and scratch pointers are fixed for its lifetime.
-int sweep_graph(fdgpu_ctx *c, int variant, int n, int nslots, hipGraphExec_t *out) {
-    const long long key = ((long long)variant << 48) | ((long long)nslots << 24) | (long long)n;
+int sweep_graph(fdgpu_ctx *c, int variant, int n, int nslots, int nteam, int K, hipGraphExec_t *out) {
+    // n < 2^24, nslots < 2^20, nteam < 2^11, K <= 16
+    const long long key = ((long long)variant << 60) | ((long long)K << 55) | ((long long)nteam << 44) |
+                          ((long long)nslots << 24) | (long long)n;
     auto it = c->graphs.find(key);
     if (it != c->graphs.end()) { *out = it->second; return FDGPU_OK; }
     hipGraph_t graph = nullptr;
     FD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = fd_launch_reset(c->d_queue, c->stream);
-    hipError_t e2 = fd_launch_sweep(variant, c->d_jobs, n, c->d_slots, nslots, c->d_queue, c->stream);
+    hipError_t e1 = fd_launch_reset(c->d_queue, c->d_teams, nteam, c->stream);
+    hipError_t e2 = fd_launch_sweep(variant, c->d_jobs, n, c->d_slots, nslots, c->d_queue, c->d_teams, nteam, K,
+                                    c->stream);
     hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -356,6 +379,8 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
         // wavefront is solved again face by face, which needs no host preset)
         J.wave = (J.init_mode != 1 && !no_wave) ? (c->wave & 3) : 0;
     }
+    long long total_nodes = 0;
+    for (int t = 0; t < n; t++) total_nodes += tabs[(size_t)t]->n;
     // queue order: largest table first (longest-processing-time-first)
     std::vector<int> order((size_t)n);
     std::iota(order.begin(), order.end(), 0);
@@ -370,12 +395,43 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
         else if (c->lanes == 0 && n > c->num_cu * c->wg_per_cu[FD_WIDE]) variant = FD_NARROW;
         if (geo_need > fd_geo_lds_capacity(variant)) variant = FD_WIDE;   // same results, larger LDS
     }
-    int nslots = mode == FDGPU_BATCHED ? std::min(n, c->num_cu * c->wg_per_cu[variant]) : 1;
+    const int cap = c->num_cu * c->wg_per_cu[variant];
+    // teams: the largest tables' wavefront passes shared by K workgroups each
+    // -- the leading tables of the queue with at least team_share % of the
+    // mean nodes per workgroup of a full device, whose passes can run as
+    // wavefronts -- as many as the idle workgroups hold when the tables are
+    // fewer than the workgroups, else at most a K-th of the workgroups
+    int nteam = 0, K = 1;
+    if (mode == FDGPU_BATCHED && !no_wave && c->team != 1) {
+        K = std::max(2, std::min(fdsweep::kTeamMax, c->team == 0 ? kTeamAuto : c->team));
+        const int arena = variant == FD_NARROW ? kNarrowArenaWords : fdsweep::kArenaWords;
+        const double share = (double)total_nodes / cap * c->team_share / 100.0;
+        for (int t = 0; t < n; t++) {
+            const fdsweep::TableJob &J = qjobs[(size_t)t];
+            const long long nn = (long long)J.nx * J.ny * J.nz;
+            if (!(J.wave & 1) || nn >= (1LL << 29) || !fdsweep::wave_fits(J.nx, J.ny, J.nz, arena) ||
+                (double)nn < share)
+                break;
+            nteam++;
+        }
+        const int most = n < cap ? (cap - n) / (K - 1) : cap / K;
+        nteam = std::min(nteam, std::min(most, 2047));
+    }
+    int nslots = mode == FDGPU_BATCHED ? std::min(n + nteam * (K - 1), cap) : 1;
     if (nslots < 1) nslots = 1;
     int rc = ensure_scratch(c, nslots, maxF, maxN, maxD, !no_wave && c->wave != 0);
     if (rc) return rc;
     rc = ensure_jobs(c, n);
     if (rc) return rc;
+    // the slots must hold the team tables' wavefront state (the leader's slot
+    // serves the team)
+    if (nteam > 0 && (!c->slot_wave || qjobs[0].nx * (long long)qjobs[0].ny * qjobs[0].nz > c->slot_node_cap))
+        nteam = 0;
+    if (nteam > 0) {
+        rc = ensure_teams(c, nteam);
+        if (rc) return rc;
+    }
+    if (nteam == 0) K = 1;
     fdsweep::TableJob *d_jobs = c->d_jobs;
     bool ok = hipMemcpyAsync(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice,
                              c->stream) == hipSuccess;
@@ -383,12 +439,13 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
     if (mode == FDGPU_BATCHED) {
         // one captured graph per launch shape: queue reset + persistent sweep
         hipGraphExec_t g = nullptr;
-        ok = ok && sweep_graph(c, variant, n, nslots, &g) == FDGPU_OK;
+        ok = ok && sweep_graph(c, variant, n, nslots, nteam, K, &g) == FDGPU_OK;
         ok = ok && hipGraphLaunch(g, c->stream) == hipSuccess;
     } else {
         for (int t = 0; t < n && ok; t++) {
-            ok = fd_launch_reset(c->d_queue, c->stream) == hipSuccess;
-            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, c->stream) == hipSuccess;
+            ok = fd_launch_reset(c->d_queue, nullptr, 0, c->stream) == hipSuccess;
+            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, nullptr, 0, 1, c->stream) ==
+                           hipSuccess;
         }
     }
     ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
@@ -412,6 +469,7 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
         if (!no_wave && (T.stats[fdsweep::kStatError] & ~1) && T.job.init_mode != 1) {
             again.push_back(&T);
             if (T.stats[fdsweep::kStatError] & ~(1 | 128)) c->resolved_again++;
+            else c->wave_again++;
         } else if (T.stats[fdsweep::kStatError] & ~1) {
             kerr = 1;
         }
@@ -517,6 +575,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
     dev_free(c, c->ray_cols, c->ray_cols_bytes);
     dev_free(c, c->ray_blk, c->ray_blk_bytes);
+    dev_free(c, c->d_teams, sizeof(fdsweep::TeamCtl) * (size_t)c->teams_cap);
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -679,6 +738,7 @@ int fdgpu_context_counters(fdgpu_ctx *c, long long *out, int n) {
     if (!c || n < 0 || (n > 0 && !out)) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
     if (n > 0) out[0] = c->resolved_again;
+    if (n > 1) out[1] = c->wave_again;
     return FDGPU_OK;
 }
 
@@ -777,7 +837,7 @@ int fdgpu_solve_noalloc(fdgpu_ctx *c, const fdgpu_table_desc *d, float *d_time, 
     W.mstamp = nullptr;
     W.lastf = nullptr;
     W.wst = nullptr; W.wt0 = nullptr; W.wt4 = nullptr;   // no wavefront state: faces one by one
-    W.wcnt = nullptr; W.wtmp = nullptr; W.wready = nullptr;
+    W.wcnt = nullptr; W.wtmp = nullptr; W.wready = nullptr; W.wclose = nullptr;
     W.wcap = 0; W.wn = 0; W.wround = nullptr;
     bool ok = hipEventRecord(c->ev0, c->stream) == hipSuccess;
     ok = ok && fd_launch_one(&J, &W, c->stream) == hipSuccess;
@@ -805,6 +865,22 @@ int fdgpu_set_wave(fdgpu_ctx *c, int flags) {
     if (!c || flags < 0 || flags > 3) return FDGPU_EINPUT;
     std::lock_guard<std::mutex> lk(c->mu);
     c->wave = flags;
+    return FDGPU_OK;
+}
+
+int fdgpu_set_team(fdgpu_ctx *c, int size, int share_pct) {
+    if (!c || size < 0 || size > fdsweep::kTeamMax || share_pct < 0) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->team = size;
+    c->team_share = share_pct;
+    return FDGPU_OK;
+}
+
+int fdgpu_get_team(fdgpu_ctx *c, int *size, int *share_pct) {
+    if (!c || !size || !share_pct) return FDGPU_EINPUT;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *size = c->team;
+    *share_pct = c->team_share;
     return FDGPU_OK;
 }
 

@@ -194,6 +194,7 @@ struct fdgpu_ctx {
     long long slot_face_cap = 0;
     long long slot_node_cap = 0;   // per-slot change stamps: nodes of the largest table
     long long resolved_again = 0;   // tables solved again after a failed consistency check
+    long long wave_again = 0;       // ... after a wavefront pass that could not finish
     char *ray_cols = nullptr;       // fdgpu_rays_trace's lane rows (kept zero between calls)
     size_t ray_cols_bytes = 0;
     char *ray_blk = nullptr;        // ... and its other device arrays, kept between calls
@@ -201,6 +202,8 @@ struct fdgpu_ctx {
     bool slot_wave = false;        // the slots hold wavefront state (sized by slot_node_cap)
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;
+    fdsweep::TeamCtl *d_teams = nullptr;   // team blocks of batched launches, grown on demand
+    int teams_cap = 0;
     fdsweep::TableJob *d_jobs = nullptr;   // descriptors of the current batch
     int jobs_cap = 0;
     std::map<long long, hipGraphExec_t> graphs;   // captured batched launches
@@ -215,6 +218,8 @@ struct fdgpu_ctx {
     double loc_timing[4] = {0.0, 0.0, 0.0, 0.0};
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     int wave = 0;              // passes run as wavefronts (fdgpu_set_wave): bit0 first, bit1 reverse
+    int team = 0;              // workgroups per large table's wavefront passes (fdgpu_set_team): 0 automatic
+    int team_share = 200;      // ... for tables of at least this % of the mean nodes per workgroup
     // this device's copy of the fixed array behind fdgpu_solve_noalloc
     // (sweep_kernel.hip `scratch`), looked up once at fdgpu_open
     char *d_fixed = nullptr;

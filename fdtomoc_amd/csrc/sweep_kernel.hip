@@ -40,21 +40,52 @@ struct DevExec {
         return atomicMax(p, v);
     }
     __device__ __forceinline__ long long clock() const { return (long long)wall_clock64(); }
+    // team members on other CUs (and other XCDs): agent scope
+    __device__ __forceinline__ void fence_release() const { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+    __device__ __forceinline__ void fence_acquire() const { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+    __device__ __forceinline__ int load_relaxed(const int *p) const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // this wave's memory operations complete (stores reached the L2)
+    __device__ __forceinline__ void wait_stores() const { __builtin_amdgcn_s_waitcnt(0); }
+    __device__ __forceinline__ void store_release(int *p, int v) const {
+        __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void pause() const { __builtin_amdgcn_s_sleep(2); }
 };
 
+// Tickets from the queue: the first nteam * K go to teams of K workgroups on
+// the first nteam tables (ticket t: table t / K, member t % K), the rest to
+// one table each.  A member waits only for members that have taken their
+// tickets already, so a team forms whenever K workgroups can run at once.
 template <int THREADS, int ARENA, int GEOL>
 __global__ void __launch_bounds__(THREADS, 2)
-    sphfd_sweep_kernel(const fdsweep::TableJob *jobs, int njobs, const fdsweep::Scratch *slots, int *queue) {
+    sphfd_sweep_kernel(const fdsweep::TableJob *jobs, int njobs, const fdsweep::Scratch *slots, int *queue,
+                       fdsweep::TeamCtl *teams, int nteam, int K) {
     __shared__ fdsweep::SharedT<ARENA, GEOL> sh;
     DevExec x;
     const fdsweep::Scratch W = slots[blockIdx.x];
     while (true) {
         if (threadIdx.x == 0) sh.job = atomicAdd(queue, 1);
         __syncthreads();
-        const int j = sh.job;
+        const int t = sh.job;
         __syncthreads();
+        int j = t, member = 0;
+        if (t < nteam * K) {
+            j = t / K;
+            member = t - j * K;
+        } else {
+            j = t - nteam * (K - 1);
+        }
         if (j >= njobs) break;
-        fdsweep::solve_table(x, sh, jobs[j], W);
+        if (j < nteam && member > 0) {
+            fdsweep::team_helper(x, sh, jobs[j], teams + j, K, member);
+        } else if (j < nteam) {
+            fdsweep::solve_table(x, sh, jobs[j], W, teams + j, K);
+            fdsweep::team_release(x, teams + j);
+        } else {
+            fdsweep::solve_table(x, sh, jobs[j], W);
+        }
     }
 }
 
@@ -159,8 +190,15 @@ __global__ void geo_kernel(fdsweep::GeoParams p, double *out, long long n) {
         fdsweep::geo_fill_item(p, e, out);
 }
 
-__global__ void reset_kernel(int *q) {
+// Before a batched launch: the ticket counter and the control words of the
+// teams' blocks (a kernel rather than a memset node: the launch is replayed
+// from a captured graph).
+__global__ void reset_kernel(int *q, fdsweep::TeamCtl *teams, int nteam) {
     if (threadIdx.x == 0) q[0] = 0;
+    for (int t = threadIdx.x; t < nteam; t += blockDim.x) {
+        fdsweep::TeamCtl &c = teams[t];
+        c.seq = 0; c.release = 0; c.abort = 0; c.bar_count = 0; c.bar_gen = 0; c.inpass = 0;
+    }
 }
 
 // slow may alias vel (an in-place update): each lane reads its element before
@@ -199,13 +237,18 @@ extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::S
 }
 
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
-                                      const fdsweep::Scratch *d_slots, int nslots, int *d_queue, hipStream_t stream) {
+                                      const fdsweep::Scratch *d_slots, int nslots, int *d_queue,
+                                      fdsweep::TeamCtl *d_teams, int nteam, int team_size, hipStream_t stream) {
+    if (nteam <= 0 || team_size < 2) {
+        nteam = 0;
+        team_size = 1;
+    }
     if (variant == FD_NARROW)
         hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds>), dim3(nslots), dim3(kNarrowThreads),
-                           0, stream, d_jobs, njobs, d_slots, d_queue);
+                           0, stream, d_jobs, njobs, d_slots, d_queue, d_teams, nteam, team_size);
     else
         hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>), dim3(nslots), dim3(kWideThreads), 0,
-                           stream, d_jobs, njobs, d_slots, d_queue);
+                           stream, d_jobs, njobs, d_slots, d_queue, d_teams, nteam, team_size);
     return hipGetLastError();
 }
 
@@ -229,8 +272,8 @@ extern "C" const void *fd_sweep_kernel_ptr(int variant) {
                                 : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>;
 }
 
-extern "C" hipError_t fd_launch_reset(int *d_queue, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, d_queue);
+extern "C" hipError_t fd_launch_reset(int *d_queue, fdsweep::TeamCtl *d_teams, int nteam, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, d_queue, d_teams, nteam);
     return hipGetLastError();
 }
 

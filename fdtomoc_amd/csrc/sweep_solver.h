@@ -10,6 +10,11 @@
 //   unsigned X::atomic_sub_u(unsigned *p, unsigned v)  returns the old value
 //   unsigned X::atomic_or(unsigned*,unsigned), X::atomic_and(...)
 //   long long X::clock() const             a monotonic tick counter (timing only)
+//   unsigned long long X::atomic_max_u64(...), unsigned X::atomic_cas(...)
+//   void X::fence_release(), X::fence_acquire()   agent-scope fences (teams)
+//   void X::wait_stores()                  this thread's stores have completed
+//   int  X::load_relaxed(const int*), void X::store_release(int*, int)
+//   void X::pause()                        back off inside a wait loop
 //
 // On the GPU X wraps threadIdx/__syncthreads/atomics (sweep_kernel.hip).
 //
@@ -101,6 +106,7 @@ enum {
     kStatTimeWaveGroup, // of which grouping each round's nodes by face type
     kStatTimeWaveEval,  // evaluations and queueing (phase 3)
     kStatTimeWaveCarry, // carried-register requests (phase 4)
+    kStatTeam,          // workgroups that solved the table (a team's size, else 1)
     kStatCount
 };
 
@@ -128,8 +134,9 @@ struct Scratch {
     float *wt0, *wt4;    // per node: t[0]/t[4] as the node left them (or its resolved carry-in)
     int *wcnt;           // per node: earlier-face predecessors still to finalise (bits 0-7) and
                          // in-face predecessors finalised (bits 8-15)
-    int *wtmp, *wready;  // per round: nodes queued for the next round, this round's by face type
-    long long wcap;      // capacity of each of the two lists
+    int *wtmp, *wready;  // the lists of the rounds: one evaluated, one filled for the next round
+    int *wclose;         // teams: per round, nodes whose earlier-face predecessors all finished
+    long long wcap;      // capacity of each of the three lists
     long long wn;        // nodes wst, wt0 and wt4 hold (0: no wavefront state)
     unsigned *wround;    // the slot's next round number (one word; starts at 1 over zeroed
                          // status words)
@@ -791,8 +798,11 @@ namespace fdsweep {
 
 // ------------------------------------------------------------- one table
 
+// Solves table J on one workgroup, or as the leader of a team of K
+// workgroups (tm: the team's block) whose other members run team_helper; the
+// caller releases the team when this returns.
 template <class X, class SH>
-FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
+FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W, TeamCtl *tm = nullptr, int K = 1) {
     const int T = x.nthr(), tid = x.tid();
     const long long N = (long long)J.nx * J.ny * J.nz;
     if (int *ms = stamps_of(J, W))
@@ -813,7 +823,6 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
         x.sync();
     }
     // the row and layer tables into LDS
-    const double *gbuf = J.geo_fill ? J.geo_fill : J.geo;
     const long long nrl = geo_rows_doubles(J.ny) + geo_layers_doubles(J.nz);
     if (nrl > SH::kGeoLds) {   // never sent by the host (it picks a build that holds them)
         if (tid == 0) {
@@ -825,26 +834,20 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
         x.sync();
         return;
     }
-    for (long long q = tid; q < nrl; q += T) sh.geol[q] = gbuf[q];
+    Geo G;
+    load_geo(x, sh, J, G);
     if (tid == 0) {
         for (int s = 0; s < kStatCount; s++) sh.st[s] = 0;
         sh.cin0 = 0.0f;
         sh.cin4 = 0.0f;
         sh.face_ctr = 0;
+        sh.st[kStatTeam] = tm ? K : 1;
     }
     x.sync();
-    if (tid == 0) sh.st[kStatTimeInit] = 0;
-    Geo G;
-    G.h = J.h; G.dq = J.dq; G.df = J.df; G.y0 = J.y0; G.z0r = J.z0r; G.sinq = J.sinq;
-    G.rh = 1.0 / J.h;
-    G.ihsq = 1. / (J.h * J.h);
-    G.idqsq = 1. / (J.dq * J.dq);
-    G.rowt = (GeoPtr)sh.geol;
-    G.layt = G.rowt + geo_rows_doubles(J.ny);
-    G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;
     int x1 = J.box[0][0], x2 = J.box[0][1], y1 = J.box[1][0], y2 = J.box[1][1], z1 = J.box[2][0], z2 = J.box[2][1];
     int dx1 = J.dflag[0], dx2 = J.dflag[1], dy1 = J.dflag[2], dy2 = J.dflag[3], dz1 = J.dflag[4], dz2 = J.dflag[5];
     int rad0 = 1, radius = J.radius0, srcwall = J.srcwall, reverse = J.reverse;
+    int gen = 0;   // team barriers passed
     const int nx = J.nx, ny = J.ny, nz = J.nz;
     int passes = 0;
 #define FD_SETBOX()                                  \
@@ -858,14 +861,17 @@ FD_HD void solve_table(X &x, SH &sh, const TableJob &J, const Scratch &W) {
             for (int w = 0; w < 7; w++) sh.headw[w] = 0;
         x.sync();
         passes++;
-        if ((J.wave & (passes == 1 ? 1 : 2)) && W.wst && N <= W.wn && N < (1LL << 29) &&
-            wave_arena_words(nx, ny, nz) <= SH::kArena) {
+        const bool wave = (J.wave & (passes == 1 ? 1 : 2)) && W.wst && N <= W.wn && N < (1LL << 29) &&
+                          wave_fits(nx, ny, nz, SH::kArena);
+        if (wave) {
             const int b0[3][2] = {{x1, x2}, {y1, y2}, {z1, z2}};
             const int f0[6] = {dx1, dx2, dy1, dy2, dz1, dz2};
             // the whole pass at once; on failure the host solves the table face by face
-            if (!wave_pass(x, sh, J, W, G, b0, f0, rad0, radius)) break;
+            if (!wave_pass(x, sh, J, W, G, b0, f0, rad0, radius, tm, tm ? K : 1, 0, &gen)) break;
             dx1 = dx2 = dy1 = dy2 = dz1 = dz2 = 0;
         }
+        // no wavefront pass now, none later: the team's other members leave
+        if (tm && (!wave || !(J.wave & 2))) team_release(x, tm);
         while (rad0 && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
             for (int g = 1; g <= J.grow[0]; g++)
                 if (dz1) { FD_SETBOX(); sweep_face<TOP>(x, sh, J, W, G); if (z1 == 0) dz1 = 0; z1--; }

@@ -21,16 +21,28 @@
 // neighbours of a node and of its in-face neighbours lie in the node's cube on
 // an earlier face, so they are final before the node can be found ready.
 //
-// Rounds (one workgroup, bulk-synchronous).  Every node of the pass has a
-// counter word: its earlier-face predecessors still to finalise (set at the
-// start of the pass from the event tables) and, beside it, how many of its
-// in-face predecessors have finalised.  A node evaluated in round R (exactly
-// as a LEVEL face evaluates it, sweep_core.h eval_node) updates the counters of
-// its neighbours that follow it; the update that leaves no earlier-face
+// Rounds (bulk-synchronous).  Every node of the pass has a counter word: its
+// earlier-face predecessors still to finalise (set at the start of the pass
+// from the event tables) and, beside it, how many of its in-face predecessors
+// have finalised.  A node evaluated in round R (exactly as a LEVEL face
+// evaluates it, sweep_core.h eval_node) updates the counters of its
+// neighbours that follow it; the update that leaves no earlier-face
 // predecessor and all in-face predecessors final queues the node for round
 // R + 1.  The number of in-face predecessors needs the order keys, which are
 // final by then.  Two neighbours are never ready in the same round (one of
-// them precedes the other), so a round reads only values final before it.
+// them precedes the other), so a round reads only values final before it,
+// and its nodes can be evaluated in any order by any number of workgroups.
+//
+// Teams.  A table's passes run on one workgroup, or on a team of K
+// workgroups (on K CUs) that share each round's list: member m evaluates the
+// m-th K-th of it.  The list sizes, the requests below and the per-face key
+// counters then live in the team's block in device memory (TeamCtl), and the
+// members meet at a barrier in device memory (team_sync: agent-scope release
+// before arriving, acquire after leaving) twice per round: after the leader
+// has swapped the lists, and after every member has evaluated its share.
+// The leader (member 0) owns the table: it runs the controller and every
+// face-by-face pass, opens each wavefront pass for the others, and resolves
+// the carried-register requests between rounds.
 //
 // Carried registers: fdsph2d also tests t[0]/t[4] as left by whichever node
 // ran before it in the sequential order (sphfd.c:6116-6121).  A node that
@@ -50,18 +62,112 @@ constexpr int kWaveCarryCap = 64;        // carried-register requests waiting at
 constexpr int kWaveEvInts = 6;           // per event: 3 packed box pairs, type, face size, keys pending
 constexpr int kWaveCarryInts = 8;        // per request: node, event, bound (2), candidate, state, cand. event, -
 constexpr unsigned kWaveRoundLimit = 1u << 29;   // a slot's rounds are renumbered from 1 past this
+constexpr int kWaveLocalMin = 1024;      // arena words left for grouping a round's nodes by face type
+constexpr int kTeamMaxEv = 3712;         // events of a pass in a team's block (at most nx + ny + nz)
+constexpr int kTeamMax = 16;             // workgroups in a team
+// a member waiting longer than this (X::clock ticks of 10 ns) gives the pass up
+constexpr long long kTeamWaitTicks = 2000000000LL;
 
 // Arena words the wavefront needs for a grid (planes' events, events, requests).
 FD_INLINE long long wave_arena_words(int nx, int ny, int nz) {
     return (long long)(nx + ny + nz) * (1 + kWaveEvInts) + (long long)kWaveCarryCap * kWaveCarryInts;
 }
 
-// Control words in SharedT::wc.
+// Whether a grid's passes can run as wavefronts in an arena of `arena` words
+// (the event tables plus room to group a round's nodes).
+FD_INLINE bool wave_fits(int nx, int ny, int nz, int arena) {
+    return wave_arena_words(nx, ny, nz) + kWaveLocalMin <= arena && nx + ny + nz <= kTeamMaxEv;
+}
+
+// Control words in SharedT::wc (kWcNCur, kWcNNext, kWcErr, kWcWaiting and
+// kWcNClose are the team's words in TeamCtl::wc when the pass runs on a team).
 enum {
-    kWcNev = 0, kWcNCur, kWcNNext, kWcDone, kWcTotal, kWcErr, kWcScanBad, kWcWaiting, kWcResolved,
-    kWcTypeCnt = 16,   // 6 counters of the list being filled
-    kWcTypeOff = 22,   // 6 offsets into the list being evaluated
+    kWcNev = 0, kWcNCur, kWcNNext, kWcDone, kWcTotal, kWcErr, kWcScanBad, kWcWaiting, kWcTeamOk, kWcNClose,
+    kWcTypeCnt = 16,   // 6 counters of the nodes being grouped
+    kWcTypeOff = 22,   // 6 offsets into the grouped nodes
 };
+
+// A team's block in device memory, one per table solved by a team.  The
+// words up to `W` are cleared before every launch; the leader writes the rest
+// when it opens a pass.
+struct TeamCtl {
+    int seq;          // wavefront passes the leader has opened
+    int release;      // 1: the leader opens no more passes (the others leave)
+    int abort;        // 1: a member waited too long (every member gives the pass up)
+    int bar_count;    // members arrived at the barrier
+    int bar_gen;      // barriers completed
+    int inpass;       // members other than the leader inside a pass
+    int pad[2];
+    Scratch W;        // the leader's slot: the pass's node state and lists
+    int box0[3][2], flags0[6], rad0, radius, stamp0;
+    unsigned base;    // the pass's first round number
+    int wc[16];       // list sizes, waiting requests, error
+    int headw[8];     // head waves counted by the other members
+    int done, seeds, carry;   // evaluations (all members), the others' seeds and requests
+    int pk[kTeamMaxEv];       // per event: keys of its face not yet final
+    int car[kWaveCarryCap * kWaveCarryInts];
+};
+
+// The list a round fills: nodes queued for the next round, with its size word.
+struct WaveQ {
+    int *next;
+    int *cnt;
+    long long cap;
+};
+
+// Barrier of a team's K members: every store a member made before it is
+// visible to every member after it.  false: a member gave up waiting (the
+// pass is abandoned by all).
+//
+// One lane per workgroup does the cache maintenance: every wave first waits
+// for its own stores to reach the CU's L2 (X::wait_stores), the workgroup
+// barrier orders that before lane 0's release (which writes the L2 back for
+// the other dies), and lane 0's acquire (which invalidates the CU's L1 and
+// the lines of other dies' memory in its L2) comes before the barrier that
+// lets the workgroup read.  The spin reads are relaxed: an acquire per
+// iteration would invalidate the caches on every poll.
+template <class X>
+FD_INLINE void team_publish(X &x) {
+    x.wait_stores();
+    x.sync();
+    if (x.tid() == 0) x.fence_release();
+}
+
+template <class X>
+FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw) {
+    team_publish(x);
+    if (x.tid() == 0) {
+        int ok = 1;
+        if (x.load_relaxed(&tm->abort)) {
+            ok = 0;
+        } else if (x.atomic_add(&tm->bar_count, 1) == K - 1) {
+            x.atomic_exch(&tm->bar_count, 0);
+            x.store_release(&tm->bar_gen, gen + 1);
+        } else {
+            const long long t0 = x.clock();
+            while (x.load_relaxed(&tm->bar_gen) == gen) {
+                if (x.load_relaxed(&tm->abort)) { ok = 0; break; }
+                if (x.clock() - t0 > kTeamWaitTicks) { x.store_release(&tm->abort, 1); ok = 0; break; }
+                x.pause();
+            }
+        }
+        x.fence_acquire();
+        *okw = ok;
+    }
+    gen++;
+    x.sync();
+    const int ok = *okw;
+    x.sync();
+    return ok != 0;
+}
+
+// The leader opens no more passes: the other members leave the table.
+template <class X>
+FD_INLINE void team_release(X &x, TeamCtl *tm) {
+    team_publish(x);
+    if (x.tid() == 0) x.store_release(&tm->release, 1);
+    x.sync();
+}
 
 // status word: bit0 final, bit1 set t[0]/t[4], bits 2.. the round
 FD_INLINE bool wv_final_in(unsigned s, unsigned base) { return (s & 1u) && (s >> 2) >= base; }
@@ -69,14 +175,17 @@ FD_INLINE bool wv_final_before(unsigned s, unsigned base, unsigned R) {
     return (s & 1u) && (s >> 2) >= base && (s >> 2) < R;
 }
 
-// Views of the pass's event tables in the workgroup arena.
+// Views of the pass's event tables in the workgroup arena (the key counters
+// and requests in the team's block when a team runs the pass).
 struct WaveTabs {
     int *mx, *my, *mz;   // plane -> event
     int *ev;             // kWaveEvInts per event
     int *car;            // kWaveCarryInts per request
+    int *pkb;            // keys not yet final of event e at pkb[pks * e]
+    int pks;
     FD_INLINE int type(int e) const { return ev[kWaveEvInts * e + 3]; }
     FD_INLINE int fsize(int e) const { return ev[kWaveEvInts * e + 4]; }
-    FD_INLINE int *pk(int e) const { return ev + kWaveEvInts * e + 5; }
+    FD_INLINE int *pk(int e) const { return pkb + pks * e; }
     FD_INLINE void box(int e, int B[3][2]) const {
         for (int a = 0; a < 3; a++) {
             const unsigned w = (unsigned)ev[kWaveEvInts * e + a];
@@ -131,12 +240,12 @@ FD_INLINE int face_fill(const FaceGeom &fg, int i, int j, int k) {
     return (k - fg.b0) * fg.wa + (i - fg.a0);
 }
 
-// Appends ready node v (face type F) to the next round's list.
-template <class X, class SH>
-FD_INLINE void wave_queue(X &x, SH &sh, const Scratch &W, long long v, int F) {
-    const int pos = x.atomic_add(&sh.wc[kWcNNext], 1);
-    x.atomic_add(&sh.wc[kWcTypeCnt + F], 1);
-    if (pos < W.wcap) W.wtmp[pos] = (int)((unsigned)v | ((unsigned)F << 29));
+// Appends ready node v (face type F) to the next round's list (a list over
+// its capacity fails the pass when the round starts).
+template <class X>
+FD_INLINE void wave_queue(X &x, const WaveQ &Q, long long v, int F) {
+    const int pos = x.atomic_add(Q.cnt, 1);
+    if (pos < Q.cap) Q.next[pos] = (int)((unsigned)v | ((unsigned)F << 29));
 }
 
 // Node index -> coordinates (indices below 2^29).
@@ -254,12 +363,35 @@ FD_NOINLINE unsigned long long wave_scan(X &x, SH &sh, const TableJob &J, const 
     return (r == ~0ull || r == 0) ? ~0ull : r - 1;
 }
 
+// Node u's earlier-face predecessors are all final: count its in-face
+// predecessors (their keys are final now), record the count, and queue u if
+// they are all final already.
+template <class X>
+FD_INLINE void wave_close(X &x, const TableJob &J, const Scratch &W, const WaveTabs &WT, const WaveQ &Q, long long u) {
+    int ui, uj, uk;
+    node_ijk(J, u, ui, uj, uk);
+    int fx[3], fy[3], fz[3];
+    cube_events(J, WT, ui, uj, uk, fx, fy, fz);
+    const int eu = imax3(fx[1], fy[1], fz[1]);
+    const int p2 = inface_preds(J, WT, ui, uj, uk, eu, fx, fy, fz);
+    const int o2 = x.atomic_add(&W.wcnt[u], (p2 << 16) | (1 << 24));
+    if (((o2 >> 8) & 255) == p2) wave_queue(x, Q, u, WT.type(eu));
+}
+
 // Evaluates ready node v of face type F as a LEVEL face would, then updates
 // the counters of its neighbours that follow it and queues those it leaves
 // ready.
+//
+// A node whose earlier-face predecessors this evaluation closes needs its
+// in-face predecessors counted from its inner-layer keys.  One workgroup
+// does that at once: the keys written this round by its other waves passed
+// through the same CU and are visible.  In a team (CL set) a key may have
+// been written this round on another CU, visible only after the next
+// barrier, so the node goes to the closing list CL instead and is counted
+// in the round's closing phase (wave_close).
 template <int F, class X, class SH>
 FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G, const WaveTabs &WT,
-                         long long v, unsigned base, unsigned R, int stamp0) {
+                         const WaveQ &Q, const WaveQ *CL, long long v, unsigned base, unsigned R, int stamp0) {
     const long long nx = J.nx, nxy = (long long)J.nx * J.ny;
     int i, j, k;
     node_ijk(J, v, i, j, k);
@@ -295,7 +427,7 @@ FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             for (int c = 0; c < kWaveCarryCap && got < 0; c++)
                 if (x.atomic_cas((unsigned *)&WT.car[kWaveCarryInts * c + 5], 0u, 3u) == 0u) got = c;
             if (got < 0) {   // no free request: try again next round
-                wave_queue(x, sh, W, v, F);
+                wave_queue(x, Q, v, F);
                 return;
             }
             int *q = WT.car + kWaveCarryInts * got;
@@ -377,36 +509,38 @@ FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
         if (same & (1u << q)) {
             // ready when the count is known and this was the last in-face predecessor
             if (((o >> 24) & 1) && ((o >> 8) & 255) + 1 == ((o >> 16) & 255))
-                wave_queue(x, sh, W, (long long)(k + dz - 1) * nxy + (long long)(j + dy - 1) * nx + (i + dx - 1),
+                wave_queue(x, Q, (long long)(k + dz - 1) * nxy + (long long)(j + dy - 1) * nx + (i + dx - 1),
                            WT.type(e));
         } else if ((o & 255) == 1) {
             closed |= 1u << q;   // this update closed its earlier-face predecessors
         }
     }
-    // for those: count their in-face predecessors (keys final now), record the
-    // count, and queue the node if they are all final already
+    // for those: count their in-face predecessors (now, or in the closing phase)
     while (closed) {
         const int q = __builtin_ctz(closed);
         closed &= closed - 1;
         const int ui = i + q % 3 - 1, uj = j + (q / 3) % 3 - 1, uk = k + q / 9 - 1;
-        int fx[3], fy[3], fz[3];
-        cube_events(J, WT, ui, uj, uk, fx, fy, fz);
-        const int eu = imax3(fx[1], fy[1], fz[1]);
-        const int p2 = inface_preds(J, WT, ui, uj, uk, eu, fx, fy, fz);
         const long long u = (long long)uk * nxy + (long long)uj * nx + ui;
-        const int o2 = x.atomic_add(&W.wcnt[u], (p2 << 16) | (1 << 24));
-        if (((o2 >> 8) & 255) == p2) wave_queue(x, sh, W, u, WT.type(eu));
+        if (CL) wave_queue(x, *CL, u, 0);
+        else wave_close(x, J, W, WT, Q, u);
     }
 }
 
 // Runs one pass from the controller state (box, growth flags, rad0, radius)
-// as a wavefront.  Returns false if the pass could not be completed (list
-// capacity, or a request that never resolves: the host then solves the table
-// face by face).
+// as a wavefront, alone (tm null) or as member `member` of a team of K
+// workgroups.  The leader (member 0, also the lone workgroup) owns the pass;
+// the others are called by team_helper with the leader's slot W and the
+// pass's parameters.  Returns false if the pass could not be completed (list
+// capacity, a request that never resolves, a member that waited too long:
+// the host then solves the table face by face).  Every member reaches the
+// same number of team barriers: each decision to leave the rounds is taken on
+// values every member reads after the same barrier.
 template <class X, class SH>
 FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo &G, const int box0[3][2],
-                         const int flags0[6], int rad0, int radius) {
+                         const int flags0[6], int rad0, int radius, TeamCtl *tm = nullptr, int K = 1,
+                         int member = 0, int *gen = nullptr) {
     const int T = x.nthr(), tid = x.tid();
+    const bool lead = member == 0;
     const int nx = J.nx, ny = J.ny, nz = J.nz;
     const long long nxy = (long long)nx * ny, N = nxy * nz;
     const long long t0 = x.clock();
@@ -415,21 +549,38 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     WT.my = WT.mx + nx;
     WT.mz = WT.my + ny;
     WT.ev = WT.mz + nz;
-    WT.car = WT.ev + (long long)kWaveEvInts * (nx + ny + nz);
+    WT.car = tm ? tm->car : WT.ev + (long long)kWaveEvInts * (nx + ny + nz);
+    WT.pkb = tm ? tm->pk : WT.ev + 5;
+    WT.pks = tm ? 1 : kWaveEvInts;
+    // the rest of the arena groups a round's nodes by face type
+    int *const lbuf = sh.arena + wave_arena_words(nx, ny, nz);
+    const long long L = (long long)SH::kArena - wave_arena_words(nx, ny, nz);
+    int *const wc = tm ? tm->wc : sh.wc;   // list sizes, requests waiting, error: shared by the team
+    auto tsync = [&]() -> bool {
+        if (!tm) { x.sync(); return true; }
+        return team_sync(x, tm, K, *gen, &sh.wc[kWcTeamOk]);
+    };
     // planes inside the starting box are interior, all others unswept so far
     for (int q = tid; q < nx + ny + nz; q += T) {
         int a, c;
         if (q < nx) { a = 0; c = q; } else if (q < nx + ny) { a = 1; c = q - nx; } else { a = 2; c = q - nx - ny; }
         WT.mx[q] = (c > box0[a][0] && c < box0[a][1]) ? -1 : kWaveNever;
     }
-    for (int q = tid; q < kWaveCarryCap * kWaveCarryInts; q += T) WT.car[q] = 0;
     for (int q = tid; q < 32; q += T) sh.wc[q] = 0;
+    if (lead) {
+        for (int q = tid; q < kWaveCarryCap * kWaveCarryInts; q += T) WT.car[q] = 0;
+        if (tm) {
+            for (int q = tid; q < 16; q += T) tm->wc[q] = 0;
+            for (int q = tid; q < 8; q += T) tm->headw[q] = 0;
+            if (tid == 0) { tm->done = 0; tm->seeds = 0; tm->carry = 0; }
+        }
+    }
     x.sync();
     if (tid == 0) {
         // replay the controller (solve_table's first-pass loop) to list the events
         int x1 = box0[0][0], x2 = box0[0][1], y1 = box0[1][0], y2 = box0[1][1], z1 = box0[2][0], z2 = box0[2][1];
         int dx1 = flags0[0], dx2 = flags0[1], dy1 = flags0[2], dy2 = flags0[3], dz1 = flags0[4], dz2 = flags0[5];
-        int ne = 0, total = 0;
+        int ne = 0, total = 0, grow = rad0, rad = radius;
         auto rec = [&](int F, int plane) {
             int *E = WT.ev + kWaveEvInts * ne;
             E[0] = (int)(((unsigned)(x1 + 1)) | ((unsigned)(x2 + 1) << 16));
@@ -445,7 +596,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             m[plane] = ne;
             ne++;
         };
-        while (rad0 && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
+        while (grow && (dx1 || dx2 || dy1 || dy2 || dz1 || dz2)) {
             for (int g = 1; g <= J.grow[0]; g++)
                 if (dz1) { rec(TOP, z1); if (z1 == 0) dz1 = 0; z1--; }
             for (int g = 1; g <= J.grow[1]; g++)
@@ -458,32 +609,56 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                 if (dy1) { rec(NORTH, y1); if (y1 == 0) dy1 = 0; y1--; }
             for (int g = 1; g <= J.grow[5]; g++)
                 if (dy2) { rec(SOUTH, y2); if (y2 == ny - 1) dy2 = 0; y2++; }
-            radius++;
-            if (radius == J.maxrad) rad0 = 0;
+            rad++;
+            if (rad == J.maxrad) grow = 0;
         }
         sh.wc[kWcNev] = ne;
         sh.wc[kWcTotal] = total;
-        // the slot's round numbers: every status word of earlier passes and
-        // tables is below this pass's base
-        unsigned b = *W.wround;
-        if (b >= kWaveRoundLimit) b = 0;
-        sh.wc[kWcResolved] = (int)b;
+        if (lead) {
+            // the slot's round numbers: every status word of earlier passes and
+            // tables is below this pass's base (0: exhausted, start again at 1)
+            unsigned b = *W.wround;
+            if (b >= kWaveRoundLimit) b = 0;
+            sh.wc[kWcDone] = (int)b;
+        }
     }
     x.sync();
-    unsigned base = (unsigned)sh.wc[kWcResolved];
-    if (base == 0) {
-        // the slot's round numbers are exhausted: clear every status word
-        for (long long q = tid; q < W.wn; q += T) W.wst[q] = 0u;
-        base = 1;
+    unsigned base;
+    if (lead) {
+        base = (unsigned)sh.wc[kWcDone];
+        if (base == 0) {
+            for (long long q = tid; q < W.wn; q += T) W.wst[q] = 0u;
+            base = 1;
+        }
+    } else {
+        base = tm->base;
     }
     const int nev = sh.wc[kWcNev];
-    const int stamp0 = sh.face_ctr + 1;
+    const int stamp0 = lead ? sh.face_ctr + 1 : tm->stamp0;
     const float cin0 = sh.cin0, cin4 = sh.cin4;
     x.sync();
-    if (tid == 0) sh.wc[kWcResolved] = 0;
+    if (tid == 0) sh.wc[kWcDone] = 0;
+    if (lead && tm) {
+        // open the pass for the other members
+        for (int e = tid; e < nev; e += T) tm->pk[e] = WT.fsize(e);
+        if (tid == 0) {
+            tm->W = W;
+            for (int a = 0; a < 3; a++) { tm->box0[a][0] = box0[a][0]; tm->box0[a][1] = box0[a][1]; }
+            for (int f = 0; f < 6; f++) tm->flags0[f] = flags0[f];
+            tm->rad0 = rad0;
+            tm->radius = radius;
+            tm->stamp0 = stamp0;
+            tm->base = base;
+        }
+        team_publish(x);
+        if (tid == 0) x.store_release(&tm->seq, x.load_relaxed(&tm->seq) + 1);
+    }
+    x.sync();
     // counters: earlier-face predecessors of every node of the pass; nodes with
-    // none and no in-face predecessor are ready for the first round
-    for (long long v = tid; v < N; v += T) {
+    // none and no in-face predecessor are ready for the first round (list wtmp)
+    const WaveQ Q0{W.wtmp, &wc[kWcNNext], W.wcap};
+    const long long first = (long long)member * T + tid, stride = (long long)K * T;
+    for (long long v = first; v < N; v += stride) {
         int i, j, k;
         node_ijk(J, v, i, j, k);
         int ex[3], ey[3], ez[3];
@@ -497,7 +672,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             const int p2 = inface_preds(J, WT, i, j, k, e, ex, ey, ez);
             W.wcnt[v] = (p2 << 16) | (1 << 24);
             if (p2 == 0) {
-                wave_queue(x, sh, W, v, WT.type(e));
+                wave_queue(x, Q0, v, WT.type(e));
                 x.atomic_add(&sh.st[kStatWaveSeeds], 1);
             }
         }
@@ -511,7 +686,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                            box0[2][1] - 1 < nz - 1 ? box0[2][1] - 1 : nz - 1};
         const long long wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = hi[2] - lo[2] + 1;
         const long long nin = (wx > 0 && wy > 0 && wz > 0) ? wx * wy * wz : 0;
-        for (long long q = tid; q < nin; q += T) {
+        for (long long q = first; q < nin; q += stride) {
             const int i = lo[0] + (int)(q % wx), j = lo[1] + (int)((q / wx) % wy), k = lo[2] + (int)(q / (wx * wy));
             for (int d = 0; d < 6; d++) {
                 const int ax = d >> 1, sg = (d & 1) ? 1 : -1;
@@ -525,32 +700,25 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             }
         }
     }
-    x.sync();
     unsigned R = base;
-    bool ok = true;
+    bool ok = true, alive = tsync();   // alive: no member gave the pass up
     int idle = 0;
-    for (;;) {
-        // round R evaluates the list filled in round R - 1 (wtmp), grouped by
-        // face type into wready so that waves of one type evaluate together
+    while (alive) {
+        // round R evaluates the list filled before it (wtmp when R - base is
+        // even, wready when odd) and fills the other
         const long long tr0 = tid == 0 ? x.clock() : 0;
-        if (tid == 0) {
-            const int n = sh.wc[kWcNNext];
-            sh.wc[kWcNCur] = n;
-            sh.wc[kWcNNext] = 0;
-            int o = 0;
-            for (int f = 0; f < 6; f++) {
-                sh.wc[kWcTypeOff + f] = o;
-                o += sh.wc[kWcTypeCnt + f];
-                sh.wc[kWcTypeCnt + f] = 0;
-            }
+        if (lead && tid == 0) {
+            wc[kWcNCur] = wc[kWcNNext];
+            wc[kWcNNext] = 0;
+            wc[kWcNClose] = 0;
             int waiting = 0;
             for (int c = 0; c < kWaveCarryCap; c++) waiting += WT.car[kWaveCarryInts * c + 5] != 0;
-            sh.wc[kWcWaiting] = waiting;
+            wc[kWcWaiting] = waiting;
         }
-        x.sync();
-        const int ncur = sh.wc[kWcNCur];
-        if (ncur > W.wcap) { ok = false; break; }
-        if (ncur == 0 && sh.wc[kWcWaiting] == 0) break;
+        if (!(alive = tsync())) break;
+        const int ncur = wc[kWcNCur], waiting = wc[kWcWaiting];
+        if (wc[kWcErr] || ncur > W.wcap) { ok = false; break; }
+        if (ncur == 0 && waiting == 0) break;
         if (ncur == 0) {
             // only carried-register requests are open: one must resolve now
             if (++idle > 2) { ok = false; break; }
@@ -558,108 +726,176 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             idle = 0;
         }
         if (R - base > (unsigned)(2 * N + 64) || R >= (1u << 30) - 2) { ok = false; break; }
-        for (int it = tid; it < ncur; it += T) {
-            const int w = W.wtmp[it];
-            W.wready[x.atomic_add(&sh.wc[kWcTypeOff + (int)((unsigned)w >> 29)], 1)] = w;
-        }
-        if (tid == 0) sh.st[kStatRoundsWave] += 1;
-        x.sync();
-        // evaluate; each evaluation queues the neighbours it leaves ready
+        const bool odd = ((R - base) & 1u) != 0;
+        const int *cur = odd ? W.wready : W.wtmp;
+        const WaveQ Q{odd ? W.wtmp : W.wready, &wc[kWcNNext], W.wcap};
+        const WaveQ CLQ{W.wclose, &wc[kWcNClose], W.wcap};
+        const WaveQ *CL = tm ? &CLQ : nullptr;
         const long long tr1 = tid == 0 ? x.clock() : 0;
-        // one face type at a time: the six evaluation routines are large, and
-        // waves running different ones at once miss in the instruction cache
-        static_for<0, 6>([&](auto FT) {
-            constexpr int Fc = decltype(FT)::value;
-            const int lo = Fc == 0 ? 0 : sh.wc[kWcTypeOff + Fc - 1], hi = sh.wc[kWcTypeOff + Fc];
-            if (lo == hi) return;   // uniform: offsets read after the barrier
-            for (int it = lo + tid; it < hi; it += T) wave_eval<Fc>(x, sh, J, W, G, WT, W.wready[it] & 0x1fffffff, base, R, stamp0);
+        // this member's share of the list, grouped by face type in LDS so that
+        // waves of one type evaluate together (the six evaluation routines are
+        // large; waves running different ones at once miss in the instruction
+        // cache); each evaluation queues the neighbours it leaves ready
+        const long long c0 = (long long)ncur * member / K, c1 = (long long)ncur * (member + 1) / K;
+        for (long long s0 = c0; s0 < c1; s0 += L) {
+            const int m = (int)(c1 - s0 < L ? c1 - s0 : L);
+            for (int f = tid; f < 6; f += T) sh.wc[kWcTypeCnt + f] = 0;
             x.sync();
-        });
-        if (tid == 0) sh.st[kStatEvals] += ncur;
-        x.sync();
+            for (int it = tid; it < m; it += T) x.atomic_add(&sh.wc[kWcTypeCnt + (int)((unsigned)cur[s0 + it] >> 29)], 1);
+            x.sync();
+            if (tid == 0) {
+                int o = 0;
+                for (int f = 0; f < 6; f++) {
+                    sh.wc[kWcTypeOff + f] = o;
+                    o += sh.wc[kWcTypeCnt + f];
+                }
+            }
+            x.sync();
+            for (int it = tid; it < m; it += T) {
+                const int w = cur[s0 + it];
+                lbuf[x.atomic_add(&sh.wc[kWcTypeOff + (int)((unsigned)w >> 29)], 1)] = w;
+            }
+            x.sync();
+            // each type's offset now ends its range
+            static_for<0, 6>([&](auto FT) {
+                constexpr int Fc = decltype(FT)::value;
+                const int lo = Fc == 0 ? 0 : sh.wc[kWcTypeOff + Fc - 1], hi = sh.wc[kWcTypeOff + Fc];
+                if (lo == hi) return;   // uniform: offsets read after the barrier
+                for (int it = lo + tid; it < hi; it += T)
+                    wave_eval<Fc>(x, sh, J, W, G, WT, Q, CL, lbuf[it] & 0x1fffffff, base, R, stamp0);
+                x.sync();
+            });
+        }
+        if (lead && tid == 0) {
+            sh.st[kStatRoundsWave] += 1;
+            sh.st[kStatEvals] += ncur;
+        }
+        if (!(alive = tsync())) break;
+        if (tm) {
+            // closing phase: every key written this round is visible now
+            const int ncl = wc[kWcNClose];
+            if (ncl > W.wcap) { ok = false; break; }   // uniform: read after the barrier
+            for (long long it = first; it < ncl; it += stride) wave_close(x, J, W, WT, Q, W.wclose[it]);
+            if (!(alive = tsync())) break;
+        }
         const long long tr2 = tid == 0 ? x.clock() : 0;
-        // carried-register requests: find each one's predecessor
-        for (int c = 0; c < kWaveCarryCap; c++) {
-            int *q = WT.car + kWaveCarryInts * c;
-            if (q[5] != 1) continue;   // uniform: read after the barrier
-            for (int guard = 0; guard < 4 * nev + 8; guard++) {
-                const int cand = q[4];
-                if (cand >= 0) {
-                    const unsigned s = W.wst[cand];
-                    if (!wv_final_in(s, base)) break;            // wait for it
-                    if (s & 2u) {                                // it set them: its t[0]/t[4]
+        if (lead) {
+            // carried-register requests: find each one's predecessor (a failure
+            // is seen by every member when the next round starts)
+            bool fail = false;
+            for (int c = 0; c < kWaveCarryCap && !fail; c++) {
+                int *q = WT.car + kWaveCarryInts * c;
+                if (q[5] != 1) continue;   // uniform: read after the barrier
+                for (int guard = 0; guard < 4 * nev + 8; guard++) {
+                    const int cand = q[4];
+                    if (cand >= 0) {
+                        const unsigned s = W.wst[cand];
+                        if (!wv_final_in(s, base)) break;            // wait for it
+                        if (s & 2u) {                                // it set them: its t[0]/t[4]
+                            x.sync();
+                            if (tid == 0) {
+                                W.wt0[q[0]] = W.wt0[cand];
+                                W.wt4[q[0]] = W.wt4[cand];
+                                q[5] = 2;
+                                wave_queue(x, Q, q[0], WT.type(q[1]));
+                            }
+                            x.sync();
+                            break;
+                        }
+                        x.sync();
+                        if (tid == 0) q[4] = -1;   // it passed its own carry on: search below it
+                        x.sync();
+                        continue;
+                    }
+                    const int e = q[6];
+                    if (e < 0) {   // before the first face: the pass's carry-in
                         x.sync();
                         if (tid == 0) {
-                            W.wt0[q[0]] = W.wt0[cand];
-                            W.wt4[q[0]] = W.wt4[cand];
+                            W.wt0[q[0]] = cin0;
+                            W.wt4[q[0]] = cin4;
                             q[5] = 2;
-                            wave_queue(x, sh, W, q[0], WT.type(q[1]));
+                            wave_queue(x, Q, q[0], WT.type(q[1]));
                         }
                         x.sync();
                         break;
                     }
-                    x.sync();
-                    if (tid == 0) q[4] = -1;   // it passed its own carry on: search below it
-                    x.sync();
-                    continue;
-                }
-                const int e = q[6];
-                if (e < 0) {   // before the first face: the pass's carry-in
+                    const unsigned long long bound = ((unsigned long long)(unsigned)q[3] << 32) | (unsigned)q[2];
+                    if (WT.fsize(e) > 0 && *WT.pk(e) != 0) break;   // keys of that face not all final yet
+                    unsigned long long best = ~0ull;
+                    if (WT.fsize(e) > 0) {
+                        best = wave_scan(x, sh, J, W, WT, e, bound, false, base);
+                        if (tid == 0) sh.st[kStatWaveScans] += 1;
+                        if (sh.wc[kWcScanBad]) {   // a key not final although its face count says so
+                            fail = true;
+                            break;
+                        }
+                    }
                     x.sync();
                     if (tid == 0) {
-                        W.wt0[q[0]] = cin0;
-                        W.wt4[q[0]] = cin4;
-                        q[5] = 2;
-                        wave_queue(x, sh, W, q[0], WT.type(q[1]));
+                        if (best == ~0ull) {   // continue at the last node of the previous face
+                            q[6] = e - 1;
+                            q[2] = -1;
+                            q[3] = -1;
+                        } else {
+                            FaceGeom fg;
+                            wave_face_geom(WT, e, fg);
+                            int ci, cj, ck;
+                            face_coords(fg, (int)(best & 0xffffffffu), ci, cj, ck);
+                            q[4] = (int)((long long)ck * nxy + (long long)cj * nx + ci);
+                            q[2] = (int)(unsigned)(best & 0xffffffffu);
+                            q[3] = (int)(unsigned)(best >> 32);
+                        }
                     }
                     x.sync();
-                    break;
                 }
-                const unsigned long long bound = ((unsigned long long)(unsigned)q[3] << 32) | (unsigned)q[2];
-                if (WT.fsize(e) > 0 && *WT.pk(e) != 0) break;   // keys of that face not all final yet
-                unsigned long long best = ~0ull;
-                if (WT.fsize(e) > 0) {
-                    best = wave_scan(x, sh, J, W, WT, e, bound, false, base);
-                    if (tid == 0) sh.st[kStatWaveScans] += 1;
-                    if (sh.wc[kWcScanBad]) {   // a key not final although its face count says so
-                        if (tid == 0) sh.st[kStatError] |= 128;
-                        ok = false;
-                        break;
-                    }
-                }
-                x.sync();
-                if (tid == 0) {
-                    if (best == ~0ull) {   // continue at the last node of the previous face
-                        q[6] = e - 1;
-                        q[2] = -1;
-                        q[3] = -1;
-                    } else {
-                        FaceGeom fg;
-                        wave_face_geom(WT, e, fg);
-                        int ci, cj, ck;
-                        face_coords(fg, (int)(best & 0xffffffffu), ci, cj, ck);
-                        q[4] = (int)((long long)ck * nxy + (long long)cj * nx + ci);
-                        q[2] = (int)(unsigned)(best & 0xffffffffu);
-                        q[3] = (int)(unsigned)(best >> 32);
-                    }
-                }
-                x.sync();
             }
-            if (!ok) break;
+            x.sync();
+            if (fail && tid == 0) wc[kWcErr] = 1;
         }
-        x.sync();
-        if (tid == 0) {
+        if (tid == 0 && lead) {
             const long long tr3 = x.clock();
             sh.st[kStatTimeWaveGroup] += (int)(tr1 - tr0);
             sh.st[kStatTimeWaveEval] += (int)(tr2 - tr1);
             sh.st[kStatTimeWaveCarry] += (int)(tr3 - tr2);
         }
-        if (!ok) break;
         R++;
     }
+    // every member leaves the rounds at the same barrier; the members' counts
+    // go to the leader
+    if (tm && alive) {
+        if (tid == 0) {
+            x.atomic_add(&tm->done, sh.wc[kWcDone]);
+            if (!lead) {
+                for (int w = 0; w < 7; w++)
+                    if (sh.headw[w]) x.atomic_add(&tm->headw[w], sh.headw[w]);
+                x.atomic_add(&tm->seeds, sh.st[kStatWaveSeeds]);
+                x.atomic_add(&tm->carry, sh.st[kStatWaveCarry]);
+                sh.st[kStatWaveSeeds] = 0;
+                sh.st[kStatWaveCarry] = 0;
+            }
+        }
+        alive = tsync();
+    }
+    if (!alive) ok = false;
+    if (!lead) return ok;
+    if (tm && !alive) {
+        // a member gave up: wait until the others have left the slot's state
+        if (tid == 0) {
+            while (x.atomic_add(&tm->inpass, 0) != 0) x.pause();
+            x.fence_acquire();
+        }
+        x.sync();
+    }
     // every node of the pass must have been evaluated exactly once
-    if (ok && sh.wc[kWcDone] != sh.wc[kWcTotal]) ok = false;
-    if (tid == 0 && !ok) sh.st[kStatError] |= 128;
+    if (ok && (tm ? tm->done : sh.wc[kWcDone]) != sh.wc[kWcTotal]) ok = false;
+    if (tid == 0) {
+        if (tm && alive) {
+            for (int w = 0; w < 7; w++) sh.headw[w] += tm->headw[w];
+            sh.st[kStatWaveSeeds] += tm->seeds;
+            sh.st[kStatWaveCarry] += tm->carry;
+        }
+        if (!ok) sh.st[kStatError] |= 128;
+    }
     x.sync();
     if (ok) {
         // t[0]/t[4] leaving the pass: those of the last node in sequential order
@@ -688,6 +924,81 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     }
     x.sync();
     return ok;
+}
+
+// The grid's row and layer tables into LDS (filled in J.geo_fill first when
+// the host sent no tables) and the stencils' geometry constants.
+template <class X, class SH>
+FD_INLINE void load_geo(X &x, SH &sh, const TableJob &J, Geo &G) {
+    const double *gbuf = J.geo_fill ? J.geo_fill : J.geo;
+    const long long nrl = geo_rows_doubles(J.ny) + geo_layers_doubles(J.nz);
+    for (long long q = x.tid(); q < nrl; q += x.nthr()) sh.geol[q] = gbuf[q];
+    G.h = J.h; G.dq = J.dq; G.df = J.df; G.y0 = J.y0; G.z0r = J.z0r; G.sinq = J.sinq;
+    G.rh = 1.0 / J.h;
+    G.ihsq = 1. / (J.h * J.h);
+    G.idqsq = 1. / (J.dq * J.dq);
+    G.rowt = (GeoPtr)sh.geol;
+    G.layt = G.rowt + geo_rows_doubles(J.ny);
+    G.nx = J.nx; G.ny = J.ny; G.nz = J.nz;
+    x.sync();
+}
+
+// Member `member` (> 0) of the team solving table J: joins every wavefront
+// pass the leader opens, with the leader's slot, until the leader releases
+// the team.  Its own slot is not used.
+template <class X, class SH>
+FD_HD void team_helper(X &x, SH &sh, const TableJob &J, TeamCtl *tm, int K, int member) {
+    const int tid = x.tid();
+    int seen = 0, gen = 0;
+    bool geo = false;
+    Geo G;
+    for (;;) {
+        if (tid == 0) {
+            // wait for the next pass (or the end of the table)
+            int go = 0;
+            const long long w0 = x.clock();
+            for (;;) {
+                if (x.load_relaxed(&tm->abort) || x.load_relaxed(&tm->release)) break;
+                if (x.load_relaxed(&tm->seq) > seen) {
+                    // inside the pass unless a member gave up meanwhile (the
+                    // leader waits for inpass 0 before it reuses its slot)
+                    x.atomic_add(&tm->inpass, 1);
+                    if (x.atomic_add(&tm->abort, 0)) x.atomic_add(&tm->inpass, -1);
+                    else go = 1;
+                    break;
+                }
+                if (x.clock() - w0 > kTeamWaitTicks) { x.atomic_exch(&tm->abort, 1); break; }
+                x.pause();
+            }
+            x.fence_acquire();
+            sh.wc[kWcTeamOk] = go;
+        }
+        x.sync();
+        const int go = sh.wc[kWcTeamOk];
+        x.sync();
+        if (!go) break;
+        seen++;
+        if (!geo) {
+            // the leader's checks passed: its geometry tables fit the arena
+            load_geo(x, sh, J, G);
+            geo = true;
+        }
+        const Scratch W = tm->W;
+        int box0[3][2], flags0[6];
+        for (int a = 0; a < 3; a++) { box0[a][0] = tm->box0[a][0]; box0[a][1] = tm->box0[a][1]; }
+        for (int f = 0; f < 6; f++) flags0[f] = tm->flags0[f];
+        const int rad0 = tm->rad0, radius = tm->radius;
+        if (tid == 0) {
+            for (int w = 0; w < 7; w++) sh.headw[w] = 0;
+            for (int q = 0; q < kStatCount; q++) sh.st[q] = 0;
+        }
+        x.sync();
+        wave_pass(x, sh, J, W, G, box0, flags0, rad0, radius, tm, K, member, &gen);
+        team_publish(x);
+        if (tid == 0) x.atomic_add(&tm->inpass, -1);
+        x.sync();
+    }
+    x.sync();
 }
 
 }  // namespace fdsweep

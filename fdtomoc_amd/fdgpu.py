@@ -20,7 +20,7 @@ STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level
               "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
     ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1",
      "wave_passes", "rounds_wave", "wave_seeds", "wave_carry", "wave_scans", "t_wave",
-     "t_wave_group", "t_wave_eval", "t_wave_carry"]
+     "t_wave_group", "t_wave_eval", "t_wave_carry", "team"]
 
 BATCHED, PLAIN = 0, 1
 
@@ -310,6 +310,8 @@ def lib():
         L.fdgpu_set_workgroup.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_set_wave.argtypes = [C.c_void_p, C.c_int]
         L.fdgpu_get_wave.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.fdgpu_set_team.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.fdgpu_get_team.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdgpu_serve_config_defaults.argtypes = [C.POINTER(CServeConfig)]
         L.fdgpu_serve_config_defaults.restype = None
         L.fdgpu_serve_start.argtypes = [C.c_void_p, C.POINTER(CServeConfig)]
@@ -567,6 +569,17 @@ class Context:
         _check(lib().fdgpu_get_wave(self._h, C.byref(b)), "fdgpu_get_wave")
         return b.value
 
+    def set_team(self, size: int, share_pct: int = 200):
+        """Teams (fdgpu_set_team): the wavefront passes of each batched table
+        of at least share_pct % of the mean nodes per workgroup shared by
+        `size` workgroups (0 automatic, 1 never); results do not depend on it."""
+        _check(lib().fdgpu_set_team(self._h, int(size), int(share_pct)), "fdgpu_set_team")
+
+    def get_team(self) -> tuple[int, int]:
+        a, b = C.c_int(), C.c_int()
+        _check(lib().fdgpu_get_team(self._h, C.byref(a), C.byref(b)), "fdgpu_get_team")
+        return a.value, b.value
+
     def get_schedule(self) -> int:
         b = C.c_int()
         _check(lib().fdgpu_get_schedule(self._h, C.byref(b)), "fdgpu_get_schedule")
@@ -574,9 +587,16 @@ class Context:
 
     def resolved_again(self) -> int:
         """Tables solved a second time after a failed kernel consistency check."""
-        c = (C.c_longlong * 1)()
-        _check(lib().fdgpu_context_counters(self._h, c, 1), "fdgpu_context_counters")
+        c = (C.c_longlong * 2)()
+        _check(lib().fdgpu_context_counters(self._h, c, 2), "fdgpu_context_counters")
         return int(c[0])
+
+    def wave_again(self) -> int:
+        """Tables solved a second time, face by face, after a wavefront pass
+        that could not finish."""
+        c = (C.c_longlong * 2)()
+        _check(lib().fdgpu_context_counters(self._h, c, 2), "fdgpu_context_counters")
+        return int(c[1])
 
     def last_kernel_ms(self) -> float:
         return float(lib().fdgpu_last_kernel_ms(self._h))

@@ -320,10 +320,22 @@ int fdgpu_set_workgroup(fdgpu_ctx *ctx, int lanes);
 int fdgpu_set_wave(fdgpu_ctx *ctx, int flags);
 int fdgpu_get_wave(fdgpu_ctx *ctx, int *flags);
 
+/* Teams (no reference counterpart; results are identical for every choice):
+ * in a batched solve with wavefront passes, each of the largest tables --
+ * the leading tables of the largest-first queue with at least `share_pct` %
+ * of the launch's mean nodes per workgroup, at most workgroups / size of
+ * them -- has its wavefront passes shared by `size` workgroups on as many
+ * CUs (2..16), instead of one.  size 0: automatic (4), 1: never.  Default
+ * 0 and 200. */
+int fdgpu_set_team(fdgpu_ctx *ctx, int size, int share_pct);
+int fdgpu_get_team(fdgpu_ctx *ctx, int *size, int *share_pct);
+
 /* Counters of the context (no reference counterpart): c[0] tables solved a
  * second time, face by face, because one of the sweep kernel's own
  * consistency checks failed on the first solve (the second solve's results
- * are kept; a second failure is FDGPU_EKERNEL); n <= 1 values. */
+ * are kept; a second failure is FDGPU_EKERNEL); c[1] tables solved a second
+ * time because a wavefront pass could not finish (list capacity, a request
+ * that never resolved, a team member that waited too long); n <= 2 values. */
 int fdgpu_context_counters(fdgpu_ctx *ctx, long long *c, int n);
 
 /* Copies a resident table (nx*ny*nz floats) to host memory. */

@@ -42,6 +42,12 @@ struct HostExec {
         return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
                    .count() / 10;
     }
+    void fence_release() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+    void fence_acquire() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+    int load_relaxed(const int *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+    void wait_stores() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+    void store_release(int *p, int v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
+    void pause() { std::this_thread::yield(); }
 };
 }  // namespace
 
@@ -64,24 +70,42 @@ struct EmuPar {
 // (Scratch::stamp), the paths the device takes for faces too large for LDS.
 using SharedGlobal = fdsweep::SharedT<fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles, 0>;
 
+// One table on `team` workgroups of nthreads threads each: workgroup 0 solves
+// it (the leader), the others join its wavefront passes as the device's team
+// members do (team 1: a lone workgroup).
 template <class SH>
-static void run_threads(int nthreads, const fdsweep::TableJob &J, const fdsweep::Scratch &W) {
-    std::unique_ptr<SH> shp(new SH());
-    SH &sh = *shp;
-    std::memset(&sh, 0, sizeof sh);
-    std::barrier<> bar(nthreads);
+static void run_threads(int nthreads, int team, const fdsweep::TableJob &J, const fdsweep::Scratch &W) {
+    std::vector<std::unique_ptr<SH>> shs;
+    std::vector<std::unique_ptr<std::barrier<>>> bars;
+    for (int g = 0; g < team; g++) {
+        shs.emplace_back(new SH());
+        std::memset(shs.back().get(), 0, sizeof(SH));
+        bars.emplace_back(new std::barrier<>(nthreads));
+    }
+    std::unique_ptr<fdsweep::TeamCtl> tm(new fdsweep::TeamCtl());
+    std::memset(tm.get(), 0, sizeof(fdsweep::TeamCtl));
     std::vector<std::thread> th;
-    for (int t = 0; t < nthreads; t++)
-        th.emplace_back([&, t]() {
-            HostExec x{t, nthreads, &bar};
-            fdsweep::solve_table(x, sh, J, W);
-        });
+    for (int g = 0; g < team; g++)
+        for (int t = 0; t < nthreads; t++)
+            th.emplace_back([&, g, t]() {
+                HostExec x{t, nthreads, bars[(size_t)g].get()};
+                SH &sh = *shs[(size_t)g];
+                if (team == 1) {
+                    fdsweep::solve_table(x, sh, J, W);
+                } else if (g == 0) {
+                    fdsweep::solve_table(x, sh, J, W, tm.get(), team);
+                    fdsweep::team_release(x, tm.get());
+                } else {
+                    fdsweep::team_helper(x, sh, J, tm.get(), team, g);
+                }
+            });
     for (auto &t : th) t.join();
 }
 
 // mode: bit0 per-node schedule state always in global scratch (else in the
 // arena where a face fits, as the device builds); bit1 the first pass as a
-// wavefront, bit2 reverse passes as wavefronts.
+// wavefront, bit2 reverse passes as wavefronts; bits 3-7 the number of
+// workgroups sharing the wavefront passes (0 or 1: one).
 extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const float *extra, int nthreads,
                          int *stats, int change_budget, int mode) {
     const int global_state = mode & 1;
@@ -116,10 +140,10 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     const long long nn = (long long)p.nx * p.ny * p.nz;
     std::vector<unsigned> wst((size_t)nn, 0u);
     std::vector<float> wt0((size_t)nn), wt4((size_t)nn);
-    std::vector<int> wl((size_t)nn), wtmp((size_t)nn), wrd((size_t)nn);
+    std::vector<int> wl((size_t)nn), wtmp((size_t)nn), wrd((size_t)nn), wcl((size_t)nn);
     unsigned wround = 1;
     W.wst = wst.data(); W.wt0 = wt0.data(); W.wt4 = wt4.data();
-    W.wcnt = wl.data(); W.wtmp = wtmp.data(); W.wready = wrd.data();
+    W.wcnt = wl.data(); W.wtmp = wtmp.data(); W.wready = wrd.data(); W.wclose = wcl.data();
     W.wcap = nn; W.wn = nn; W.wround = &wround;
     fdsweep::TableJob J;
     std::memset(&J, 0, sizeof J);
@@ -148,7 +172,8 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     J.wave = (mode >> 1) & 3;
     std::vector<double> geo((size_t)fdsweep::geo_doubles(p.ny, p.nz));
     J.geo_fill = geo.data();
-    if (global_state) run_threads<SharedGlobal>(nthreads, J, W);
-    else run_threads<fdsweep::Shared>(nthreads, J, W);
+    const int team = std::max(1, std::min(fdsweep::kTeamMax, (mode >> 3) & 31));
+    if (global_state) run_threads<SharedGlobal>(nthreads, team, J, W);
+    else run_threads<fdsweep::Shared>(nthreads, team, J, W);
     return 0;
 }

@@ -107,3 +107,40 @@ def test_wavefront_carried_registers_exercised(emu):
     assert tot["wave_carry"] > 0 and tot["wave_scans"] >= tot["wave_carry"]
     assert tot["wave_seeds"] > 0
     assert tot["rounds_wave"] < tot["rounds_face"]
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_team_wavefront_passes_match_oracle(emu, seed):
+    """Wavefront passes shared by a team of 2-4 workgroups (each of 1-5 host
+    threads, meeting at the team barrier the device uses): the same grid bit
+    for bit as the oracle, the same passes, every evaluation accounted for."""
+    par, v = make_case(seed + 900)
+    s = ob.slowness(v)
+    t_ref, st = ob.solve(par, s)
+    team = [2, 3, 4][seed % 3]
+    nthreads = [1, 2, 5][seed % 3 if seed % 2 else (seed + 1) % 3]
+    wave = [1, 3][seed % 2]
+    t, est = emu.solve(par, s, nthreads, change_budget=[0, 8][seed % 2], wave=wave, team=team,
+                       global_state=bool(seed % 5 == 4))
+    assert est["error"] == 0
+    assert est["team"] == team
+    assert est["passes"] == st["passes"] and est["restarts"] == st["restarts"]
+    assert est["wave_passes"] == (st["passes"] if wave == 3 else 1)
+    assert int((t.view(np.uint32) != t_ref.view(np.uint32)).sum()) == 0
+
+
+def test_team_carried_registers_and_counts(emu):
+    """Teams resolve the carried-register requests their members file and
+    report the same seeds, requests and head waves as one workgroup does."""
+    tot = {}
+    for seed in range(700, 706):
+        par, v = make_case(seed)
+        s = ob.slowness(v)
+        t1, e1 = emu.solve(par, s, 4, wave=3)
+        t3, e3 = emu.solve(par, s, 4, wave=3, team=3)
+        assert int((t1.view(np.uint32) != t3.view(np.uint32)).sum()) == 0
+        for k in ("wave_seeds", "wave_carry", "passes", "restarts", "rounds_wave", "evals") + \
+                tuple(f"headw{i}" for i in range(7)):
+            assert e1[k] == e3[k], k
+        tot["wave_carry"] = tot.get("wave_carry", 0) + e3["wave_carry"]
+    assert tot["wave_carry"] > 0

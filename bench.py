@@ -201,8 +201,13 @@ PROFILED_TRAFFIC = {
 
 
 def profiled_traffic(workload: str, sched: dict):
+    """The profiled HBM traffic of the schedule a run picked: wavefront flags,
+    CHANGE budget and, with wavefronts, the team choice (none: [1, x])."""
+    team = list(sched.get("team") or [1, 0])
     for p in PROFILED_TRAFFIC.get(workload, []):
-        if p["wave"] == sched.get("wave") and p["change_budget"] == sched.get("change_budget"):
+        if p["wave"] != sched.get("wave") or p["change_budget"] != sched.get("change_budget"):
+            continue
+        if p["wave"] == 0 or list(p.get("team", [1, 0])) == team or (team[0] == 1 and p.get("team", [1])[0] == 1):
             return p["bytes"], p["source"]
     return None, None
 

@@ -1,9 +1,11 @@
 """Autotuner of the sweep's schedules.
 
-Two knobs, both without effect on the results.  The wavefront flags
+Three knobs, none with an effect on the results.  The wavefront flags
 (`fdgpu_set_wave`): every pass face by face (flags 0), the first pass as one
 dataflow wavefront over the table (flags 1), or every pass as a wavefront
-(flags 3); passes run face by face use the per-face schedule below.
+(flags 3); passes run face by face use the per-face schedule below.  Teams
+(`fdgpu_set_team`): the largest tables' wavefront passes shared by several
+workgroups, planned per table by the library.
 
 Reverse-pass faces first try the CHANGE schedule (evaluate the whole face at
 once, then re-evaluate what changed); when changes cascade through a face, the
@@ -26,6 +28,8 @@ import time
 from . import fdgpu
 
 CANDIDATES = (0, 2, 4, 8, 16, 32)
+# (size, share) of fdgpu_set_team: no teams, and the planner at 150 % load
+TEAM_CANDIDATES = ((1, 0), (0, 150))
 
 
 def tune_change_budget(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES, sample: int | None = None,
@@ -64,8 +68,10 @@ def tune_schedule(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES, samp
                   objective: str = "wall") -> tuple[tuple[int, int], dict]:
     """Times the passes face by face under each candidate budget (flags 0), the
     first pass as a wavefront with reverse passes face by face (flags 1), and
-    every pass as a wavefront (flags 3); keeps the fastest on `ctx` and returns
-    ((wave flags, budget), {"w<flags>/b<budget>" or "w3": seconds})."""
+    every pass as a wavefront (flags 3); with wavefronts, also the team
+    choices of TEAM_CANDIDATES against the context's own.  Keeps the fastest
+    on `ctx` and returns ((wave flags, budget), {"w<flags>/b<budget>", "w3" or
+    "w<flags>/team<size>/<share>": seconds})."""
     ctx.set_wave(1)
     best_b, t1 = tune_change_budget(ctx, specs, candidates, sample, objective)
     timings = {f"w1/b{b}": v for b, v in t1.items()}
@@ -79,4 +85,19 @@ def tune_schedule(ctx: "fdgpu.Context", specs: list, candidates=CANDIDATES, samp
     wave, budget = min(options, key=options.get)
     ctx.set_wave(wave)
     ctx.set_schedule(budget)
+    if wave != 0:
+        # teams for the largest tables' wavefront passes: the context's choice
+        # (timed above) against none and against a plan at a higher load
+        base_team = ctx.get_team()
+        best_t, best_s = base_team, options[(wave, budget)]
+        for cand in TEAM_CANDIDATES:
+            if cand == base_team:
+                continue
+            ctx.set_team(*cand)
+            _, tt = tune_change_budget(ctx, specs, (budget,), sample, objective)
+            timings[f"w{wave}/team{cand[0]}/{cand[1]}"] = tt[budget]
+            if tt[budget] < best_s:
+                best_t, best_s = cand, tt[budget]
+        ctx.set_team(*best_t)
+        ctx.set_schedule(budget)
     return (wave, budget), timings

@@ -36,12 +36,13 @@ extern "C" hipError_t fd_noalloc_scratch(void **ptr);
 // One table by one wide workgroup, descriptor and scratch layout by value.
 extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::Scratch *W, hipStream_t stream);
 
-// The persistent sweep over njobs tables on nslots workgroups; the first
-// nteam tables are each solved by a team of team_size workgroups (nteam 0:
-// none).
+// The persistent sweep over njobs tables on nslots workgroups; table i <
+// nteam is solved by a team of d_team_first[i + 1] - d_team_first[i]
+// workgroups (nteam 0: every table by one).
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue,
-                                      fdsweep::TeamCtl *d_teams, int nteam, int team_size, hipStream_t stream);
+                                      fdsweep::TeamCtl *d_teams, const int *d_team_first, int nteam,
+                                      hipStream_t stream);
 // Kernel address of a sweep build, for occupancy queries.
 extern "C" const void *fd_sweep_kernel_ptr(int variant);
 // Sets the table queue counter to 0 and clears the control words of nteam

@@ -278,8 +278,14 @@ int ensure_jobs(fdgpu_ctx *c, int n) {
     return FDGPU_OK;
 }
 
-// Workgroups per team when the context leaves the choice automatic.
-constexpr int kTeamAuto = 4;
+// Tables below this many nodes are never solved by a team: their rounds hold
+// too few nodes to share.
+constexpr long long kTeamMinNodes = 1LL << 20;
+
+// Team blocks and the teams' first tickets (cap + 1 ints) in one allocation.
+size_t team_bytes(int cap) {
+    return cap > 0 ? sizeof(fdsweep::TeamCtl) * (size_t)cap + sizeof(int) * (size_t)(cap + 1) : 0;
+}
 
 // Team blocks for nteam teams (the captured launches keep their address, so
 // growing them drops the graphs).
@@ -287,30 +293,70 @@ int ensure_teams(fdgpu_ctx *c, int nteam) {
     if (nteam <= c->teams_cap) return FDGPU_OK;
     (void)hipStreamSynchronize(c->stream);
     clear_graphs(c);
-    dev_free(c, c->d_teams, sizeof(fdsweep::TeamCtl) * (size_t)c->teams_cap);
+    dev_free(c, c->d_teams, team_bytes(c->teams_cap));
     c->d_teams = nullptr;
+    c->d_team_first = nullptr;
     c->teams_cap = 0;
     const int cap = std::max(nteam, 64);
-    int rc = dev_alloc(c, (void **)&c->d_teams, sizeof(fdsweep::TeamCtl) * (size_t)cap);
+    int rc = dev_alloc(c, (void **)&c->d_teams, team_bytes(cap));
     if (rc) return rc;
+    c->d_team_first = (int *)(c->d_teams + cap);
     c->teams_cap = cap;
     return FDGPU_OK;
+}
+
+// Team sizes of the leading tables of a largest-first queue (sizes n, nodes),
+// all tables of at most `cap` workgroups' worth of time.  A table's time is
+// modelled as its node count times the measured time ratio of a team of K
+// (one table of 4 M nodes alone on MI355X: 2 workgroups 0.71 of one
+// workgroup's time, 4: 0.52, 8: 0.41, 16: 0.37); the plan gives each table
+// the smallest team that finishes within a batch time T, and takes the
+// smallest T whose workgroup time fits `cap` workgroups at `load` (1: every
+// workgroup busy for T).  Returns the team sizes of the leading tables
+// (sizes 1 end the list).
+std::vector<int> plan_teams(const std::vector<long long> &n, int cap, double load) {
+    static const double ratio[17] = {0, 1.0, 0.71, 0.60, 0.52, 0.48, 0.45, 0.43, 0.41,
+                                     0.40, 0.395, 0.39, 0.385, 0.38, 0.375, 0.372, 0.37};
+    std::vector<int> K(n.size(), 1);
+    if (n.empty()) return {};
+    auto assign = [&](double T) {
+        double cost = 0.0;
+        for (size_t i = 0; i < n.size(); i++) {
+            int k = 1;
+            while (k < fdsweep::kTeamMax && n[i] >= kTeamMinNodes && (double)n[i] * ratio[k] > T) k++;
+            K[i] = k;
+            cost += k * (double)n[i] * ratio[k];
+        }
+        return cost;
+    };
+    double lo = (double)n[0] * ratio[fdsweep::kTeamMax], hi = (double)n[0];
+    for (int it = 0; it < 60; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (assign(mid) <= cap * mid * load) hi = mid;
+        else lo = mid;
+    }
+    assign(hi);
+    size_t m = 0;
+    while (m < K.size() && K[m] > 1) m++;
+    K.resize(m);
+    return K;
 }
 
 // The batched launch as a graph (a queue-reset kernel, then the persistent
 // sweep kernel), captured once per (build, table count, workgroups) and
 // replayed; descriptors and scratch pointers are fixed for its lifetime.
-int sweep_graph(fdgpu_ctx *c, int variant, int n, int nslots, int nteam, int K, hipGraphExec_t *out) {
-    // n < 2^24, nslots < 2^20, nteam < 2^11, K <= 16
-    const long long key = ((long long)variant << 60) | ((long long)K << 55) | ((long long)nteam << 44) |
-                          ((long long)nslots << 24) | (long long)n;
+int sweep_graph(fdgpu_ctx *c, int variant, int n, int nslots, int nteam, hipGraphExec_t *out) {
+    // n < 2^24, nslots < 2^20, nteam < 2^11 (the team sizes are read from
+    // d_team_first at run time)
+    const long long key = ((long long)variant << 60) | ((long long)nteam << 44) | ((long long)nslots << 24) |
+                          (long long)n;
     auto it = c->graphs.find(key);
     if (it != c->graphs.end()) { *out = it->second; return FDGPU_OK; }
     hipGraph_t graph = nullptr;
     FD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = fd_launch_reset(c->d_queue, c->d_teams, nteam, c->stream);
-    hipError_t e2 = fd_launch_sweep(variant, c->d_jobs, n, c->d_slots, nslots, c->d_queue, c->d_teams, nteam, K,
-                                    c->stream);
+    hipError_t e2 = fd_launch_sweep(variant, c->d_jobs, n, c->d_slots, nslots, c->d_queue, c->d_teams,
+                                    c->d_team_first, nteam, c->stream);
     hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -396,30 +442,50 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
         if (geo_need > fd_geo_lds_capacity(variant)) variant = FD_WIDE;   // same results, larger LDS
     }
     const int cap = c->num_cu * c->wg_per_cu[variant];
-    // teams: the largest tables' wavefront passes shared by K workgroups each
-    // -- the leading tables of the queue with at least team_share % of the
-    // mean nodes per workgroup of a full device, whose passes can run as
-    // wavefronts -- as many as the idle workgroups hold when the tables are
-    // fewer than the workgroups, else at most a K-th of the workgroups
-    int nteam = 0, K = 1;
+    // teams: the largest tables' wavefront passes shared by several
+    // workgroups each -- planned (plan_teams) or, with a fixed size, the
+    // leading tables of at least team_share % of the mean nodes per workgroup
+    // of a full device (at most as many as the idle workgroups hold when the
+    // tables are fewer, else a size-th of the workgroups) -- for tables whose
+    // passes can run as wavefronts
+    std::vector<int> teamK;
     if (mode == FDGPU_BATCHED && !no_wave && c->team != 1) {
-        K = std::max(2, std::min(fdsweep::kTeamMax, c->team == 0 ? kTeamAuto : c->team));
         const int arena = variant == FD_NARROW ? kNarrowArenaWords : fdsweep::kArenaWords;
-        const double share = (double)total_nodes / cap * c->team_share / 100.0;
+        int lead = 0;   // leading tables that can run as a team
+        std::vector<long long> sizes((size_t)n);
         for (int t = 0; t < n; t++) {
             const fdsweep::TableJob &J = qjobs[(size_t)t];
-            const long long nn = (long long)J.nx * J.ny * J.nz;
-            if (!(J.wave & 1) || nn >= (1LL << 29) || !fdsweep::wave_fits(J.nx, J.ny, J.nz, arena) ||
-                (double)nn < share)
-                break;
-            nteam++;
+            sizes[(size_t)t] = (long long)J.nx * J.ny * J.nz;
+            if (lead == t && (J.wave & 1) && sizes[(size_t)t] < (1LL << 29) &&
+                fdsweep::wave_fits(J.nx, J.ny, J.nz, arena))
+                lead++;
         }
-        const int most = n < cap ? (cap - n) / (K - 1) : cap / K;
-        nteam = std::min(nteam, std::min(most, 2047));
+        if (c->team == 0) {
+            teamK = plan_teams(sizes, cap, (c->team_share > 0 ? c->team_share : 100) / 100.0);
+        } else {
+            const int K = std::min(fdsweep::kTeamMax, c->team);
+            const double share = (double)total_nodes / cap * (c->team_share > 0 ? c->team_share : 200) / 100.0;
+            const int most = n < cap ? (cap - n) / (K - 1) : cap / K;
+            for (int t = 0; t < n && (int)teamK.size() < most && (double)sizes[(size_t)t] >= share &&
+                            sizes[(size_t)t] >= kTeamMinNodes;
+                 t++)
+                teamK.push_back(K);
+        }
+        if ((int)teamK.size() > lead) teamK.resize((size_t)lead);
+        if (teamK.size() > 2047) teamK.resize(2047);
     }
-    int nslots = mode == FDGPU_BATCHED ? std::min(n + nteam * (K - 1), cap) : 1;
+    int nteam = (int)teamK.size(), extra = 0;
+    for (int k : teamK) extra += k - 1;
+    int nslots = mode == FDGPU_BATCHED ? std::min(n + extra, cap) : 1;
     if (nslots < 1) nslots = 1;
     int rc = ensure_scratch(c, nslots, maxF, maxN, maxD, !no_wave && c->wave != 0);
+    if (nteam > 0 && (rc == FDGPU_ENOMEM || !c->slot_wave)) {
+        // the team members' extra slots do not fit the budget (or the slots
+        // hold no wavefront state): one workgroup per table
+        nteam = 0;
+        nslots = std::max(1, std::min(n, cap));
+        rc = ensure_scratch(c, nslots, maxF, maxN, maxD, !no_wave && c->wave != 0);
+    }
     if (rc) return rc;
     rc = ensure_jobs(c, n);
     if (rc) return rc;
@@ -427,25 +493,29 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
     // serves the team)
     if (nteam > 0 && (!c->slot_wave || qjobs[0].nx * (long long)qjobs[0].ny * qjobs[0].nz > c->slot_node_cap))
         nteam = 0;
+    std::vector<int> first((size_t)nteam + 1, 0);
     if (nteam > 0) {
         rc = ensure_teams(c, nteam);
         if (rc) return rc;
+        for (int i = 0; i < nteam; i++) first[(size_t)i + 1] = first[(size_t)i] + teamK[(size_t)i];
     }
-    if (nteam == 0) K = 1;
     fdsweep::TableJob *d_jobs = c->d_jobs;
     bool ok = hipMemcpyAsync(d_jobs, qjobs.data(), sizeof(fdsweep::TableJob) * (size_t)n, hipMemcpyHostToDevice,
                              c->stream) == hipSuccess;
+    if (nteam > 0)
+        ok = ok && hipMemcpyAsync(c->d_team_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice,
+                                  c->stream) == hipSuccess;
     ok = ok && hipEventRecord(c->ev0, c->stream) == hipSuccess;
     if (mode == FDGPU_BATCHED) {
         // one captured graph per launch shape: queue reset + persistent sweep
         hipGraphExec_t g = nullptr;
-        ok = ok && sweep_graph(c, variant, n, nslots, nteam, K, &g) == FDGPU_OK;
+        ok = ok && sweep_graph(c, variant, n, nslots, nteam, &g) == FDGPU_OK;
         ok = ok && hipGraphLaunch(g, c->stream) == hipSuccess;
     } else {
         for (int t = 0; t < n && ok; t++) {
             ok = fd_launch_reset(c->d_queue, nullptr, 0, c->stream) == hipSuccess;
-            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, nullptr, 0, 1, c->stream) ==
-                           hipSuccess;
+            ok = ok && fd_launch_sweep(FD_WIDE, d_jobs + t, 1, c->d_slots, 1, c->d_queue, nullptr, nullptr, 0,
+                                       c->stream) == hipSuccess;
         }
     }
     ok = ok && hipEventRecord(c->ev1, c->stream) == hipSuccess;
@@ -476,7 +546,14 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
     }
     if (!again.empty()) {
         const float ms0 = c->last_ms;
+        // the first attempt's failure reason and team size stay readable
+        std::vector<std::pair<int, int>> first_try;
+        for (Table *T : again) first_try.emplace_back(T->stats[fdsweep::kStatWaveFail], T->stats[fdsweep::kStatTeam]);
         const int rc2 = run_tables(c, again, mode, true);
+        for (size_t q = 0; q < again.size(); q++) {
+            again[q]->stats[fdsweep::kStatWaveFail] = first_try[q].first;
+            again[q]->stats[fdsweep::kStatTeam] = first_try[q].second;
+        }
         c->last_ms += ms0;
         if (rc2 == FDGPU_EKERNEL) kerr = 1;
         else if (rc2) return rc2;
@@ -575,7 +652,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
     dev_free(c, c->ray_cols, c->ray_cols_bytes);
     dev_free(c, c->ray_blk, c->ray_blk_bytes);
-    dev_free(c, c->d_teams, sizeof(fdsweep::TeamCtl) * (size_t)c->teams_cap);
+    dev_free(c, c->d_teams, team_bytes(c->teams_cap));
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

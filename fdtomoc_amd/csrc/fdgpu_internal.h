@@ -203,6 +203,7 @@ struct fdgpu_ctx {
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;
     fdsweep::TeamCtl *d_teams = nullptr;   // team blocks of batched launches, grown on demand
+    int *d_team_first = nullptr;           // ... and the teams' first tickets (after the blocks)
     int teams_cap = 0;
     fdsweep::TableJob *d_jobs = nullptr;   // descriptors of the current batch
     int jobs_cap = 0;
@@ -218,8 +219,8 @@ struct fdgpu_ctx {
     double loc_timing[4] = {0.0, 0.0, 0.0, 0.0};
     int change_budget = 8;     // per-face schedule knob (fdgpu_set_schedule)
     int wave = 0;              // passes run as wavefronts (fdgpu_set_wave): bit0 first, bit1 reverse
-    int team = 0;              // workgroups per large table's wavefront passes (fdgpu_set_team): 0 automatic
-    int team_share = 200;      // ... for tables of at least this % of the mean nodes per workgroup
+    int team = 0;              // workgroups per large table's wavefront passes (fdgpu_set_team): 0 planned
+    int team_share = 0;        // planned: load %; fixed size: tables of at least this % of the mean share
     // this device's copy of the fixed array behind fdgpu_solve_noalloc
     // (sweep_kernel.hip `scratch`), looked up once at fdgpu_open
     char *d_fixed = nullptr;

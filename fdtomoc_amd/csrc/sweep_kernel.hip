@@ -46,41 +46,49 @@ struct DevExec {
     __device__ __forceinline__ int load_relaxed(const int *p) const {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // this wave's memory operations complete (stores reached the L2)
-    __device__ __forceinline__ void wait_stores() const { __builtin_amdgcn_s_waitcnt(0); }
+    // this wave's stores have reached the L2; also a compiler barrier, so no
+    // store is scheduled after the wait
+    __device__ __forceinline__ void wait_stores() const { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
     __device__ __forceinline__ void store_release(int *p, int v) const {
         __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ __forceinline__ void pause() const { __builtin_amdgcn_s_sleep(2); }
 };
 
-// Tickets from the queue: the first nteam * K go to teams of K workgroups on
-// the first nteam tables (ticket t: table t / K, member t % K), the rest to
-// one table each.  A member waits only for members that have taken their
-// tickets already, so a team forms whenever K workgroups can run at once.
+// Tickets from the queue: table i < nteam has tickets team_first[i] ..
+// team_first[i + 1] - 1 (its team: the first is the leader), every later
+// table one ticket.  A member waits only for members that have taken their
+// tickets already, so a team forms whenever its workgroups can run at once.
 template <int THREADS, int ARENA, int GEOL>
 __global__ void __launch_bounds__(THREADS, 2)
     sphfd_sweep_kernel(const fdsweep::TableJob *jobs, int njobs, const fdsweep::Scratch *slots, int *queue,
-                       fdsweep::TeamCtl *teams, int nteam, int K) {
+                       fdsweep::TeamCtl *teams, const int *team_first, int nteam) {
     __shared__ fdsweep::SharedT<ARENA, GEOL> sh;
     DevExec x;
     const fdsweep::Scratch W = slots[blockIdx.x];
+    const int tt = nteam > 0 ? team_first[nteam] : 0;
     while (true) {
         if (threadIdx.x == 0) sh.job = atomicAdd(queue, 1);
         __syncthreads();
         const int t = sh.job;
         __syncthreads();
-        int j = t, member = 0;
-        if (t < nteam * K) {
-            j = t / K;
-            member = t - j * K;
+        int j, member = 0, K = 1;
+        if (t < tt) {
+            int lo = 0, hi = nteam - 1;   // the last team whose first ticket is <= t
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (team_first[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+            member = t - team_first[lo];
+            K = team_first[lo + 1] - team_first[lo];
         } else {
-            j = t - nteam * (K - 1);
+            j = nteam + (t - tt);
         }
         if (j >= njobs) break;
-        if (j < nteam && member > 0) {
+        if (K > 1 && member > 0) {
             fdsweep::team_helper(x, sh, jobs[j], teams + j, K, member);
-        } else if (j < nteam) {
+        } else if (K > 1) {
             fdsweep::solve_table(x, sh, jobs[j], W, teams + j, K);
             fdsweep::team_release(x, teams + j);
         } else {
@@ -238,17 +246,15 @@ extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::S
 
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue,
-                                      fdsweep::TeamCtl *d_teams, int nteam, int team_size, hipStream_t stream) {
-    if (nteam <= 0 || team_size < 2) {
-        nteam = 0;
-        team_size = 1;
-    }
+                                      fdsweep::TeamCtl *d_teams, const int *d_team_first, int nteam,
+                                      hipStream_t stream) {
+    if (nteam <= 0 || !d_teams || !d_team_first) nteam = 0;
     if (variant == FD_NARROW)
         hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds>), dim3(nslots), dim3(kNarrowThreads),
-                           0, stream, d_jobs, njobs, d_slots, d_queue, d_teams, nteam, team_size);
+                           0, stream, d_jobs, njobs, d_slots, d_queue, d_teams, d_team_first, nteam);
     else
         hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>), dim3(nslots), dim3(kWideThreads), 0,
-                           stream, d_jobs, njobs, d_slots, d_queue, d_teams, nteam, team_size);
+                           stream, d_jobs, njobs, d_slots, d_queue, d_teams, d_team_first, nteam);
     return hipGetLastError();
 }
 

@@ -107,6 +107,9 @@ enum {
     kStatTimeWaveEval,  // evaluations and queueing (phase 3)
     kStatTimeWaveCarry, // carried-register requests (phase 4)
     kStatTeam,          // workgroups that solved the table (a team's size, else 1)
+    kStatWaveFail,      // why a wavefront pass failed: 1 list capacity, 2 a key not final at a
+                        // request's face scan, 3 requests that never resolved, 4 round limit,
+                        // 5 closing list capacity, 6 evaluation count, 7 a member waited too long
     kStatCount
 };
 

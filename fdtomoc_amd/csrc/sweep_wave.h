@@ -79,10 +79,12 @@ FD_INLINE bool wave_fits(int nx, int ny, int nz, int arena) {
     return wave_arena_words(nx, ny, nz) + kWaveLocalMin <= arena && nx + ny + nz <= kTeamMaxEv;
 }
 
-// Control words in SharedT::wc (kWcNCur, kWcNNext, kWcErr, kWcWaiting and
-// kWcNClose are the team's words in TeamCtl::wc when the pass runs on a team).
+// Control words in SharedT::wc (the list sizes kWcNCur/kWcNNext and
+// kWcNClose/kWcNClose1, indexed by round parity, kWcErr and kWcWaiting are
+// the team's words in TeamCtl::wc when the pass runs on a team).
 enum {
-    kWcNev = 0, kWcNCur, kWcNNext, kWcDone, kWcTotal, kWcErr, kWcScanBad, kWcWaiting, kWcTeamOk, kWcNClose,
+    kWcNev = 0, kWcNCur, kWcNNext, kWcDone, kWcTotal, kWcErr, kWcScanBad, kWcWaiting, kWcTeamOk,
+    kWcNClose, kWcNClose1,
     kWcTypeCnt = 16,   // 6 counters of the nodes being grouped
     kWcTypeOff = 22,   // 6 offsets into the grouped nodes
 };
@@ -90,22 +92,30 @@ enum {
 // A team's block in device memory, one per table solved by a team.  The
 // words up to `W` are cleared before every launch; the leader writes the rest
 // when it opens a pass.
+//
+// Members on different dies reach it through different L2 caches.  A word
+// that a member updates with an atomic while another writes plain stores
+// into the same cache line is not safe from the latter's write-back, so every
+// word that members change while others run (the barrier and pass words, the
+// list sizes, the key counters, the requests, the counts) is only ever
+// written and read with atomics, and the one part written with plain stores
+// (the pass parameters, by the leader before it opens a pass) has cache
+// lines of its own.
 struct TeamCtl {
-    int seq;          // wavefront passes the leader has opened
+    alignas(128) int seq;   // wavefront passes the leader has opened
     int release;      // 1: the leader opens no more passes (the others leave)
     int abort;        // 1: a member waited too long (every member gives the pass up)
     int bar_count;    // members arrived at the barrier
     int bar_gen;      // barriers completed
     int inpass;       // members other than the leader inside a pass
-    int pad[2];
-    Scratch W;        // the leader's slot: the pass's node state and lists
+    alignas(128) Scratch W;   // the leader's slot: the pass's node state and lists
     int box0[3][2], flags0[6], rad0, radius, stamp0;
     unsigned base;    // the pass's first round number
-    int wc[16];       // list sizes, waiting requests, error
+    alignas(128) int wc[16];  // list sizes, waiting requests, error
     int headw[8];     // head waves counted by the other members
     int done, seeds, carry;   // evaluations (all members), the others' seeds and requests
-    int pk[kTeamMaxEv];       // per event: keys of its face not yet final
-    int car[kWaveCarryCap * kWaveCarryInts];
+    alignas(128) int pk[kTeamMaxEv];   // per event: keys of its face not yet final
+    alignas(128) int car[kWaveCarryCap * kWaveCarryInts];
 };
 
 // The list a round fills: nodes queued for the next round, with its size word.
@@ -141,6 +151,9 @@ FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw) {
         if (x.load_relaxed(&tm->abort)) {
             ok = 0;
         } else if (x.atomic_add(&tm->bar_count, 1) == K - 1) {
+            // the last to arrive: it acquires the others' stores, then
+            // releases them with its own to everyone who sees the new count
+            x.fence_acquire();
             x.atomic_exch(&tm->bar_count, 0);
             x.store_release(&tm->bar_gen, gen + 1);
         } else {
@@ -150,8 +163,8 @@ FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw) {
                 if (x.clock() - t0 > kTeamWaitTicks) { x.store_release(&tm->abort, 1); ok = 0; break; }
                 x.pause();
             }
+            x.fence_acquire();
         }
-        x.fence_acquire();
         *okw = ok;
     }
     gen++;
@@ -409,12 +422,12 @@ FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
         int slot = -1;
         for (int c = 0; c < kWaveCarryCap; c++) {
             const int *q = WT.car + kWaveCarryInts * c;
-            if (q[5] == 2 && q[0] == (int)v) { slot = c; break; }
+            if (x.load_relaxed(&q[5]) == 2 && x.load_relaxed(&q[0]) == (int)v) { slot = c; break; }
         }
         if (slot >= 0) {
             tin0 = W.wt0[v];
             tin4 = W.wt4[v];
-            WT.car[kWaveCarryInts * slot + 5] = 0;   // free the request
+            x.atomic_exch(&WT.car[kWaveCarryInts * slot + 5], 0);   // free the request
         } else {
             // file a request (the node is queued again when it is resolved)
             FaceGeom g2;
@@ -431,12 +444,12 @@ FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                 return;
             }
             int *q = WT.car + kWaveCarryInts * got;
-            q[0] = (int)v;
-            q[1] = e;
-            q[2] = face_fill(g2, i, j, k);
-            q[3] = (int)key;
-            q[4] = -1;
-            q[6] = e;
+            x.atomic_exch(&q[0], (int)v);
+            x.atomic_exch(&q[1], e);
+            x.atomic_exch(&q[2], face_fill(g2, i, j, k));
+            x.atomic_exch(&q[3], (int)key);
+            x.atomic_exch(&q[4], -1);
+            x.atomic_exch(&q[6], e);
             x.atomic_add(&sh.st[kStatWaveCarry], 1);
             x.atomic_exch(&q[5], 1);   // waiting
             return;
@@ -568,11 +581,15 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     }
     for (int q = tid; q < 32; q += T) sh.wc[q] = 0;
     if (lead) {
-        for (int q = tid; q < kWaveCarryCap * kWaveCarryInts; q += T) WT.car[q] = 0;
+        for (int q = tid; q < kWaveCarryCap * kWaveCarryInts; q += T) x.atomic_exch(&WT.car[q], 0);
         if (tm) {
-            for (int q = tid; q < 16; q += T) tm->wc[q] = 0;
-            for (int q = tid; q < 8; q += T) tm->headw[q] = 0;
-            if (tid == 0) { tm->done = 0; tm->seeds = 0; tm->carry = 0; }
+            for (int q = tid; q < 16; q += T) x.atomic_exch(&tm->wc[q], 0);
+            for (int q = tid; q < 8; q += T) x.atomic_exch(&tm->headw[q], 0);
+            if (tid == 0) {
+                x.atomic_exch(&tm->done, 0);
+                x.atomic_exch(&tm->seeds, 0);
+                x.atomic_exch(&tm->carry, 0);
+            }
         }
     }
     x.sync();
@@ -640,7 +657,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     if (tid == 0) sh.wc[kWcDone] = 0;
     if (lead && tm) {
         // open the pass for the other members
-        for (int e = tid; e < nev; e += T) tm->pk[e] = WT.fsize(e);
+        for (int e = tid; e < nev; e += T) x.atomic_exch(&tm->pk[e], WT.fsize(e));
         if (tid == 0) {
             tm->W = W;
             for (int a = 0; a < 3; a++) { tm->box0[a][0] = box0[a][0]; tm->box0[a][1] = box0[a][1]; }
@@ -656,7 +673,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     x.sync();
     // counters: earlier-face predecessors of every node of the pass; nodes with
     // none and no in-face predecessor are ready for the first round (list wtmp)
-    const WaveQ Q0{W.wtmp, &wc[kWcNNext], W.wcap};
+    const WaveQ Q0{W.wtmp, &wc[kWcNCur], W.wcap};
     const long long first = (long long)member * T + tid, stride = (long long)K * T;
     for (long long v = first; v < N; v += stride) {
         int i, j, k;
@@ -700,38 +717,35 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             }
         }
     }
+    // Rounds.  Round R evaluates list wtmp (R - base even) or wready (odd),
+    // whose size is wc[kWcNCur + parity], and fills the other list and the
+    // other size word.  Two barriers per round: after the evaluations, and
+    // after the closing phase, during which the leader also resolves the
+    // carried-register requests (no node value, status or key changes then)
+    // and clears the size words the next round fills.
     unsigned R = base;
     bool ok = true, alive = tsync();   // alive: no member gave the pass up
-    int idle = 0;
+    int idle = 0, why = 0;
     while (alive) {
-        // round R evaluates the list filled before it (wtmp when R - base is
-        // even, wready when odd) and fills the other
         const long long tr0 = tid == 0 ? x.clock() : 0;
-        if (lead && tid == 0) {
-            wc[kWcNCur] = wc[kWcNNext];
-            wc[kWcNNext] = 0;
-            wc[kWcNClose] = 0;
-            int waiting = 0;
-            for (int c = 0; c < kWaveCarryCap; c++) waiting += WT.car[kWaveCarryInts * c + 5] != 0;
-            wc[kWcWaiting] = waiting;
-        }
-        if (!(alive = tsync())) break;
-        const int ncur = wc[kWcNCur], waiting = wc[kWcWaiting];
-        if (wc[kWcErr] || ncur > W.wcap) { ok = false; break; }
+        const unsigned par = (R - base) & 1u;
+        // read after the barrier
+        const int ncur = x.load_relaxed(&wc[kWcNCur + par]), waiting = x.load_relaxed(&wc[kWcWaiting]);
+        const int err = x.load_relaxed(&wc[kWcErr]);
+        if (err || ncur > W.wcap) { ok = false; why = err ? 2 : 1; break; }
         if (ncur == 0 && waiting == 0) break;
         if (ncur == 0) {
             // only carried-register requests are open: one must resolve now
-            if (++idle > 2) { ok = false; break; }
+            if (++idle > 2) { ok = false; why = 3; break; }
         } else {
             idle = 0;
         }
-        if (R - base > (unsigned)(2 * N + 64) || R >= (1u << 30) - 2) { ok = false; break; }
-        const bool odd = ((R - base) & 1u) != 0;
-        const int *cur = odd ? W.wready : W.wtmp;
-        const WaveQ Q{odd ? W.wtmp : W.wready, &wc[kWcNNext], W.wcap};
-        const WaveQ CLQ{W.wclose, &wc[kWcNClose], W.wcap};
+        if (R - base > (unsigned)(2 * N + 64) || R >= (1u << 30) - 2) { ok = false; why = 4; break; }
+        const int *cur = par ? W.wready : W.wtmp;
+        const WaveQ Q{par ? W.wtmp : W.wready, &wc[kWcNCur + (1 - par)], W.wcap};
+        const WaveQ CLQ{W.wclose, &wc[kWcNClose + par], W.wcap};
         const WaveQ *CL = tm ? &CLQ : nullptr;
-        const long long tr1 = tid == 0 ? x.clock() : 0;
+        long long tr1 = 0;
         // this member's share of the list, grouped by face type in LDS so that
         // waves of one type evaluate together (the six evaluation routines are
         // large; waves running different ones at once miss in the instruction
@@ -756,6 +770,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                 lbuf[x.atomic_add(&sh.wc[kWcTypeOff + (int)((unsigned)w >> 29)], 1)] = w;
             }
             x.sync();
+            if (tid == 0 && tr1 == 0) tr1 = x.clock();
             // each type's offset now ends its range
             static_for<0, 6>([&](auto FT) {
                 constexpr int Fc = decltype(FT)::value;
@@ -771,56 +786,61 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             sh.st[kStatEvals] += ncur;
         }
         if (!(alive = tsync())) break;
-        if (tm) {
-            // closing phase: every key written this round is visible now
-            const int ncl = wc[kWcNClose];
-            if (ncl > W.wcap) { ok = false; break; }   // uniform: read after the barrier
-            for (long long it = first; it < ncl; it += stride) wave_close(x, J, W, WT, Q, W.wclose[it]);
-            if (!(alive = tsync())) break;
-        }
         const long long tr2 = tid == 0 ? x.clock() : 0;
+        // closing phase: every key written this round is visible now
+        const int ncl = x.load_relaxed(&wc[kWcNClose + par]);
+        if (ncl > W.wcap) { ok = false; why = 5; break; }   // uniform: read after the barrier
+        for (long long it = first; it < ncl; it += stride) wave_close(x, J, W, WT, Q, W.wclose[it]);
         if (lead) {
-            // carried-register requests: find each one's predecessor (a failure
-            // is seen by every member when the next round starts)
+            if (tid == 0) {
+                // every member has read this round's size: the round after
+                // next fills that word; the next round fills the other
+                // closing word
+                x.atomic_exch(&wc[kWcNCur + par], 0);
+                x.atomic_exch(&wc[kWcNClose + (1 - par)], 0);
+            }
+            // carried-register requests: find each one's predecessor (a
+            // failure is seen by every member when the next round starts)
             bool fail = false;
             for (int c = 0; c < kWaveCarryCap && !fail; c++) {
                 int *q = WT.car + kWaveCarryInts * c;
-                if (q[5] != 1) continue;   // uniform: read after the barrier
+                if (x.load_relaxed(&q[5]) != 1) continue;   // uniform: no member changes requests now
                 for (int guard = 0; guard < 4 * nev + 8; guard++) {
-                    const int cand = q[4];
+                    const int cand = x.load_relaxed(&q[4]), qv = x.load_relaxed(&q[0]), qe = x.load_relaxed(&q[1]);
                     if (cand >= 0) {
                         const unsigned s = W.wst[cand];
                         if (!wv_final_in(s, base)) break;            // wait for it
                         if (s & 2u) {                                // it set them: its t[0]/t[4]
                             x.sync();
                             if (tid == 0) {
-                                W.wt0[q[0]] = W.wt0[cand];
-                                W.wt4[q[0]] = W.wt4[cand];
-                                q[5] = 2;
-                                wave_queue(x, Q, q[0], WT.type(q[1]));
+                                W.wt0[qv] = W.wt0[cand];
+                                W.wt4[qv] = W.wt4[cand];
+                                x.atomic_exch(&q[5], 2);
+                                wave_queue(x, Q, qv, WT.type(qe));
                             }
                             x.sync();
                             break;
                         }
                         x.sync();
-                        if (tid == 0) q[4] = -1;   // it passed its own carry on: search below it
+                        if (tid == 0) x.atomic_exch(&q[4], -1);   // it passed its own carry on: search below it
                         x.sync();
                         continue;
                     }
-                    const int e = q[6];
+                    const int e = x.load_relaxed(&q[6]);
                     if (e < 0) {   // before the first face: the pass's carry-in
                         x.sync();
                         if (tid == 0) {
-                            W.wt0[q[0]] = cin0;
-                            W.wt4[q[0]] = cin4;
-                            q[5] = 2;
-                            wave_queue(x, Q, q[0], WT.type(q[1]));
+                            W.wt0[qv] = cin0;
+                            W.wt4[qv] = cin4;
+                            x.atomic_exch(&q[5], 2);
+                            wave_queue(x, Q, qv, WT.type(qe));
                         }
                         x.sync();
                         break;
                     }
-                    const unsigned long long bound = ((unsigned long long)(unsigned)q[3] << 32) | (unsigned)q[2];
-                    if (WT.fsize(e) > 0 && *WT.pk(e) != 0) break;   // keys of that face not all final yet
+                    const unsigned long long bound = ((unsigned long long)(unsigned)x.load_relaxed(&q[3]) << 32) |
+                                                     (unsigned)x.load_relaxed(&q[2]);
+                    if (WT.fsize(e) > 0 && x.load_relaxed(WT.pk(e)) != 0) break;   // keys of that face not all final yet
                     unsigned long long best = ~0ull;
                     if (WT.fsize(e) > 0) {
                         best = wave_scan(x, sh, J, W, WT, e, bound, false, base);
@@ -833,29 +853,35 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                     x.sync();
                     if (tid == 0) {
                         if (best == ~0ull) {   // continue at the last node of the previous face
-                            q[6] = e - 1;
-                            q[2] = -1;
-                            q[3] = -1;
+                            x.atomic_exch(&q[6], e - 1);
+                            x.atomic_exch(&q[2], -1);
+                            x.atomic_exch(&q[3], -1);
                         } else {
                             FaceGeom fg;
                             wave_face_geom(WT, e, fg);
                             int ci, cj, ck;
                             face_coords(fg, (int)(best & 0xffffffffu), ci, cj, ck);
-                            q[4] = (int)((long long)ck * nxy + (long long)cj * nx + ci);
-                            q[2] = (int)(unsigned)(best & 0xffffffffu);
-                            q[3] = (int)(unsigned)(best >> 32);
+                            x.atomic_exch(&q[4], (int)((long long)ck * nxy + (long long)cj * nx + ci));
+                            x.atomic_exch(&q[2], (int)(unsigned)(best & 0xffffffffu));
+                            x.atomic_exch(&q[3], (int)(unsigned)(best >> 32));
                         }
                     }
                     x.sync();
                 }
             }
             x.sync();
-            if (fail && tid == 0) wc[kWcErr] = 1;
+            if (tid == 0) {
+                int nw = 0;
+                for (int c = 0; c < kWaveCarryCap; c++) nw += x.load_relaxed(&WT.car[kWaveCarryInts * c + 5]) != 0;
+                x.atomic_exch(&wc[kWcWaiting], nw);
+                if (fail) x.atomic_exch(&wc[kWcErr], 1);
+            }
         }
+        if (!(alive = tsync())) break;
         if (tid == 0 && lead) {
             const long long tr3 = x.clock();
-            sh.st[kStatTimeWaveGroup] += (int)(tr1 - tr0);
-            sh.st[kStatTimeWaveEval] += (int)(tr2 - tr1);
+            sh.st[kStatTimeWaveGroup] += (int)((tr1 ? tr1 : tr2) - tr0);
+            sh.st[kStatTimeWaveEval] += (int)(tr2 - (tr1 ? tr1 : tr2));
             sh.st[kStatTimeWaveCarry] += (int)(tr3 - tr2);
         }
         R++;
@@ -876,7 +902,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
         }
         alive = tsync();
     }
-    if (!alive) ok = false;
+    if (!alive) { ok = false; why = 7; }
     if (!lead) return ok;
     if (tm && !alive) {
         // a member gave up: wait until the others have left the slot's state
@@ -887,14 +913,17 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
         x.sync();
     }
     // every node of the pass must have been evaluated exactly once
-    if (ok && (tm ? tm->done : sh.wc[kWcDone]) != sh.wc[kWcTotal]) ok = false;
+    if (ok && (tm ? x.load_relaxed(&tm->done) : sh.wc[kWcDone]) != sh.wc[kWcTotal]) { ok = false; why = 6; }
     if (tid == 0) {
         if (tm && alive) {
-            for (int w = 0; w < 7; w++) sh.headw[w] += tm->headw[w];
-            sh.st[kStatWaveSeeds] += tm->seeds;
-            sh.st[kStatWaveCarry] += tm->carry;
+            for (int w = 0; w < 7; w++) sh.headw[w] += x.load_relaxed(&tm->headw[w]);
+            sh.st[kStatWaveSeeds] += x.load_relaxed(&tm->seeds);
+            sh.st[kStatWaveCarry] += x.load_relaxed(&tm->carry);
         }
-        if (!ok) sh.st[kStatError] |= 128;
+        if (!ok) {
+            sh.st[kStatError] |= 128;
+            sh.st[kStatWaveFail] = why;
+        }
     }
     x.sync();
     if (ok) {

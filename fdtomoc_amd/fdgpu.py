@@ -20,7 +20,7 @@ STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level
               "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
     ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1",
      "wave_passes", "rounds_wave", "wave_seeds", "wave_carry", "wave_scans", "t_wave",
-     "t_wave_group", "t_wave_eval", "t_wave_carry", "team"]
+     "t_wave_group", "t_wave_eval", "t_wave_carry", "team", "wave_fail"]
 
 BATCHED, PLAIN = 0, 1
 
@@ -569,10 +569,12 @@ class Context:
         _check(lib().fdgpu_get_wave(self._h, C.byref(b)), "fdgpu_get_wave")
         return b.value
 
-    def set_team(self, size: int, share_pct: int = 200):
-        """Teams (fdgpu_set_team): the wavefront passes of each batched table
-        of at least share_pct % of the mean nodes per workgroup shared by
-        `size` workgroups (0 automatic, 1 never); results do not depend on it."""
+    def set_team(self, size: int, share_pct: int = 0):
+        """Teams (fdgpu_set_team): the wavefront passes of the largest batched
+        tables shared by several workgroups -- size 0: planned per table
+        (share_pct: the planner's load %, 0 for 100); 2..16: that size for
+        tables of at least share_pct % (0 for 200) of the mean nodes per
+        workgroup; 1: never.  Results do not depend on it."""
         _check(lib().fdgpu_set_team(self._h, int(size), int(share_pct)), "fdgpu_set_team")
 
     def get_team(self) -> tuple[int, int]:

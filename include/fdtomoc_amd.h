@@ -321,12 +321,14 @@ int fdgpu_set_wave(fdgpu_ctx *ctx, int flags);
 int fdgpu_get_wave(fdgpu_ctx *ctx, int *flags);
 
 /* Teams (no reference counterpart; results are identical for every choice):
- * in a batched solve with wavefront passes, each of the largest tables --
- * the leading tables of the largest-first queue with at least `share_pct` %
- * of the launch's mean nodes per workgroup, at most workgroups / size of
- * them -- has its wavefront passes shared by `size` workgroups on as many
- * CUs (2..16), instead of one.  size 0: automatic (4), 1: never.  Default
- * 0 and 200. */
+ * in a batched solve with wavefront passes, the largest tables have their
+ * wavefront passes shared by several workgroups on as many CUs (2..16)
+ * instead of one.  size 0 (default): planned per table -- each table gets
+ * the smallest team expected to finish within a common batch time, the
+ * shortest whose work fills the device's workgroups at `share_pct` % (0:
+ * 100); size 2..16: that size for the leading tables of the largest-first
+ * queue with at least `share_pct` % (0: 200) of the mean nodes per
+ * workgroup of a full device; size 1: never. */
 int fdgpu_set_team(fdgpu_ctx *ctx, int size, int share_pct);
 int fdgpu_get_team(fdgpu_ctx *ctx, int *size, int *share_pct);
 

@@ -15,7 +15,7 @@ STAT_NAMES = ["passes", "restarts", "faces_level", "faces_change", "rounds_level
               "fallbacks", "evals", "defers", "error", "last_wall"] + [f"headw{i}" for i in range(7)] + \
     ["skips", "t_gather", "t_sort", "t_level", "t_change", "t_init", "t_level_pre", "t_level_eval", "walks", "t_change1",
      "wave_passes", "rounds_wave", "wave_seeds", "wave_carry", "wave_scans", "t_wave",
-     "t_wave_group", "t_wave_eval", "t_wave_carry", "team"]
+     "t_wave_group", "t_wave_eval", "t_wave_carry", "team", "wave_fail"]
 
 _lib = None
 

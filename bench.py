@@ -196,7 +196,9 @@ PROFILED_TRAFFIC = {
     "c3": [{"wave": 0, "change_budget": 2, "bytes": 398.23e9,
             "source": "profiles/sphfd_sweep_kernel_c3_256windows.md"},
            {"wave": 3, "change_budget": 2, "bytes": 1484.32e9,
-            "source": "profiles/sphfd_sweep_kernel_c3_256windows_wavefront.md"}],
+            "source": "profiles/sphfd_sweep_kernel_c3_256windows_wavefront.md"},
+           {"wave": 3, "change_budget": 2, "team": [0, 150], "bytes": 1679.90e9,
+            "source": "profiles/sphfd_sweep_kernel_c3_256windows_teams.md"}],
 }
 
 

@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -305,37 +306,72 @@ int ensure_teams(fdgpu_ctx *c, int nteam) {
     return FDGPU_OK;
 }
 
-// Team sizes of the leading tables of a largest-first queue (sizes n, nodes),
-// all tables of at most `cap` workgroups' worth of time.  A table's time is
-// modelled as its node count times the measured time ratio of a team of K
-// (one table of 4 M nodes alone on MI355X: 2 workgroups 0.71 of one
-// workgroup's time, 4: 0.52, 8: 0.41, 16: 0.37); the plan gives each table
-// the smallest team that finishes within a batch time T, and takes the
-// smallest T whose workgroup time fits `cap` workgroups at `load` (1: every
-// workgroup busy for T).  Returns the team sizes of the leading tables
-// (sizes 1 end the list).
+// Team sizes of the leading tables of a largest-first queue (sizes n, nodes)
+// on `cap` workgroups.  A table's time is modelled as its node count times
+// the measured time ratio of a team of K (one table of 4 M nodes alone on
+// MI355X: 2 workgroups 0.71 of one workgroup's time, 4: 0.52, 8: 0.41, 16:
+// 0.37).  For a batch time T each table gets the smallest team that finishes
+// within T.  load > 0: the smallest T whose workgroup time fits `cap`
+// workgroups at that load (1: every workgroup busy for T).  load 0: the T
+// (among 64 between the largest table's time on 16 workgroups and on one)
+// whose plan has the shortest makespan when its tickets are handed out in
+// queue order, as the kernel does (a team starts once K workgroups are
+// free).  Returns the team sizes of the leading tables (sizes 1 end the list).
 std::vector<int> plan_teams(const std::vector<long long> &n, int cap, double load) {
     static const double ratio[17] = {0, 1.0, 0.71, 0.60, 0.52, 0.48, 0.45, 0.43, 0.41,
                                      0.40, 0.395, 0.39, 0.385, 0.38, 0.375, 0.372, 0.37};
     std::vector<int> K(n.size(), 1);
-    if (n.empty()) return {};
+    if (n.empty() || cap < 2) return {};
     auto assign = [&](double T) {
         double cost = 0.0;
         for (size_t i = 0; i < n.size(); i++) {
             int k = 1;
-            while (k < fdsweep::kTeamMax && n[i] >= kTeamMinNodes && (double)n[i] * ratio[k] > T) k++;
+            while (k < fdsweep::kTeamMax && k < cap && n[i] >= kTeamMinNodes && (double)n[i] * ratio[k] > T) k++;
             K[i] = k;
             cost += k * (double)n[i] * ratio[k];
         }
         return cost;
     };
-    double lo = (double)n[0] * ratio[fdsweep::kTeamMax], hi = (double)n[0];
-    for (int it = 0; it < 60; it++) {
-        const double mid = 0.5 * (lo + hi);
-        if (assign(mid) <= cap * mid * load) hi = mid;
-        else lo = mid;
+    const double lo0 = (double)n[0] * ratio[fdsweep::kTeamMax], hi0 = (double)n[0];
+    if (load > 0.0) {
+        double lo = lo0, hi = hi0;
+        for (int it = 0; it < 60; it++) {
+            const double mid = 0.5 * (lo + hi);
+            if (assign(mid) <= cap * mid * load) hi = mid;
+            else lo = mid;
+        }
+        assign(hi);
+    } else {
+        // list schedule of the tickets: workgroups' free times in a min-heap
+        auto makespan = [&]() {
+            std::priority_queue<double, std::vector<double>, std::greater<double>> fr;
+            for (int w = 0; w < cap; w++) fr.push(0.0);
+            double end = 0.0;
+            std::vector<double> got;
+            for (size_t i = 0; i < n.size(); i++) {
+                got.clear();
+                for (int k = 0; k < K[i]; k++) {
+                    got.push_back(fr.top());
+                    fr.pop();
+                }
+                const double t = *std::max_element(got.begin(), got.end()) + (double)n[i] * ratio[K[i]];
+                for (int k = 0; k < K[i]; k++) fr.push(t);
+                end = std::max(end, t);
+            }
+            return end;
+        };
+        double bestT = hi0, best = -1.0;
+        for (int j = 0; j <= 64; j++) {
+            const double T = lo0 + (hi0 - lo0) * j / 64.0;
+            assign(T);
+            const double m = makespan();
+            if (best < 0.0 || m < best * (1.0 - 1e-9)) {
+                best = m;
+                bestT = T;
+            }
+        }
+        assign(bestT);
     }
-    assign(hi);
     size_t m = 0;
     while (m < K.size() && K[m] > 1) m++;
     K.resize(m);
@@ -461,7 +497,7 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
                 lead++;
         }
         if (c->team == 0) {
-            teamK = plan_teams(sizes, cap, (c->team_share > 0 ? c->team_share : 100) / 100.0);
+            teamK = plan_teams(sizes, cap, c->team_share / 100.0);
         } else {
             const int K = std::min(fdsweep::kTeamMax, c->team);
             const double share = (double)total_nodes / cap * (c->team_share > 0 ? c->team_share : 200) / 100.0;

@@ -46,6 +46,9 @@ struct DevExec {
     __device__ __forceinline__ int load_relaxed(const int *p) const {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __device__ __forceinline__ void store_relaxed(int *p, int v) const {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // this wave's stores have reached the L2; also a compiler barrier, so no
     // store is scheduled after the wait
     __device__ __forceinline__ void wait_stores() const { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }

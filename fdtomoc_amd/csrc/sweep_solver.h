@@ -13,7 +13,8 @@
 //   unsigned long long X::atomic_max_u64(...), unsigned X::atomic_cas(...)
 //   void X::fence_release(), X::fence_acquire()   agent-scope fences (teams)
 //   void X::wait_stores()                  this thread's stores have completed
-//   int  X::load_relaxed(const int*), void X::store_release(int*, int)
+//   int  X::load_relaxed(const int*), void X::store_relaxed(int*, int),
+//   void X::store_release(int*, int)
 //   void X::pause()                        back off inside a wait loop
 //
 // On the GPU X wraps threadIdx/__syncthreads/atomics (sweep_kernel.hip).

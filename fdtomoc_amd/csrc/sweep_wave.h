@@ -118,11 +118,14 @@ struct TeamCtl {
     alignas(128) int car[kWaveCarryCap * kWaveCarryInts];
 };
 
-// The list a round fills: nodes queued for the next round, with its size word.
+// The list a round fills: nodes queued for the next round, with its size
+// word.  In a team the entries are written with atomic stores (they bypass
+// the caches, so the barrier after the closing phase needs no write-back).
 struct WaveQ {
     int *next;
     int *cnt;
     long long cap;
+    bool team;
 };
 
 // Barrier of a team's K members: every store a member made before it is
@@ -143,9 +146,19 @@ FD_INLINE void team_publish(X &x) {
     if (x.tid() == 0) x.fence_release();
 }
 
+//
+// light: the words written since the last barrier that members read after
+// this one were all written and are all read with atomics, so it only
+// orders (each wave waits for its stores) and neither writes the L2 back nor
+// invalidates it.
 template <class X>
-FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw) {
-    team_publish(x);
+FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw, bool light = false) {
+    if (light) {
+        x.wait_stores();
+        x.sync();
+    } else {
+        team_publish(x);
+    }
     if (x.tid() == 0) {
         int ok = 1;
         if (x.load_relaxed(&tm->abort)) {
@@ -153,9 +166,10 @@ FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw) {
         } else if (x.atomic_add(&tm->bar_count, 1) == K - 1) {
             // the last to arrive: it acquires the others' stores, then
             // releases them with its own to everyone who sees the new count
-            x.fence_acquire();
+            if (!light) x.fence_acquire();
             x.atomic_exch(&tm->bar_count, 0);
-            x.store_release(&tm->bar_gen, gen + 1);
+            if (light) x.store_relaxed(&tm->bar_gen, gen + 1);
+            else x.store_release(&tm->bar_gen, gen + 1);
         } else {
             const long long t0 = x.clock();
             while (x.load_relaxed(&tm->bar_gen) == gen) {
@@ -163,7 +177,7 @@ FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw) {
                 if (x.clock() - t0 > kTeamWaitTicks) { x.store_release(&tm->abort, 1); ok = 0; break; }
                 x.pause();
             }
-            x.fence_acquire();
+            if (!light) x.fence_acquire();
         }
         *okw = ok;
     }
@@ -258,7 +272,17 @@ FD_INLINE int face_fill(const FaceGeom &fg, int i, int j, int k) {
 template <class X>
 FD_INLINE void wave_queue(X &x, const WaveQ &Q, long long v, int F) {
     const int pos = x.atomic_add(Q.cnt, 1);
-    if (pos < Q.cap) Q.next[pos] = (int)((unsigned)v | ((unsigned)F << 29));
+    const int w = (int)((unsigned)v | ((unsigned)F << 29));
+    if (pos < Q.cap) {
+        if (Q.team) x.store_relaxed(&Q.next[pos], w);
+        else Q.next[pos] = w;
+    }
+}
+
+// An entry of a list (read with an atomic load in a team, see WaveQ).
+template <class X>
+FD_INLINE int wave_entry(X &x, const int *list, long long i, bool team) {
+    return team ? x.load_relaxed(list + i) : list[i];
 }
 
 // Node index -> coordinates (indices below 2^29).
@@ -425,8 +449,9 @@ FD_INLINE void wave_eval(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             if (x.load_relaxed(&q[5]) == 2 && x.load_relaxed(&q[0]) == (int)v) { slot = c; break; }
         }
         if (slot >= 0) {
-            tin0 = W.wt0[v];
-            tin4 = W.wt4[v];
+            // the resolved carry-in: the leader stored it atomically
+            tin0 = __builtin_bit_cast(float, x.load_relaxed((const int *)&W.wt0[v]));
+            tin4 = __builtin_bit_cast(float, x.load_relaxed((const int *)&W.wt4[v]));
             x.atomic_exch(&WT.car[kWaveCarryInts * slot + 5], 0);   // free the request
         } else {
             // file a request (the node is queued again when it is resolved)
@@ -569,9 +594,9 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     int *const lbuf = sh.arena + wave_arena_words(nx, ny, nz);
     const long long L = (long long)SH::kArena - wave_arena_words(nx, ny, nz);
     int *const wc = tm ? tm->wc : sh.wc;   // list sizes, requests waiting, error: shared by the team
-    auto tsync = [&]() -> bool {
+    auto tsync = [&](bool light = false) -> bool {
         if (!tm) { x.sync(); return true; }
-        return team_sync(x, tm, K, *gen, &sh.wc[kWcTeamOk]);
+        return team_sync(x, tm, K, *gen, &sh.wc[kWcTeamOk], light);
     };
     // planes inside the starting box are interior, all others unswept so far
     for (int q = tid; q < nx + ny + nz; q += T) {
@@ -673,7 +698,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
     x.sync();
     // counters: earlier-face predecessors of every node of the pass; nodes with
     // none and no in-face predecessor are ready for the first round (list wtmp)
-    const WaveQ Q0{W.wtmp, &wc[kWcNCur], W.wcap};
+    const WaveQ Q0{W.wtmp, &wc[kWcNCur], W.wcap, tm != nullptr};
     const long long first = (long long)member * T + tid, stride = (long long)K * T;
     for (long long v = first; v < N; v += stride) {
         int i, j, k;
@@ -742,8 +767,8 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
         }
         if (R - base > (unsigned)(2 * N + 64) || R >= (1u << 30) - 2) { ok = false; why = 4; break; }
         const int *cur = par ? W.wready : W.wtmp;
-        const WaveQ Q{par ? W.wtmp : W.wready, &wc[kWcNCur + (1 - par)], W.wcap};
-        const WaveQ CLQ{W.wclose, &wc[kWcNClose + par], W.wcap};
+        const WaveQ Q{par ? W.wtmp : W.wready, &wc[kWcNCur + (1 - par)], W.wcap, tm != nullptr};
+        const WaveQ CLQ{W.wclose, &wc[kWcNClose + par], W.wcap, true};
         const WaveQ *CL = tm ? &CLQ : nullptr;
         long long tr1 = 0;
         // this member's share of the list, grouped by face type in LDS so that
@@ -755,7 +780,8 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             const int m = (int)(c1 - s0 < L ? c1 - s0 : L);
             for (int f = tid; f < 6; f += T) sh.wc[kWcTypeCnt + f] = 0;
             x.sync();
-            for (int it = tid; it < m; it += T) x.atomic_add(&sh.wc[kWcTypeCnt + (int)((unsigned)cur[s0 + it] >> 29)], 1);
+            for (int it = tid; it < m; it += T)
+                x.atomic_add(&sh.wc[kWcTypeCnt + (int)((unsigned)wave_entry(x, cur, s0 + it, tm) >> 29)], 1);
             x.sync();
             if (tid == 0) {
                 int o = 0;
@@ -766,7 +792,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
             }
             x.sync();
             for (int it = tid; it < m; it += T) {
-                const int w = cur[s0 + it];
+                const int w = wave_entry(x, cur, s0 + it, tm);
                 lbuf[x.atomic_add(&sh.wc[kWcTypeOff + (int)((unsigned)w >> 29)], 1)] = w;
             }
             x.sync();
@@ -790,7 +816,7 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
         // closing phase: every key written this round is visible now
         const int ncl = x.load_relaxed(&wc[kWcNClose + par]);
         if (ncl > W.wcap) { ok = false; why = 5; break; }   // uniform: read after the barrier
-        for (long long it = first; it < ncl; it += stride) wave_close(x, J, W, WT, Q, W.wclose[it]);
+        for (long long it = first; it < ncl; it += stride) wave_close(x, J, W, WT, Q, wave_entry(x, W.wclose, it, true));
         if (lead) {
             if (tid == 0) {
                 // every member has read this round's size: the round after
@@ -813,8 +839,8 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                         if (s & 2u) {                                // it set them: its t[0]/t[4]
                             x.sync();
                             if (tid == 0) {
-                                W.wt0[qv] = W.wt0[cand];
-                                W.wt4[qv] = W.wt4[cand];
+                                x.store_relaxed((int *)&W.wt0[qv], __builtin_bit_cast(int, W.wt0[cand]));
+                                x.store_relaxed((int *)&W.wt4[qv], __builtin_bit_cast(int, W.wt4[cand]));
                                 x.atomic_exch(&q[5], 2);
                                 wave_queue(x, Q, qv, WT.type(qe));
                             }
@@ -830,8 +856,8 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                     if (e < 0) {   // before the first face: the pass's carry-in
                         x.sync();
                         if (tid == 0) {
-                            W.wt0[qv] = cin0;
-                            W.wt4[qv] = cin4;
+                            x.store_relaxed((int *)&W.wt0[qv], __builtin_bit_cast(int, cin0));
+                            x.store_relaxed((int *)&W.wt4[qv], __builtin_bit_cast(int, cin4));
                             x.atomic_exch(&q[5], 2);
                             wave_queue(x, Q, qv, WT.type(qe));
                         }
@@ -877,7 +903,9 @@ FD_INLINE bool wave_pass(X &x, SH &sh, const TableJob &J, const Scratch &W, Geo 
                 if (fail) x.atomic_exch(&wc[kWcErr], 1);
             }
         }
-        if (!(alive = tsync())) break;
+        // what this phase wrote (list entries and sizes, counters, requests,
+        // resolved carry-ins) went through atomics: an ordering barrier
+        if (!(alive = tsync(true))) break;
         if (tid == 0 && lead) {
             const long long tr3 = x.clock();
             sh.st[kStatTimeWaveGroup] += (int)((tr1 ? tr1 : tr2) - tr0);

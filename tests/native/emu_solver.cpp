@@ -45,6 +45,7 @@ struct HostExec {
     void fence_release() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
     void fence_acquire() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
     int load_relaxed(const int *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+    void store_relaxed(int *p, int v) { __atomic_store_n(p, v, __ATOMIC_RELAXED); }
     void wait_stores() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
     void store_release(int *p, int v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
     void pause() { std::this_thread::yield(); }

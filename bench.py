@@ -991,12 +991,18 @@ def run_rays(args):
     keys = np.array(keys)
     # the rays as fdgpu_ray records, built once (a C caller hands its own)
     rays = fdgpu.ray_array(np.asarray(ids)[keys], coords)
+    # derivative arrays sized by the first call and reused by the others, as
+    # an inversion loop that traces every iteration keeps its buffers
+    res, idx, val = ctx.trace_rays(grid, rays)
+    bufs = (np.empty(len(idx), np.int32), np.empty(len(idx), np.float32))
+    bufs[0][:] = 0
+    bufs[1][:] = 0
     for _ in range(args.warmup):
-        res, idx, val = ctx.trace_rays(grid, rays)
+        res, idx, val = ctx.trace_rays(grid, rays, out=bufs)
     walls, kms = [], []
     for s in range(args.steps):
         t0 = time.perf_counter()
-        res, idx, val = ctx.trace_rays(grid, rays)
+        res, idx, val = ctx.trace_rays(grid, rays, out=bufs)
         walls.append(time.perf_counter() - t0)
         kms.append(ctx.last_kernel_ms())
         print(f"# step {s}: {walls[-1] * 1e3:.1f} ms wall, {kms[-1]:.1f} ms kernels", file=sys.stderr, flush=True)

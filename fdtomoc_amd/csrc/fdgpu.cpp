@@ -62,14 +62,17 @@ void release_scratch(fdgpu_ctx *c) {
 
 // The ray tracer's cached device arrays (regrown by its next call).
 void release_ray_columns(fdgpu_ctx *c) {
-    if (!c->ray_cols && !c->ray_blk) return;
+    if (!c->ray_cols && !c->ray_blk && !c->ray_out) return;
     (void)hipStreamSynchronize(c->stream);
     dev_free(c, c->ray_cols, c->ray_cols_bytes);
     dev_free(c, c->ray_blk, c->ray_blk_bytes);
+    dev_free(c, c->ray_out, c->ray_out_bytes);
     c->ray_cols = nullptr;
     c->ray_cols_bytes = 0;
     c->ray_blk = nullptr;
     c->ray_blk_bytes = 0;
+    c->ray_out = nullptr;
+    c->ray_out_bytes = 0;
 }
 
 // Memory that outlives a launch (tables, models): when the budget is short,
@@ -77,7 +80,7 @@ void release_ray_columns(fdgpu_ctx *c) {
 // next launch or trace regrows them within what is left.
 int dev_alloc_resident(fdgpu_ctx *c, void **p, size_t bytes) {
     int rc = dev_alloc(c, p, bytes);
-    if (rc == FDGPU_ENOMEM && (c->d_scratch || c->ray_cols || c->ray_blk)) {
+    if (rc == FDGPU_ENOMEM && (c->d_scratch || c->ray_cols || c->ray_blk || c->ray_out)) {
         release_scratch(c);
         release_ray_columns(c);
         rc = dev_alloc(c, p, bytes);
@@ -688,6 +691,7 @@ int fdgpu_close(fdgpu_ctx *c) {
     dev_free(c, c->d_slots, sizeof(fdsweep::Scratch) * (size_t)c->slots_cap);
     dev_free(c, c->ray_cols, c->ray_cols_bytes);
     dev_free(c, c->ray_blk, c->ray_blk_bytes);
+    dev_free(c, c->ray_out, c->ray_out_bytes);
     dev_free(c, c->d_teams, team_bytes(c->teams_cap));
     if (c->d_queue) (void)hipFree(c->d_queue);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

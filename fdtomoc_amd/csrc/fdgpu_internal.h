@@ -199,6 +199,8 @@ struct fdgpu_ctx {
     size_t ray_cols_bytes = 0;
     char *ray_blk = nullptr;        // ... and its other device arrays, kept between calls
     size_t ray_blk_bytes = 0;
+    char *ray_out = nullptr;        // ... and its derivatives in output order
+    size_t ray_out_bytes = 0;
     bool slot_wave = false;        // the slots hold wavefront state (sized by slot_node_cap)
     int slot_dim_cap = 0;          // per-slot last full-plane faces: largest dimension
     int *d_queue = nullptr;

@@ -200,6 +200,7 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     std::vector<RayResult> got;
     std::vector<int> pending = order, retry;
     double ms_total = 0.0;
+    bool one_launch = false;
     while (ok && !pending.empty()) {
         retry.clear();
         for (size_t p0 = 0; ok && p0 < pending.size(); p0 += (size_t)lanes) {
@@ -224,6 +225,17 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
             if (!ok) break;
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) ms_total += ms;
+            // every ray in this one launch (only the first pass holds all n),
+            // every derivative fitted: they are placed in output order on
+            // the device and copied once (below)
+            bool fits = m == n;
+            for (int k = 0; fits && k < m; k++)
+                if (got[(size_t)k].o.ndu > 0 && got[(size_t)k].offset < 0) fits = false;
+            if (fits) {
+                one_launch = true;
+                for (int k = 0; k < m; k++) outs[(size_t)chunk_ray[(size_t)k]] = got[(size_t)k].o;
+                continue;
+            }
             const long long filled = std::min<long long>((long long)used, capacity);
             const size_t base = h_idx.size();
             h_idx.resize(base + (size_t)filled);
@@ -269,6 +281,36 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
         total += o.ndu;
     }
     if (total > du_cap || (total > 0 && (!du_index || !du_value))) return FDGPU_ESPACE;
+    if (one_launch) {
+        // the launch's packed derivatives to their output offsets on the
+        // device, then one copy into the caller's arrays
+        if (total == 0) return FDGPU_OK;
+        const size_t b_dst = align256(sizeof(long long) * (size_t)n);
+        const size_t need = b_dst + align256(sizeof(int) * (size_t)total) + align256(sizeof(float) * (size_t)total);
+        if (c->ray_out_bytes < need) {
+            (void)hipStreamSynchronize(c->stream);
+            dev_free(c, c->ray_out, c->ray_out_bytes);
+            c->ray_out = nullptr;
+            c->ray_out_bytes = 0;
+            int rc = dev_alloc(c, (void **)&c->ray_out, need);
+            if (rc) return rc;
+            c->ray_out_bytes = need;
+        }
+        long long *d_dst = (long long *)c->ray_out;
+        int *d_oidx = (int *)(c->ray_out + b_dst);
+        float *d_oval = (float *)(c->ray_out + b_dst + align256(sizeof(int) * (size_t)total));
+        std::vector<long long> dst((size_t)n);
+        for (int k = 0; k < n; k++) dst[(size_t)k] = results[chunk_ray[(size_t)k]].offset;
+        ok = hipMemcpyAsync(d_dst, dst.data(), sizeof(long long) * (size_t)n, hipMemcpyHostToDevice, c->stream) ==
+                 hipSuccess &&
+             fd_launch_ray_place(d_res, d_dst, n, d_idx, d_val, d_oidx, d_oval, c->stream) == hipSuccess &&
+             hipMemcpyAsync(du_index, d_oidx, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, c->stream) ==
+                 hipSuccess &&
+             hipMemcpyAsync(du_value, d_oval, sizeof(float) * (size_t)total, hipMemcpyDeviceToHost, c->stream) ==
+                 hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+        return ok ? FDGPU_OK : FDGPU_EHIP;
+    }
     for (int r = 0; r < n; r++) {
         const int k = outs[(size_t)r].ndu;
         if (k <= 0) continue;

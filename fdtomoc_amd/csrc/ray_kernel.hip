@@ -140,3 +140,34 @@ extern "C" hipError_t fd_launch_rays(const fdray::RayGrid *grid, const fdray::Ra
                        idx, val, capacity, stage);
     return hipGetLastError();
 }
+
+namespace {
+
+constexpr int kPlaceThreads = 256;   // four rays per workgroup, one wave each
+
+__global__ __launch_bounds__(kPlaceThreads) void ray_place_kernel(const RayResult *__restrict__ res,
+                                                                  const long long *__restrict__ dst, int n,
+                                                                  const int *__restrict__ idx,
+                                                                  const float *__restrict__ val, int *out_idx,
+                                                                  float *out_val) {
+    const int k = blockIdx.x * (kPlaceThreads / 64) + (int)(threadIdx.x >> 6);
+    if (k >= n) return;
+    const int m = res[k].o.ndu;
+    const long long from = res[k].offset, to = dst[k];
+    if (m <= 0 || from < 0) return;
+    for (int q = (int)(threadIdx.x & 63); q < m; q += 64) {
+        out_idx[to + q] = idx[from + q];
+        out_val[to + q] = val[from + q];
+    }
+}
+
+}  // namespace
+
+extern "C" hipError_t fd_launch_ray_place(const RayResult *results, const long long *dst, int n, const int *idx,
+                                          const float *val, int *out_idx, float *out_val, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int per = kPlaceThreads / 64;
+    hipLaunchKernelGGL(ray_place_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(kPlaceThreads), 0, stream,
+                       results, dst, n, idx, val, out_idx, out_val);
+    return hipGetLastError();
+}

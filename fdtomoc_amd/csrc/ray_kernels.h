@@ -26,3 +26,9 @@ extern "C" hipError_t fd_launch_rays(const fdray::RayGrid *grid, const fdray::Ra
                                      RayResult *results,
                                      unsigned long long *cursor, int *idx, float *val, long long capacity,
                                      hipStream_t stream);
+
+// Copies each ray's packed derivatives to its place in output order: ray k of
+// the launch (results[k]) has results[k].o.ndu entries at results[k].offset
+// of (idx, val), which go to dst[k] of (out_idx, out_val).  One wave per ray.
+extern "C" hipError_t fd_launch_ray_place(const RayResult *results, const long long *dst, int n, const int *idx,
+                                          const float *val, int *out_idx, float *out_val, hipStream_t stream);

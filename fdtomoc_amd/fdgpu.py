@@ -413,7 +413,7 @@ class Context:
         the results are in the batch afterwards (LocBatch.results)."""
         _check(lib().fdgpu_locate(self._h, C.byref(params), batch.arr, batch.n), "fdgpu_locate")
 
-    def trace_rays(self, grid, rays, du_cap: int | None = None):
+    def trace_rays(self, grid, rays, du_cap: int | None = None, out=None):
         """sphrayderv's rays through resident tables (fdgpu_rays_trace).
         grid: a fdtomoc_amd.rayderv.RayGrid; rays: a RAY_DTYPE array
         (ray_array) or (table, ex, ey, ez, xs, ys, zs) tuples.  Returns
@@ -421,7 +421,10 @@ class Context:
         segments, derivative count and offset, event time and derivatives,
         angles) and the packed derivatives, each ray's in index order, rays
         in input order.  du_cap: size the derivative arrays to it (an
-        FDGPU_ESPACE error when they are too small) instead of growing them."""
+        FDGPU_ESPACE error when they are too small) instead of growing them.
+        out: (int32 array, float32 array) of equal length to receive the
+        derivatives (reused by repeated calls; grown if too small), in place
+        of new arrays; the returned ones are views of them."""
         if not isinstance(rays, np.ndarray):
             rays = ray_array([r[0] for r in rays], [r[1:7] for r in rays])
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
@@ -430,10 +433,20 @@ class Context:
                      len(gz), gx.ctypes.data, gy.ctypes.data, gz.ctypes.data, grid.ido1d, grid.max_segments)
         n = len(rays)
         res = np.zeros(max(n, 1), dtype=RAY_RESULT_DTYPE)
-        cap = du_cap if du_cap is not None else 128 * max(n, 1)
+        if out is not None:
+            idx, val = out
+            if not (isinstance(idx, np.ndarray) and idx.dtype == np.int32 and idx.flags.c_contiguous and
+                    isinstance(val, np.ndarray) and val.dtype == np.float32 and val.flags.c_contiguous and
+                    len(idx) == len(val)):
+                raise ValueError("out: contiguous int32 and float32 arrays of one length")
+            cap = len(idx) if du_cap is None else min(du_cap, len(idx))
+        else:
+            cap = du_cap if du_cap is not None else 128 * max(n, 1)
+            idx = val = None
         while True:
-            idx = np.empty(max(cap, 1), dtype=np.int32)
-            val = np.empty(max(cap, 1), dtype=np.float32)
+            if idx is None or len(idx) < max(cap, 1):
+                idx = np.empty(max(cap, 1), dtype=np.int32)
+                val = np.empty(max(cap, 1), dtype=np.float32)
             rc = lib().fdgpu_rays_trace(self._h, C.byref(g), rays.ctypes.data_as(C.POINTER(CRay)), n,
                                         res.ctypes.data_as(C.POINTER(CRayResult)), idx.ctypes.data,
                                         val.ctypes.data, cap)

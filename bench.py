@@ -197,7 +197,7 @@ PROFILED_TRAFFIC = {
             "source": "profiles/sphfd_sweep_kernel_c3_256windows.md"},
            {"wave": 3, "change_budget": 2, "bytes": 1484.32e9,
             "source": "profiles/sphfd_sweep_kernel_c3_256windows_wavefront.md"},
-           {"wave": 3, "change_budget": 2, "team": [0, 150], "bytes": 1679.90e9,
+           {"wave": 3, "change_budget": 2, "team": [0, 150], "bytes": 1717.57e9,
             "source": "profiles/sphfd_sweep_kernel_c3_256windows_teams.md"}],
 }
 

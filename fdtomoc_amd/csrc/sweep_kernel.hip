@@ -209,6 +209,7 @@ __global__ void reset_kernel(int *q, fdsweep::TeamCtl *teams, int nteam) {
     for (int t = threadIdx.x; t < nteam; t += blockDim.x) {
         fdsweep::TeamCtl &c = teams[t];
         c.seq = 0; c.release = 0; c.abort = 0; c.bar_count = 0; c.bar_gen = 0; c.inpass = 0;
+        c.wait_ticks = fdsweep::kTeamWaitTicks;
     }
 }
 

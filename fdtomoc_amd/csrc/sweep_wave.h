@@ -65,7 +65,8 @@ constexpr unsigned kWaveRoundLimit = 1u << 29;   // a slot's rounds are renumber
 constexpr int kWaveLocalMin = 1024;      // arena words left for grouping a round's nodes by face type
 constexpr int kTeamMaxEv = 3712;         // events of a pass in a team's block (at most nx + ny + nz)
 constexpr int kTeamMax = 16;             // workgroups in a team
-// a member waiting longer than this (X::clock ticks of 10 ns) gives the pass up
+// a member waiting longer than this (X::clock ticks of 10 ns) gives the pass
+// up, unless the team's block sets another limit
 constexpr long long kTeamWaitTicks = 2000000000LL;
 
 // Arena words the wavefront needs for a grid (planes' events, events, requests).
@@ -108,6 +109,7 @@ struct TeamCtl {
     int bar_count;    // members arrived at the barrier
     int bar_gen;      // barriers completed
     int inpass;       // members other than the leader inside a pass
+    long long wait_ticks;   // how long a member waits before it gives up (X::clock ticks)
     alignas(128) Scratch W;   // the leader's slot: the pass's node state and lists
     int box0[3][2], flags0[6], rad0, radius, stamp0;
     unsigned base;    // the pass's first round number
@@ -171,10 +173,10 @@ FD_INLINE bool team_sync(X &x, TeamCtl *tm, int K, int &gen, int *okw, bool ligh
             if (light) x.store_relaxed(&tm->bar_gen, gen + 1);
             else x.store_release(&tm->bar_gen, gen + 1);
         } else {
-            const long long t0 = x.clock();
+            const long long t0 = x.clock(), lim = tm->wait_ticks > 0 ? tm->wait_ticks : kTeamWaitTicks;
             while (x.load_relaxed(&tm->bar_gen) == gen) {
                 if (x.load_relaxed(&tm->abort)) { ok = 0; break; }
-                if (x.clock() - t0 > kTeamWaitTicks) { x.store_release(&tm->abort, 1); ok = 0; break; }
+                if (x.clock() - t0 > lim) { x.store_release(&tm->abort, 1); ok = 0; break; }
                 x.pause();
             }
             if (!light) x.fence_acquire();
@@ -1013,7 +1015,7 @@ FD_HD void team_helper(X &x, SH &sh, const TableJob &J, TeamCtl *tm, int K, int 
         if (tid == 0) {
             // wait for the next pass (or the end of the table)
             int go = 0;
-            const long long w0 = x.clock();
+            const long long w0 = x.clock(), lim = tm->wait_ticks > 0 ? tm->wait_ticks : kTeamWaitTicks;
             for (;;) {
                 if (x.load_relaxed(&tm->abort) || x.load_relaxed(&tm->release)) break;
                 if (x.load_relaxed(&tm->seq) > seen) {
@@ -1024,7 +1026,7 @@ FD_HD void team_helper(X &x, SH &sh, const TableJob &J, TeamCtl *tm, int K, int 
                     else go = 1;
                     break;
                 }
-                if (x.clock() - w0 > kTeamWaitTicks) { x.atomic_exch(&tm->abort, 1); break; }
+                if (x.clock() - w0 > lim) { x.atomic_exch(&tm->abort, 1); break; }
                 x.pause();
             }
             x.fence_acquire();

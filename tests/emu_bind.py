@@ -32,19 +32,20 @@ def lib():
 
 
 def solve(par: "ob.Par", slow: np.ndarray, nthreads: int = 4, extra=None, change_budget: int = 8,
-          global_state: bool = False, wave: int = 0, team: int = 1):
+          global_state: bool = False, wave: int = 0, team: int = 1, missing_member: bool = False):
     """global_state: keep every face's schedule state (LEVEL pending counters
     and successor masks, CHANGE queue marks) in global scratch, the device's
     path for faces too large for LDS.  wave: bit0 the first pass as one
     dataflow wavefront, bit1 the reverse passes too.  team: workgroups (of
-    nthreads threads each) sharing the wavefront passes."""
+    nthreads threads each) sharing the wavefront passes; missing_member: the
+    team's last member never starts."""
     slow = np.ascontiguousarray(slow, dtype=np.float32)
     time = np.empty_like(slow)
     stats = np.zeros(len(STAT_NAMES), dtype=np.int32)
     ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float32)
     rc = lib().emu_solve(C.byref(par.to_c()), slow.ctypes.data, time.ctypes.data,
                          None if ex is None else ex.ctypes.data, nthreads, stats.ctypes.data, change_budget,
-                         (1 if global_state else 0) | (wave << 1) | (team << 3))
+                         (1 if global_state else 0) | (wave << 1) | (team << 3) | (256 if missing_member else 0))
     if rc != 0:
         raise RuntimeError(f"emu_solve failed: {rc}")
     return time, dict(zip(STAT_NAMES, stats.tolist()))

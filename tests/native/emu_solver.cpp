@@ -75,7 +75,8 @@ using SharedGlobal = fdsweep::SharedT<fdsweep::kArenaWords, fdsweep::kGeoLdsDoub
 // it (the leader), the others join its wavefront passes as the device's team
 // members do (team 1: a lone workgroup).
 template <class SH>
-static void run_threads(int nthreads, int team, const fdsweep::TableJob &J, const fdsweep::Scratch &W) {
+static void run_threads(int nthreads, int team, const fdsweep::TableJob &J, const fdsweep::Scratch &W,
+                        bool missing = false) {
     std::vector<std::unique_ptr<SH>> shs;
     std::vector<std::unique_ptr<std::barrier<>>> bars;
     for (int g = 0; g < team; g++) {
@@ -85,8 +86,10 @@ static void run_threads(int nthreads, int team, const fdsweep::TableJob &J, cons
     }
     std::unique_ptr<fdsweep::TeamCtl> tm(new fdsweep::TeamCtl());
     std::memset(tm.get(), 0, sizeof(fdsweep::TeamCtl));
+    // a missing member: the others give the pass up after 20 ms
+    tm->wait_ticks = missing ? 2000000 : 0;
     std::vector<std::thread> th;
-    for (int g = 0; g < team; g++)
+    for (int g = 0; g < team - (missing ? 1 : 0); g++)
         for (int t = 0; t < nthreads; t++)
             th.emplace_back([&, g, t]() {
                 HostExec x{t, nthreads, bars[(size_t)g].get()};
@@ -106,7 +109,8 @@ static void run_threads(int nthreads, int team, const fdsweep::TableJob &J, cons
 // mode: bit0 per-node schedule state always in global scratch (else in the
 // arena where a face fits, as the device builds); bit1 the first pass as a
 // wavefront, bit2 reverse passes as wavefronts; bits 3-7 the number of
-// workgroups sharing the wavefront passes (0 or 1: one).
+// workgroups sharing the wavefront passes (0 or 1: one); bit 8 the team's last
+// member never starts (its failure path).
 extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const float *extra, int nthreads,
                          int *stats, int change_budget, int mode) {
     const int global_state = mode & 1;
@@ -174,7 +178,8 @@ extern "C" int emu_solve(const EmuPar *pc, const float *slow, float *time, const
     std::vector<double> geo((size_t)fdsweep::geo_doubles(p.ny, p.nz));
     J.geo_fill = geo.data();
     const int team = std::max(1, std::min(fdsweep::kTeamMax, (mode >> 3) & 31));
-    if (global_state) run_threads<SharedGlobal>(nthreads, team, J, W);
-    else run_threads<fdsweep::Shared>(nthreads, team, J, W);
+    const bool missing = team > 1 && ((mode >> 8) & 1);
+    if (global_state) run_threads<SharedGlobal>(nthreads, team, J, W, missing);
+    else run_threads<fdsweep::Shared>(nthreads, team, J, W, missing);
     return 0;
 }

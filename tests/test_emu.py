@@ -144,3 +144,19 @@ def test_team_carried_registers_and_counts(emu):
             assert e1[k] == e3[k], k
         tot["wave_carry"] = tot.get("wave_carry", 0) + e3["wave_carry"]
     assert tot["wave_carry"] > 0
+
+
+@pytest.mark.parametrize("team", [2, 4])
+def test_team_member_that_never_starts(emu, team):
+    """A team whose last member never starts: the others give the pass up
+    after the team's wait limit (20 ms here), leave their barriers, and the
+    leader reports the failed wavefront (error bit 7, reason 7) instead of
+    waiting for ever; the host would then solve the table face by face."""
+    import time as _time
+    par, v = make_case(901)
+    s = ob.slowness(v)
+    t0 = _time.monotonic()
+    _, est = emu.solve(par, s, 2, wave=3, team=team, missing_member=True)
+    assert _time.monotonic() - t0 < 60
+    assert est["error"] & 128
+    assert est["wave_fail"] == 7

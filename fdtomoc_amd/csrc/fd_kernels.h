@@ -38,7 +38,7 @@ extern "C" hipError_t fd_launch_one(const fdsweep::TableJob *J, const fdsweep::S
 
 // The persistent sweep over njobs tables on nslots workgroups; table i <
 // nteam is solved by a team of d_team_first[i + 1] - d_team_first[i]
-// workgroups (nteam 0: every table by one).
+// workgroups (nteam 0: every table by one; the narrow build has no teams).
 extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jobs, int njobs,
                                       const fdsweep::Scratch *d_slots, int nslots, int *d_queue,
                                       fdsweep::TeamCtl *d_teams, const int *d_team_first, int nteam,

@@ -488,7 +488,7 @@ int run_tables(fdgpu_ctx *c, const std::vector<Table *> &tabs, int mode, bool no
     // tables are fewer, else a size-th of the workgroups) -- for tables whose
     // passes can run as wavefronts
     std::vector<int> teamK;
-    if (mode == FDGPU_BATCHED && !no_wave && c->team != 1) {
+    if (mode == FDGPU_BATCHED && !no_wave && c->team != 1 && variant == FD_WIDE) {
         const int arena = variant == FD_NARROW ? kNarrowArenaWords : fdsweep::kArenaWords;
         int lead = 0;   // leading tables that can run as a team
         std::vector<long long> sizes((size_t)n);

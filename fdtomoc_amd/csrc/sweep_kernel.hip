@@ -62,13 +62,27 @@ struct DevExec {
 // team_first[i + 1] - 1 (its team: the first is the leader), every later
 // table one ticket.  A member waits only for members that have taken their
 // tickets already, so a team forms whenever its workgroups can run at once.
-template <int THREADS, int ARENA, int GEOL>
+//
+// TEAMS false (the narrow build, used when tables outnumber the CUs and teams
+// never form): the team code is not compiled in.
+template <int THREADS, int ARENA, int GEOL, bool TEAMS>
 __global__ void __launch_bounds__(THREADS, 2)
     sphfd_sweep_kernel(const fdsweep::TableJob *jobs, int njobs, const fdsweep::Scratch *slots, int *queue,
                        fdsweep::TeamCtl *teams, const int *team_first, int nteam) {
     __shared__ fdsweep::SharedT<ARENA, GEOL> sh;
     DevExec x;
     const fdsweep::Scratch W = slots[blockIdx.x];
+    if (!TEAMS) {
+        while (true) {
+            if (threadIdx.x == 0) sh.job = atomicAdd(queue, 1);
+            __syncthreads();
+            const int j = sh.job;
+            __syncthreads();
+            if (j >= njobs) break;
+            fdsweep::solve_table(x, sh, jobs[j], W);
+        }
+        return;
+    }
     const int tt = nteam > 0 ? team_first[nteam] : 0;
     while (true) {
         if (threadIdx.x == 0) sh.job = atomicAdd(queue, 1);
@@ -89,13 +103,15 @@ __global__ void __launch_bounds__(THREADS, 2)
             j = nteam + (t - tt);
         }
         if (j >= njobs) break;
-        if (K > 1 && member > 0) {
-            fdsweep::team_helper(x, sh, jobs[j], teams + j, K, member);
-        } else if (K > 1) {
-            fdsweep::solve_table(x, sh, jobs[j], W, teams + j, K);
-            fdsweep::team_release(x, teams + j);
-        } else {
-            fdsweep::solve_table(x, sh, jobs[j], W);
+        if constexpr (TEAMS) {
+            if (K > 1 && member > 0) {
+                fdsweep::team_helper(x, sh, jobs[j], teams + j, K, member);
+            } else if (K > 1) {
+                fdsweep::solve_table(x, sh, jobs[j], W, teams + j, K);
+                fdsweep::team_release(x, teams + j);
+            } else {
+                fdsweep::solve_table(x, sh, jobs[j], W);
+            }
         }
     }
 }
@@ -254,10 +270,10 @@ extern "C" hipError_t fd_launch_sweep(int variant, const fdsweep::TableJob *d_jo
                                       hipStream_t stream) {
     if (nteam <= 0 || !d_teams || !d_team_first) nteam = 0;
     if (variant == FD_NARROW)
-        hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds>), dim3(nslots), dim3(kNarrowThreads),
-                           0, stream, d_jobs, njobs, d_slots, d_queue, d_teams, d_team_first, nteam);
+        hipLaunchKernelGGL((sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds, false>), dim3(nslots),
+                           dim3(kNarrowThreads), 0, stream, d_jobs, njobs, d_slots, d_queue, d_teams, d_team_first, 0);
     else
-        hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>), dim3(nslots), dim3(kWideThreads), 0,
+        hipLaunchKernelGGL((sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles, true>), dim3(nslots), dim3(kWideThreads), 0,
                            stream, d_jobs, njobs, d_slots, d_queue, d_teams, d_team_first, nteam);
     return hipGetLastError();
 }
@@ -278,8 +294,9 @@ extern "C" hipError_t fd_launch_serve(fdsweep::ServeCtl *ctl, const fdsweep::Ser
 }
 
 extern "C" const void *fd_sweep_kernel_ptr(int variant) {
-    return variant == FD_NARROW ? (const void *)sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds>
-                                : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles>;
+    return variant == FD_NARROW
+               ? (const void *)sphfd_sweep_kernel<kNarrowThreads, kNarrowArenaWords, kNarrowGeoLds, false>
+               : (const void *)sphfd_sweep_kernel<kWideThreads, fdsweep::kArenaWords, fdsweep::kGeoLdsDoubles, true>;
 }
 
 extern "C" hipError_t fd_launch_reset(int *d_queue, fdsweep::TeamCtl *d_teams, int nteam, hipStream_t stream) {

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <map>
+#include <unordered_map>
 #include <numeric>
 #include <tuple>
 #include <vector>
@@ -56,41 +57,60 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
     if (hipSetDevice(c->device) != hipSuccess) return FDGPU_EHIP;
 
     // tables: each distinct one once, of the grid's dimensions
-    std::map<int, int> slot;
+    std::unordered_map<int, int> slot;
     std::vector<const float *> tabs;
+    std::vector<int> ray_slot((size_t)n);
     for (int r = 0; r < n; r++) {
-        if (slot.count(rays[r].table)) continue;
-        auto it = c->tables.find(rays[r].table);
-        if (it == c->tables.end() || !it->second.d_time) return FDGPU_EINPUT;
-        const fdgpu_par &tp = it->second.par;
-        if (tp.nx != G.nx || tp.ny != G.ny || tp.nz != G.nz) return FDGPU_EINPUT;
-        slot[rays[r].table] = (int)tabs.size();
-        tabs.push_back(it->second.d_time);
+        auto sl = slot.find(rays[r].table);
+        if (sl == slot.end()) {
+            auto it = c->tables.find(rays[r].table);
+            if (it == c->tables.end() || !it->second.d_time) return FDGPU_EINPUT;
+            const fdgpu_par &tp = it->second.par;
+            if (tp.nx != G.nx || tp.ny != G.ny || tp.nz != G.nz) return FDGPU_EINPUT;
+            sl = slot.emplace(rays[r].table, (int)tabs.size()).first;
+            tabs.push_back(it->second.d_time);
+        }
+        ray_slot[(size_t)r] = sl->second;
     }
     // rays grouped by table (a wave's lanes then read one table), and within
     // a table by the event-station distance in cells, longest first (a wave
-    // runs as long as its longest ray, so its lanes should walk alike)
-    std::vector<int> order((size_t)n);
-    std::vector<double> reach((size_t)n);
+    // runs as long as its longest ray, so its lanes should walk alike): one
+    // integer key per ray (table slot, then the distance's float bits
+    // reversed), ties in input order
+    std::vector<std::pair<unsigned long long, int>> keyed((size_t)n);
     for (int r = 0; r < n; r++) {
         const double di = (rays[r].ex - rays[r].xs) / G.df, dj = (rays[r].ey - rays[r].ys) / G.dq,
                      dk = (rays[r].ez - rays[r].zs) / G.h;
-        reach[(size_t)r] = fabs(di) + fabs(dj) + fabs(dk);   // cells a walk crosses, about
+        const float reach = (float)(fabs(di) + fabs(dj) + fabs(dk));   // cells a walk crosses, about
+        unsigned bits;
+        memcpy(&bits, &reach, sizeof bits);   // non-negative: the bits order as the values
+        keyed[(size_t)r] = {((unsigned long long)ray_slot[(size_t)r] << 32) | (0xffffffffu - bits), r};
     }
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        if (rays[a].table != rays[b].table) return rays[a].table < rays[b].table;
-        return reach[(size_t)a] > reach[(size_t)b];
-    });
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int> order((size_t)n);
+    for (int r = 0; r < n; r++) order[(size_t)r] = keyed[(size_t)r].second;
     // each station's Cartesian point once (a batch names few stations)
-    std::map<std::tuple<double, double, double>, std::tuple<float, float, float>> points;
+    struct PointKey {
+        double x, y, z;
+        bool operator==(const PointKey &o) const { return x == o.x && y == o.y && z == o.z; }
+    };
+    struct PointHash {
+        size_t operator()(const PointKey &k) const {
+            unsigned long long a, b, d;
+            memcpy(&a, &k.x, 8);
+            memcpy(&b, &k.y, 8);
+            memcpy(&d, &k.z, 8);
+            return (size_t)(a * 0x9e3779b97f4a7c15ull ^ (b + 0x632be59bd9b4e019ull) * 0xc2b2ae3d27d4eb4full ^ d);
+        }
+    };
+    std::unordered_map<PointKey, std::tuple<float, float, float>, PointHash> points;
     std::vector<fdray::RayIn> in((size_t)n);
     for (int r = 0; r < n; r++) {
         const fdgpu_ray &R = rays[r];
         fdray::RayIn &q = in[(size_t)r];
         q.ex = R.ex; q.ey = R.ey; q.ez = R.ez;
         q.xs = R.xs; q.ys = R.ys; q.zs = R.zs;
-        const auto key = std::make_tuple(R.xs, R.ys, R.zs);
+        const PointKey key{R.xs, R.ys, R.zs};
         auto pt = points.find(key);
         if (pt == points.end()) {
             float x2, y2, z2;
@@ -98,7 +118,7 @@ int fdgpu_rays_trace(fdgpu_ctx *c, const fdgpu_ray_grid *gp, const fdgpu_ray *ra
             pt = points.emplace(key, std::make_tuple(x2, y2, z2)).first;
         }
         std::tie(q.x2, q.y2, q.z2) = pt->second;
-        q.table = slot[R.table];
+        q.table = ray_slot[(size_t)r];
     }
     const std::vector<double> trig = fdray::face_trig(G.nx, G.ny, G.x0, G.y0, G.df, G.dq);
     std::vector<int> guess = fdray::coarse_guess(G.gx, G.nxc, G.x0, G.df, G.nx);
